@@ -1,0 +1,122 @@
+/*
+ * VariantDBSCAN on AMD Instinct MI355X (gfx950): C ABI of libSharedVDBSCAN.so.
+ *
+ * The first entry point is the reference library's own symbol, unchanged, so
+ * existing callers (ctypes wrappers, dlopen harnesses) load this library in
+ * its place. The vdb_* entry points expose the same engine with device
+ * pointers, statistics and the neighbour table.
+ *
+ * All entry points are synchronous, return 0 on success and a nonzero
+ * VDB_E* code on failure (vdb_last_error() describes it), and serialise on
+ * an internal per-device mutex, so concurrent callers are safe.
+ */
+#ifndef VDBSCAN_H
+#define VDBSCAN_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VDB_OK 0
+#define VDB_EINVAL 1       /* bad argument */
+#define VDB_ENONFINITE 2   /* NaN or infinite coordinate */
+#define VDB_EDEVICE 3      /* HIP runtime failure */
+#define VDB_ENOMEM 4       /* allocation failure */
+#define VDB_EUNSUPPORTED 5 /* request outside what this build handles */
+
+/*
+ * Replaces libVDBSCAN of the reference (src/main.cpp:50; prototype also in
+ * src/c_test_prog.h:26; bound by src/variantdbscan.py:66-69).
+ * retArr[v*datasetSize + i] = cluster of point i under variant v, 0 = noise,
+ * clusters numbered 1.. per variant. Every variant is clustered as the
+ * reference clusters a variant from scratch (bit-identical to a one-variant
+ * reference call). MBBsize only shaped the reference's R-tree; it must be > 0
+ * and is otherwise ignored. verbose prints progress to stdout.
+ * Extension: returns nonzero on invalid input (the reference always returns 0).
+ */
+int libVDBSCAN(double *inputx, double *inputy, unsigned int datasetSize,
+               double *inputEpsilon, unsigned int *inputMinpts,
+               unsigned int numVariants, int MBBsize, unsigned int *retArr,
+               bool verbose);
+
+/* ---------------------------------------------------------------------- */
+/* Extended API (no counterpart symbol in the reference).                  */
+
+/* flags */
+#define VDB_DEVICE_PTRS 1u /* point / output arrays are device pointers */
+
+/* path selection for vdb_cluster */
+#define VDB_PATH_AUTO 0    /* batched engine when applicable, else fused */
+#define VDB_PATH_FUSED 1   /* per variant: count, union-find, border; no table */
+#define VDB_PATH_TABLE 2   /* per variant: count/scan/fill CSR table, union-find over it */
+#define VDB_PATH_BATCH 3   /* all variants from one distance-sorted truncated table */
+
+#define VDB_PREC_FP64 0      /* every distance test in binary64 */
+#define VDB_PREC_FP32_RECHECK 1 /* float32 screening, binary64 inside the error band */
+
+typedef struct vdb_options {
+    int path;                   /* VDB_PATH_* */
+    int precision;              /* VDB_PREC_* */
+    uint64_t table_budget_bytes;/* 0 = most of the free device memory */
+    int verbose;
+} vdb_options;
+
+typedef struct vdb_stats {
+    double ms_total;       /* device time of the whole call (events) */
+    double ms_prepare;     /* binning + grids */
+    double ms_count;       /* neighbour counting / truncated table */
+    double ms_scan;
+    double ms_fill;
+    double ms_uf;          /* union-find */
+    double ms_border;
+    double ms_labels;
+    uint64_t pairs;        /* eps-neighbour pairs (table path, last variant) */
+    uint64_t rechecked;    /* pairs decided by the binary64 recheck */
+    uint64_t candidates;   /* distance tests (when measured) */
+    uint32_t tiles;        /* table tiles */
+    int path_used;
+} vdb_stats;
+
+const char *vdb_last_error(void);
+int vdb_version(void);
+
+/* Launch all work on this hipStream_t (NULL: the library's own stream). */
+int vdb_set_stream(void *hip_stream);
+/* Free the pooled device memory of the current device. */
+int vdb_release(void);
+
+/* Cluster n points under nvar variants. labels has nvar*n entries laid out
+ * as retArr of libVDBSCAN. flags & VDB_DEVICE_PTRS: x, y, labels are device
+ * pointers (eps and minpts are always host arrays). opt/stats may be NULL. */
+int vdb_cluster(const double *x, const double *y, uint32_t n, const double *eps,
+                const uint32_t *minpts, uint32_t nvar, uint32_t *labels,
+                uint32_t flags, const vdb_options *opt, vdb_stats *stats);
+
+/* Reference visit order: mapping[k] = caller index of the point at binned
+ * position k (src/main.cpp:459-605, 300x300 bins). */
+int vdb_rank_order(const double *x, const double *y, uint32_t n, uint32_t *mapping,
+                   uint32_t flags);
+
+/* counts[i] = number of points within eps of point i, itself included
+ * (the neighbour-set size of src/DBScan.cpp:547). */
+int vdb_neighbor_counts(const double *x, const double *y, uint32_t n, double eps,
+                        uint32_t *counts, uint32_t flags, int precision,
+                        uint64_t *rechecked);
+
+/* eps-neighbour table in CSR form, caller order: offsets[n+1], ids[total],
+ * optional d2[total] (float32 squared distances). Ids inside a row are in
+ * grid order, not sorted. Replaces the GPU prototype's table construction
+ * (src/gpu/GPU.cu:3479-4557 + :4598-4742). If *total exceeds capacity,
+ * nothing beyond offsets is written and the call returns 0 with *total set:
+ * call again with buffers of that size. */
+int vdb_neighbor_table(const double *x, const double *y, uint32_t n, double eps,
+                       uint64_t *offsets, uint32_t *ids, float *d2, uint64_t capacity,
+                       uint64_t *total, uint32_t flags, int precision, vdb_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,178 @@
+"""ctypes bindings of libSharedVDBSCAN.so (declared in include/vdbscan.h).
+
+The library is the HIP build in ``variantdbscan_amd/lib``. There is no CPU
+fallback: if the library is missing or a call fails, an exception is raised.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+LIB_NAME = "libSharedVDBSCAN.so"
+LIB_PATH = os.path.join(LIB_DIR, LIB_NAME)
+
+DEVICE_PTRS = 1
+
+PATH_AUTO, PATH_FUSED, PATH_TABLE, PATH_BATCH = 0, 1, 2, 3
+PREC_FP64, PREC_FP32_RECHECK = 0, 1
+
+EXPORTED = ("libVDBSCAN", "vdb_last_error", "vdb_version", "vdb_set_stream", "vdb_release",
+            "vdb_cluster", "vdb_rank_order", "vdb_neighbor_counts", "vdb_neighbor_table")
+
+
+class VdbOptions(ctypes.Structure):
+    _fields_ = [("path", ctypes.c_int), ("precision", ctypes.c_int),
+                ("table_budget_bytes", ctypes.c_uint64), ("verbose", ctypes.c_int)]
+
+
+class VdbStats(ctypes.Structure):
+    _fields_ = [("ms_total", ctypes.c_double), ("ms_prepare", ctypes.c_double),
+                ("ms_count", ctypes.c_double), ("ms_scan", ctypes.c_double),
+                ("ms_fill", ctypes.c_double), ("ms_uf", ctypes.c_double),
+                ("ms_border", ctypes.c_double), ("ms_labels", ctypes.c_double),
+                ("pairs", ctypes.c_uint64), ("rechecked", ctypes.c_uint64),
+                ("candidates", ctypes.c_uint64), ("tiles", ctypes.c_uint32),
+                ("path_used", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class VdbError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load (once) the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("%s not found: build it with __graft_entry__.build() or "
+                          "make -C variantdbscan_amd/csrc" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    L.libVDBSCAN.argtypes = [vp, vp, ctypes.c_uint, vp, vp, ctypes.c_uint, ctypes.c_int, vp,
+                             ctypes.c_bool]
+    L.libVDBSCAN.restype = ctypes.c_int
+    L.vdb_last_error.argtypes = []
+    L.vdb_last_error.restype = ctypes.c_char_p
+    L.vdb_version.restype = ctypes.c_int
+    L.vdb_set_stream.argtypes = [vp]
+    L.vdb_set_stream.restype = ctypes.c_int
+    L.vdb_release.restype = ctypes.c_int
+    L.vdb_cluster.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, ctypes.c_uint32, vp,
+                              ctypes.c_uint32, ctypes.POINTER(VdbOptions),
+                              ctypes.POINTER(VdbStats)]
+    L.vdb_cluster.restype = ctypes.c_int
+    L.vdb_rank_order.argtypes = [vp, vp, ctypes.c_uint32, vp, ctypes.c_uint32]
+    L.vdb_rank_order.restype = ctypes.c_int
+    L.vdb_neighbor_counts.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_double, vp,
+                                      ctypes.c_uint32, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_uint64)]
+    L.vdb_neighbor_counts.restype = ctypes.c_int
+    L.vdb_neighbor_table.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_double, vp, vp, vp,
+                                     ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
+                                     ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(VdbStats)]
+    L.vdb_neighbor_table.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        msg = load().vdb_last_error()
+        raise VdbError("vdbscan error %d: %s" % (rc, msg.decode() if msg else ""))
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def cluster(x, y, eps, minpts, path=PATH_AUTO, precision=PREC_FP64, verbose=False,
+            table_budget_bytes=0, return_stats=False):
+    """Labels (V, n) uint32 for host arrays; retArr layout of libVDBSCAN."""
+    L = load()
+    x, y = _f64(x), _f64(y)
+    eps = _f64(np.atleast_1d(eps))
+    mp = _u32(np.atleast_1d(minpts))
+    if len(x) != len(y) or len(eps) != len(mp):
+        raise ValueError("length mismatch")
+    n, v = len(x), len(eps)
+    out = np.zeros(n * v, dtype=np.uint32)
+    opt = VdbOptions(path, precision, table_budget_bytes, 1 if verbose else 0)
+    st = VdbStats()
+    _check(L.vdb_cluster(_ptr(x), _ptr(y), n, _ptr(eps), _ptr(mp), v, _ptr(out), 0,
+                         ctypes.byref(opt), ctypes.byref(st)))
+    lab = out.reshape(v, n)
+    return (lab, st.as_dict()) if return_stats else lab
+
+
+def cluster_device(x_ptr, y_ptr, n, eps, minpts, labels_ptr, path=PATH_AUTO,
+                   precision=PREC_FP64, stats=True):
+    """Device-resident inputs/outputs (raw device pointers, e.g. tensor.data_ptr())."""
+    L = load()
+    eps = _f64(np.atleast_1d(eps))
+    mp = _u32(np.atleast_1d(minpts))
+    opt = VdbOptions(path, precision, 0, 0)
+    st = VdbStats()
+    _check(L.vdb_cluster(ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), n, _ptr(eps),
+                         _ptr(mp), len(eps), ctypes.c_void_p(labels_ptr), DEVICE_PTRS,
+                         ctypes.byref(opt), ctypes.byref(st) if stats else None))
+    return st.as_dict()
+
+
+def rank_order(x, y):
+    L = load()
+    x, y = _f64(x), _f64(y)
+    m = np.zeros(len(x), dtype=np.uint32)
+    _check(L.vdb_rank_order(_ptr(x), _ptr(y), len(x), _ptr(m), 0))
+    return m
+
+
+def neighbor_counts(x, y, eps, precision=PREC_FP64, return_rechecked=False):
+    L = load()
+    x, y = _f64(x), _f64(y)
+    c = np.zeros(len(x), dtype=np.uint32)
+    rc = ctypes.c_uint64(0)
+    _check(L.vdb_neighbor_counts(_ptr(x), _ptr(y), len(x), float(eps), _ptr(c), 0,
+                                 precision, ctypes.byref(rc)))
+    return (c, rc.value) if return_rechecked else c
+
+
+def neighbor_table(x, y, eps, with_d2=False, precision=PREC_FP64):
+    """(offsets uint64[n+1], ids uint32[P], d2 float32[P] or None), caller order."""
+    L = load()
+    x, y = _f64(x), _f64(y)
+    n = len(x)
+    off = np.zeros(n + 1, dtype=np.uint64)
+    total = ctypes.c_uint64(0)
+    st = VdbStats()
+    _check(L.vdb_neighbor_table(_ptr(x), _ptr(y), n, float(eps), _ptr(off), None, None, 0,
+                                ctypes.byref(total), 0, precision, ctypes.byref(st)))
+    cap = max(int(total.value), 1)
+    ids = np.zeros(cap, dtype=np.uint32)
+    d2 = np.zeros(cap, dtype=np.float32) if with_d2 else None
+    _check(L.vdb_neighbor_table(_ptr(x), _ptr(y), n, float(eps), _ptr(off), _ptr(ids),
+                                _ptr(d2) if with_d2 else None, cap, ctypes.byref(total), 0,
+                                precision, ctypes.byref(st)))
+    t = int(total.value)
+    return off, ids[:t], (d2[:t] if with_d2 else None)
+
+
+def last_error():
+    msg = load().vdb_last_error()
+    return msg.decode() if msg else ""

@@ -1,0 +1,381 @@
+// C ABI of libSharedVDBSCAN.so (declared in include/vdbscan.h).
+#include "vdbscan.h"
+
+#include "vdb_batch.h"
+#include "vdb_cluster.h"
+#include "vdb_prims.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <vector>
+
+using namespace vdb;
+
+namespace {
+
+// Grid side for a window of radius 1: any pair accepted at eps has
+// |dx| <= eps*(1+2^-50) (s <= T(eps) bounds the rounded difference), and the
+// computed cell coordinate differs from the exact one by < 2^-31 for grids
+// below 2^20 cells per axis, so cells of side eps*(1+2^-24) keep every
+// accepted pair within one cell per axis.
+double plain_cell_side(double eps) { return eps * (1.0 + std::ldexp(1.0, -24)); }
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    hipStream_t s;
+    explicit Timer(hipStream_t st) : s(st)
+    {
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        hipEventRecord(a, s);
+    }
+    ~Timer()
+    {
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+    }
+    double ms()
+    {
+        hipEventRecord(b, s);
+        hipEventSynchronize(b);
+        float m = 0;
+        hipEventElapsedTime(&m, a, b);
+        return m;
+    }
+};
+
+int upload_points(Context &c, const double *x, const double *y, uint32_t n, uint32_t flags,
+                  Points &P)
+{
+    P.n = n;
+    if (flags & VDB_DEVICE_PTRS) {
+        P.x = x;
+        P.y = y;
+        return kOk;
+    }
+    double *dx, *dy;
+    VDB_TRY(c.get("in.x", n, &dx));
+    VDB_TRY(c.get("in.y", n, &dy));
+    VDB_HIP(hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, c.stream()));
+    VDB_HIP(hipMemcpyAsync(dy, y, sizeof(double) * n, hipMemcpyHostToDevice, c.stream()));
+    P.x = dx;
+    P.y = dy;
+    return kOk;
+}
+
+size_t default_budget(const vdb_options *opt)
+{
+    if (opt && opt->table_budget_bytes) return (size_t)opt->table_budget_bytes;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return (size_t)1 << 32;
+    size_t keep = (size_t)8 << 30;
+    return fr > 2 * keep ? fr - keep : fr / 2;
+}
+
+// per-variant paths: one grid per distinct eps, counts shared by its variants
+int run_per_variant(Context &c, Points &P, const double *eps, const uint32_t *minpts,
+                    uint32_t nvar, uint32_t *labels_dev, int mode, int precision,
+                    size_t budget, vdb_stats *st, bool verbose)
+{
+    std::map<double, std::vector<uint32_t>> by_eps;
+    for (uint32_t v = 0; v < nvar; v++) by_eps[eps[v]].push_back(v);
+    uint32_t *counts;
+    VDB_TRY(c.get("pv.counts", P.n, &counts));
+    unsigned long long *rc;
+    VDB_TRY(c.get("pv.recheck", 1, &rc));
+    VDB_HIP(hipMemsetAsync(rc, 0, sizeof(unsigned long long), c.stream()));
+    PhaseTimes pt;
+    for (auto &kv : by_eps) {
+        double e = kv.first;
+        Grid G;
+        Items I;
+        bool enl = false;
+        Timer t0(c.stream());
+        VDB_TRY(build_grid(c, P, plain_cell_side(e), 1, "g1", G, &enl));
+        VDB_TRY(build_items(c, G, "g1", I));
+        if (st) st->ms_prepare += t0.ms();
+        Timer t1(c.stream());
+        VDB_TRY(count_neighbors(c, G, I, e, precision, counts, (uint64_t *)rc));
+        if (st) st->ms_count += t1.ms();
+        if (verbose)
+            printf("\n[vdbscan] eps=%g grid %ux%u, %u non-empty cells, largest %u points",
+                   e, G.ncx, G.ncy, G.ncell, G.max_cell);
+        for (uint32_t v : kv.second) {
+            VDB_TRY(cluster_variant(c, P, G, I, e, minpts[v], counts, mode, budget,
+                                    labels_dev + (size_t)v * P.n, st ? &pt : nullptr));
+        }
+    }
+    if (st) {
+        st->ms_scan += pt.scan;
+        st->ms_fill += pt.fill;
+        st->ms_uf += pt.uf;
+        st->ms_border += pt.border;
+        st->ms_labels += pt.labels;
+        st->pairs = pt.pairs;
+        st->tiles += pt.tiles;
+        unsigned long long r = 0;
+        VDB_HIP(hipMemcpy(&r, rc, sizeof r, hipMemcpyDeviceToHost));
+        st->rechecked = r;
+    }
+    return kOk;
+}
+
+int cluster_impl(const double *x, const double *y, uint32_t n, const double *eps,
+                 const uint32_t *minpts, uint32_t nvar, uint32_t *labels, uint32_t flags,
+                 const vdb_options *opt, vdb_stats *st)
+{
+    if (st) memset(st, 0, sizeof(*st));
+    if (n == 0 || nvar == 0) return kOk;
+    if (!x || !y || !eps || !minpts || !labels) {
+        set_error("NULL array argument");
+        return kErrInvalid;
+    }
+    if ((uint64_t)n * nvar > 0xffffffffull) {
+        set_error("datasetSize*numVariants exceeds 2^32 labels");
+        return kErrInvalid;
+    }
+    Context *cp = nullptr;
+    VDB_TRY(get_context(&cp));
+    Context &c = *cp;
+    std::lock_guard<std::mutex> lock(c.mu);
+    Timer total(c.stream());
+    int path = opt ? opt->path : VDB_PATH_AUTO;
+    int precision = opt ? opt->precision : VDB_PREC_FP64;
+    bool verbose = opt && opt->verbose;
+    Points P;
+    VDB_TRY(upload_points(c, x, y, n, flags, P));
+    Timer tprep(c.stream());
+    VDB_TRY(build_rank_order(c, P, "pts"));
+    if (st) st->ms_prepare += tprep.ms();
+    uint32_t *out = labels;
+    if (!(flags & VDB_DEVICE_PTRS)) VDB_TRY(c.get("out.labels", (size_t)n * nvar, &out));
+    int used = path;
+    if (path == VDB_PATH_AUTO || path == VDB_PATH_BATCH) {
+        BatchResult br;
+        int s = cluster_batch(c, P, eps, minpts, nvar, precision, out, st, verbose, &br);
+        if (s == kOk) {
+            used = VDB_PATH_BATCH;
+        } else if (s == kErrUnsupported && path == VDB_PATH_AUTO) {
+            used = VDB_PATH_FUSED;
+        } else {
+            return s;
+        }
+    }
+    if (used == VDB_PATH_FUSED || used == VDB_PATH_TABLE) {
+        VDB_TRY(run_per_variant(c, P, eps, minpts, nvar, out, used == VDB_PATH_TABLE ? 1 : 0,
+                                precision, default_budget(opt), st, verbose));
+    }
+    if (!(flags & VDB_DEVICE_PTRS)) {
+        VDB_HIP(hipMemcpyAsync(labels, out, sizeof(uint32_t) * (size_t)n * nvar,
+                               hipMemcpyDeviceToHost, c.stream()));
+    }
+    VDB_HIP(hipStreamSynchronize(c.stream()));
+    if (st) {
+        st->ms_total = total.ms();
+        st->path_used = used;
+    }
+    return kOk;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *vdb_last_error(void) { return last_error(); }
+
+int vdb_version(void) { return 10000; }
+
+int vdb_set_stream(void *s)
+{
+    Context *c = nullptr;
+    VDB_TRY(get_context(&c));
+    std::lock_guard<std::mutex> lock(c->mu);
+    c->set_stream((hipStream_t)s);
+    return kOk;
+}
+
+int vdb_release(void)
+{
+    Context *c = nullptr;
+    VDB_TRY(get_context(&c));
+    std::lock_guard<std::mutex> lock(c->mu);
+    c->release_all();
+    return kOk;
+}
+
+int vdb_cluster(const double *x, const double *y, uint32_t n, const double *eps,
+                const uint32_t *minpts, uint32_t nvar, uint32_t *labels, uint32_t flags,
+                const vdb_options *opt, vdb_stats *stats)
+{
+    return cluster_impl(x, y, n, eps, minpts, nvar, labels, flags, opt, stats);
+}
+
+int vdb_rank_order(const double *x, const double *y, uint32_t n, uint32_t *mapping,
+                   uint32_t flags)
+{
+    if (n == 0) return kOk;
+    if (!x || !y || !mapping) {
+        set_error("NULL array argument");
+        return kErrInvalid;
+    }
+    Context *cp = nullptr;
+    VDB_TRY(get_context(&cp));
+    Context &c = *cp;
+    std::lock_guard<std::mutex> lock(c.mu);
+    Points P;
+    VDB_TRY(upload_points(c, x, y, n, flags, P));
+    VDB_TRY(build_rank_order(c, P, "pts"));
+    VDB_HIP(hipMemcpyAsync(mapping, P.mapping, sizeof(uint32_t) * n,
+                           (flags & VDB_DEVICE_PTRS) ? hipMemcpyDeviceToDevice
+                                                     : hipMemcpyDeviceToHost,
+                           c.stream()));
+    VDB_HIP(hipStreamSynchronize(c.stream()));
+    return kOk;
+}
+
+int vdb_neighbor_counts(const double *x, const double *y, uint32_t n, double eps,
+                        uint32_t *counts, uint32_t flags, int precision, uint64_t *rechecked)
+{
+    if (n == 0) return kOk;
+    if (!x || !y || !counts) {
+        set_error("NULL array argument");
+        return kErrInvalid;
+    }
+    Context *cp = nullptr;
+    VDB_TRY(get_context(&cp));
+    Context &c = *cp;
+    std::lock_guard<std::mutex> lock(c.mu);
+    Points P;
+    VDB_TRY(upload_points(c, x, y, n, flags, P));
+    VDB_TRY(build_rank_order(c, P, "pts"));
+    Grid G;
+    Items I;
+    bool enl;
+    VDB_TRY(build_grid(c, P, plain_cell_side(eps), 1, "g1", G, &enl));
+    VDB_TRY(build_items(c, G, "g1", I));
+    uint32_t *cpos, *ccal;
+    unsigned long long *rc;
+    VDB_TRY(c.get("nc.pos", n, &cpos));
+    VDB_TRY(c.get("nc.caller", n, &ccal));
+    VDB_TRY(c.get("nc.recheck", 1, &rc));
+    VDB_HIP(hipMemsetAsync(rc, 0, sizeof(unsigned long long), c.stream()));
+    VDB_TRY(count_neighbors(c, G, I, eps, precision, cpos, (uint64_t *)rc));
+    uint32_t *dst = (flags & VDB_DEVICE_PTRS) ? counts : ccal;
+    VDB_TRY(pos_to_caller(c, P, G, cpos, dst));
+    if (!(flags & VDB_DEVICE_PTRS))
+        VDB_HIP(hipMemcpyAsync(counts, ccal, sizeof(uint32_t) * n, hipMemcpyDeviceToHost,
+                               c.stream()));
+    VDB_HIP(hipStreamSynchronize(c.stream()));
+    if (rechecked) {
+        unsigned long long r = 0;
+        VDB_HIP(hipMemcpy(&r, rc, sizeof r, hipMemcpyDeviceToHost));
+        *rechecked = r;
+    }
+    return kOk;
+}
+
+int vdb_neighbor_table(const double *x, const double *y, uint32_t n, double eps,
+                       uint64_t *offsets, uint32_t *ids, float *d2, uint64_t capacity,
+                       uint64_t *total, uint32_t flags, int precision, vdb_stats *st)
+{
+    if (st) memset(st, 0, sizeof(*st));
+    if (!total) {
+        set_error("NULL total");
+        return kErrInvalid;
+    }
+    *total = 0;
+    if (n == 0) {
+        if (offsets) offsets[0] = 0;
+        return kOk;
+    }
+    if (!x || !y || !offsets) {
+        set_error("NULL array argument");
+        return kErrInvalid;
+    }
+    Context *cp = nullptr;
+    VDB_TRY(get_context(&cp));
+    Context &c = *cp;
+    std::lock_guard<std::mutex> lock(c.mu);
+    Timer ttot(c.stream());
+    Points P;
+    VDB_TRY(upload_points(c, x, y, n, flags, P));
+    Timer tprep(c.stream());
+    VDB_TRY(build_rank_order(c, P, "pts"));
+    Grid G;
+    Items I;
+    bool enl;
+    VDB_TRY(build_grid(c, P, plain_cell_side(eps), 1, "g1", G, &enl));
+    VDB_TRY(build_items(c, G, "g1", I));
+    if (st) st->ms_prepare = tprep.ms();
+    bool dev = flags & VDB_DEVICE_PTRS;
+    uint64_t *doff = offsets;
+    uint32_t *dids = ids;
+    float *dd2 = d2;
+    if (!dev) {
+        VDB_TRY(c.get("nt.off", (size_t)n + 1, &doff));
+        // host output: stage through the pool, sized by the capacity given
+        if (capacity && ids) VDB_TRY(c.get("nt.ids", capacity, &dids));
+        if (capacity && d2) VDB_TRY(c.get("nt.d2", capacity, &dd2));
+    }
+    PhaseTimes pt;
+    uint64_t tot = 0;
+    VDB_TRY(neighbor_table(c, P, G, I, eps, precision, doff, (ids && capacity) ? dids : nullptr,
+                           (d2 && capacity) ? dd2 : nullptr,
+                           (ids && capacity) ? capacity : 0, &tot, &pt));
+    *total = tot;
+    if (!dev) {
+        VDB_HIP(hipMemcpyAsync(offsets, doff, sizeof(uint64_t) * ((size_t)n + 1),
+                               hipMemcpyDeviceToHost, c.stream()));
+        if (tot <= capacity && ids && tot)
+            VDB_HIP(hipMemcpyAsync(ids, dids, sizeof(uint32_t) * tot, hipMemcpyDeviceToHost,
+                                   c.stream()));
+        if (tot <= capacity && d2 && tot)
+            VDB_HIP(hipMemcpyAsync(d2, dd2, sizeof(float) * tot, hipMemcpyDeviceToHost,
+                                   c.stream()));
+    }
+    VDB_HIP(hipStreamSynchronize(c.stream()));
+    if (st) {
+        st->ms_count = pt.count;
+        st->ms_scan = pt.scan;
+        st->ms_fill = pt.fill;
+        st->pairs = tot;
+        st->rechecked = pt.recheck;
+        st->ms_total = ttot.ms();
+        st->path_used = VDB_PATH_TABLE;
+    }
+    return kOk;
+}
+
+int libVDBSCAN(double *inputx, double *inputy, unsigned int datasetSize, double *inputEpsilon,
+               unsigned int *inputMinpts, unsigned int numVariants, int MBBsize,
+               unsigned int *retArr, bool verbose)
+{
+    if (verbose) printf("Verbose mode on"); // src/main.cpp:52-53
+    if (MBBsize <= 0) {
+        // the reference divides by MBBsize (src/main.cpp:118)
+        set_error("MBBsize must be positive");
+        return kErrInvalid;
+    }
+    vdb_options opt;
+    memset(&opt, 0, sizeof opt);
+    opt.path = VDB_PATH_AUTO;
+    opt.precision = VDB_PREC_FP64;
+    opt.verbose = verbose ? 1 : 0;
+    vdb_stats st;
+    int s = vdb_cluster(inputx, inputy, datasetSize, inputEpsilon, inputMinpts, numVariants,
+                        retArr, 0, &opt, &st);
+    if (verbose) {
+        if (s == 0)
+            printf("\n[vdbscan] %u points x %u variants, path %d, %.3f ms device time\n",
+                   datasetSize, numVariants, st.path_used, st.ms_total);
+        else
+            printf("\n[vdbscan] error %d: %s\n", s, last_error());
+        fflush(stdout);
+    }
+    return s;
+}
+
+} // extern "C"

@@ -1,0 +1,57 @@
+// Per-variant clustering over a grid of cell side >= eps (window radius 1).
+//
+// Labels follow the reference's sequential expansion (src/DBScan.cpp:523-609)
+// in closed form:
+//   * core points (>= minpts eps-neighbours, self included) are grouped into
+//     connected components; each component is rooted at its smallest binned
+//     rank, and clusters are numbered 1.. in ascending root rank;
+//   * a non-core point takes the cluster of the smallest root among its core
+//     neighbours, but only when that root's rank is below its own; otherwise
+//     the outer loop reached it first and it stays noise (0).
+#pragma once
+
+#include "vdb_grid.h"
+
+namespace vdb {
+
+struct PhaseTimes {
+    float count = 0, scan = 0, fill = 0, uf = 0, border = 0, labels = 0;
+    uint64_t pairs = 0;      // eps-neighbour pairs (self pairs included)
+    uint32_t tiles = 0;      // table tiles (table mode)
+    uint64_t recheck = 0;    // pairs resolved by the FP64 recheck (FP32 mode)
+};
+
+// Work items: (cell, first origin) groups of kOrigins origins of one cell.
+struct Items {
+    uint32_t n = 0;
+    uint32_t *cell = nullptr;
+    uint32_t *o0 = nullptr;
+};
+
+constexpr int kOrigins = 8;
+
+int build_items(Context &c, const Grid &G, const char *tag, Items &I);
+
+// counts[p] = |N_eps(p)| for every position p of G (self included).
+// precision 0 = FP64 exact, 1 = FP32 with an FP64 recheck inside the error band.
+int count_neighbors(Context &c, const Grid &G, const Items &I, double eps, int precision,
+                    uint32_t *counts, uint64_t *recheck_dev);
+
+// Full per-variant clustering; labels written in caller order to out (device).
+// mode 0 = fused (no table), 1 = CSR table (count/scan/fill, union-find over
+// the table), tiled when the table exceeds table_budget bytes.
+int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, double eps,
+                    uint32_t minpts, const uint32_t *counts, int mode, size_t table_budget,
+                    uint32_t *out, PhaseTimes *pt);
+
+// Scatter a per-position array to caller order.
+int pos_to_caller(Context &c, const Points &P, const Grid &G, const uint32_t *by_pos,
+                  uint32_t *by_caller);
+
+// Neighbour table for external callers: rows and ids in caller order (ids
+// unsorted inside a row), optional float32 squared distances.
+int neighbor_table(Context &c, const Points &P, const Grid &G, const Items &I, double eps,
+                   int precision, uint64_t *offsets /*n+1, device*/, uint32_t *ids,
+                   float *d2, uint64_t capacity, uint64_t *total, PhaseTimes *pt);
+
+} // namespace vdb

@@ -1,0 +1,773 @@
+// Per-variant neighbour counting, neighbour table and clustering kernels.
+//
+// Work decomposition: a wavefront takes one item = up to kOrigins consecutive
+// origin points of one grid cell. The 64 lanes sweep the candidate points of
+// the cell's 3x3 window (three contiguous runs of the cell-sorted arrays, one
+// per grid row), 64 candidates per step, and every lane tests its candidate
+// against all origins of the item; the origin coordinates are wave-uniform.
+// A ballot per origin gives the accepted lanes, so counts are popcounts and
+// table rows are written as contiguous runs (coalesced stores).
+#include "vdb_cluster.h"
+#include "vdb_prims.h"
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace vdb {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr uint32_t kNone = 0xffffffffu;
+
+struct GridView {
+    const double *xs, *ys;
+    const uint32_t *cell_start, *row_lo, *row_hi;
+    int W;
+};
+
+GridView view(const Grid &G)
+{
+    return GridView{G.xs, G.ys, G.cell_start, G.row_lo, G.row_hi, 2 * G.R + 1};
+}
+
+// Visit the window of cell c in 64-candidate blocks. With `half`, only rows
+// below the centre row and the centre row up to (excluding) qmax are visited:
+// every unordered pair inside the window is then seen from one side.
+template <class F>
+__device__ __forceinline__ void window_blocks(const GridView &g, uint32_t c, bool half,
+                                              uint32_t qmax, F &&f)
+{
+    int R = g.W / 2;
+    for (int r = 0; r < g.W; r++) {
+        int d = r - R;
+        if (half && d > 0) break;
+        uint32_t lo = g.row_lo[(size_t)c * g.W + r], hi = g.row_hi[(size_t)c * g.W + r];
+        if (lo >= hi) continue;
+        uint32_t a = g.cell_start[lo], e = g.cell_start[hi];
+        if (half && d == 0 && qmax < e) e = qmax;
+        for (uint32_t q0 = a; q0 < e; q0 += kWave) {
+            uint32_t q = q0 + lane_id();
+            bool v = q < e;
+            double qx = v ? g.xs[q] : __longlong_as_double(0x7ff8000000000000ll);
+            double qy = v ? g.ys[q] : __longlong_as_double(0x7ff8000000000000ll);
+            f(q, v, qx, qy);
+        }
+    }
+}
+
+// FP32 screening with an exact FP64 decision inside the band. Coordinates
+// are taken relative to the item's first origin r: every origin and candidate
+// of a window lies within 3w of r per axis, so converting a difference to
+// float32 and evaluating dx*dx+dy*dy in float32 stays within band of the
+// binary64 value s (band = w*w*2^-14, see FP32_BAND in band_for()).
+struct Band {
+    float lo, hi; // accept if s32 <= lo, reject if s32 > hi, else recheck
+};
+
+__device__ __forceinline__ float dist2_f32(float ox, float oy, float qx, float qy)
+{
+    float dx = __fsub_rn(ox, qx);
+    float dy = __fsub_rn(oy, qy);
+    return __fmaf_rn(dx, dx, __fmul_rn(dy, dy));
+}
+
+// ----------------------------------------------------------------- items
+__global__ void k_item_counts(const uint32_t *cell_start, uint32_t ncell, uint32_t *cnt)
+{
+    uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= ncell) return;
+    uint32_t n = cell_start[c + 1] - cell_start[c];
+    cnt[c] = (n + kOrigins - 1) / kOrigins;
+}
+
+__global__ void k_item_fill(const uint32_t *cell_start, const uint32_t *excl, uint32_t ncell,
+                            uint32_t *item_cell, uint32_t *item_o0)
+{
+    uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= ncell) return;
+    uint32_t a = cell_start[c], e = cell_start[c + 1], k = excl[c];
+    for (uint32_t o = a; o < e; o += kOrigins, k++) {
+        item_cell[k] = c;
+        item_o0[k] = o;
+    }
+}
+
+// ------------------------------------------------------------- counting
+template <int PREC>
+__global__ void __launch_bounds__(kBlock)
+    k_count(GridView g, const uint32_t *__restrict__ item_cell,
+            const uint32_t *__restrict__ item_o0, uint32_t nitems, double T, Band band,
+            uint32_t *__restrict__ counts, unsigned long long *recheck)
+{
+    uint32_t it = blockIdx.x * kWavesPerBlock + wave_id_in_block();
+    if (it >= nitems) return;
+    uint32_t c = item_cell[it], o0 = item_o0[it];
+    uint32_t nb = g.cell_start[c + 1] - o0;
+    nb = nb < kOrigins ? nb : kOrigins;
+    double ox[kOrigins], oy[kOrigins];
+    float fx[kOrigins], fy[kOrigins];
+    uint32_t cnt[kOrigins];
+    double rx = g.xs[o0], ry = g.ys[o0];
+#pragma unroll
+    for (int b = 0; b < kOrigins; b++) {
+        uint32_t ob = o0 + (b < (int)nb ? b : nb - 1);
+        ox[b] = g.xs[ob];
+        oy[b] = g.ys[ob];
+        fx[b] = (float)__dsub_rn(ox[b], rx);
+        fy[b] = (float)__dsub_rn(oy[b], ry);
+        cnt[b] = 0;
+    }
+    uint32_t nre = 0;
+    window_blocks(g, c, false, 0, [&](uint32_t q, bool v, double qx, double qy) {
+        if (PREC == 0) {
+#pragma unroll
+            for (int b = 0; b < kOrigins; b++)
+                cnt[b] += popc64(ballot(dist2_exact(ox[b], oy[b], qx, qy) <= T));
+        } else {
+            float gx = (float)__dsub_rn(qx, rx), gy = (float)__dsub_rn(qy, ry);
+#pragma unroll
+            for (int b = 0; b < kOrigins; b++) {
+                float s = dist2_f32(fx[b], fy[b], gx, gy);
+                bool acc = s <= band.lo;
+                uint64_t amb = ballot(v && b < (int)nb && !acc && s <= band.hi);
+                if (amb) {
+                    if ((amb >> lane_id()) & 1ull) acc = dist2_exact(ox[b], oy[b], qx, qy) <= T;
+                    nre += popc64(amb);
+                }
+                cnt[b] += popc64(ballot(acc));
+            }
+        }
+    });
+    uint32_t mine = 0;
+#pragma unroll
+    for (int b = 0; b < kOrigins; b++)
+        if (lane_id() == b) mine = cnt[b];
+    if (lane_id() < (int)nb) counts[o0 + lane_id()] = mine;
+    if (PREC == 1 && nre && lane_id() == 0) atomicAdd(recheck, (unsigned long long)nre);
+}
+
+// ----------------------------------------------------------------- table
+// ids[row_off[p] + k] = idmap[q] (or q when idmap is null) for the k-th
+// accepted candidate q of origin p; rows located through rowmap.
+template <int PREC>
+__global__ void __launch_bounds__(kBlock)
+    k_fill(GridView g, const uint32_t *__restrict__ item_cell,
+           const uint32_t *__restrict__ item_o0, uint32_t it0, uint32_t it1, double T,
+           Band band, const uint64_t *__restrict__ offsets, uint64_t off_base,
+           const uint32_t *__restrict__ rowmap, const uint32_t *__restrict__ idmap,
+           uint32_t *__restrict__ ids, float *__restrict__ d2, uint32_t *overflow)
+{
+    uint32_t it = it0 + blockIdx.x * kWavesPerBlock + wave_id_in_block();
+    if (it >= it1) return;
+    uint32_t c = item_cell[it], o0 = item_o0[it];
+    uint32_t nb = g.cell_start[c + 1] - o0;
+    nb = nb < kOrigins ? nb : kOrigins;
+    double ox[kOrigins], oy[kOrigins];
+    float fx[kOrigins], fy[kOrigins];
+    uint64_t run[kOrigins], lim[kOrigins];
+    double rx = g.xs[o0], ry = g.ys[o0];
+#pragma unroll
+    for (int b = 0; b < kOrigins; b++) {
+        uint32_t ob = o0 + (b < (int)nb ? b : nb - 1);
+        ox[b] = g.xs[ob];
+        oy[b] = g.ys[ob];
+        fx[b] = (float)__dsub_rn(ox[b], rx);
+        fy[b] = (float)__dsub_rn(oy[b], ry);
+        uint32_t row = rowmap ? rowmap[ob] : ob;
+        run[b] = offsets[row] - off_base;
+        lim[b] = offsets[row + 1] - off_base;
+    }
+    bool over = false;
+    window_blocks(g, c, false, 0, [&](uint32_t q, bool v, double qx, double qy) {
+        uint32_t id = v ? (idmap ? idmap[q] : q) : 0u;
+        float gx = 0.f, gy = 0.f;
+        if (PREC == 1) {
+            gx = (float)__dsub_rn(qx, rx);
+            gy = (float)__dsub_rn(qy, ry);
+        }
+#pragma unroll
+        for (int b = 0; b < kOrigins; b++) {
+            if (b >= (int)nb) break;
+            double s = 0.0;
+            bool acc;
+            if (PREC == 0) {
+                s = dist2_exact(ox[b], oy[b], qx, qy);
+                acc = s <= T;
+            } else {
+                float s32 = dist2_f32(fx[b], fy[b], gx, gy);
+                acc = s32 <= band.lo;
+                bool amb = v && !acc && s32 <= band.hi;
+                if (ballot(amb)) {
+                    if (amb) acc = dist2_exact(ox[b], oy[b], qx, qy) <= T;
+                }
+                s = (double)s32;
+            }
+            uint64_t m = ballot(acc);
+            if (m) {
+                if (acc) {
+                    uint64_t dst = run[b] + (uint64_t)mask_rank(m);
+                    // the count pass sized every row; never write past it
+                    if (dst < lim[b]) {
+                        ids[dst] = id;
+                        if (d2)
+                            d2[dst] = PREC == 0 ? (float)s
+                                                : (float)dist2_exact(ox[b], oy[b], qx, qy);
+                    } else {
+                        over = true;
+                    }
+                }
+                run[b] += (uint64_t)popc64(m);
+            }
+        }
+    });
+    if (over) atomicOr(overflow, 1u);
+}
+
+// ------------------------------------------------------------ union-find
+// Parents live in cell-position space; every parent pointer goes to a node
+// of smaller binned rank, so a root is the smallest rank of its tree. Loads
+// and stores use agent scope so that no wave spins on a stale L1 copy.
+__device__ __forceinline__ uint32_t ld_par(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_par(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ uint32_t uf_find(uint32_t *parent, uint32_t x)
+{
+    while (true) {
+        uint32_t p = ld_par(parent + x);
+        if (p == x) return x;
+        uint32_t gp = ld_par(parent + p);
+        if (gp == p) return p;
+        st_par(parent + x, gp); // path halving: gp is an ancestor of x
+        x = gp;
+    }
+}
+
+__device__ void uf_unite(uint32_t *parent, const uint32_t *rank, uint32_t a, uint32_t b)
+{
+    while (true) {
+        a = uf_find(parent, a);
+        b = uf_find(parent, b);
+        if (a == b) return;
+        if (rank[a] > rank[b]) {
+            uint32_t t = a;
+            a = b;
+            b = t;
+        }
+        // hook the larger-rank root under the smaller one
+        if (atomicCAS(parent + b, b, a) == b) return;
+    }
+}
+
+__global__ void k_core_init(const uint32_t *counts, uint32_t n, uint32_t minpts,
+                            uint8_t *core, uint32_t *parent, uint32_t *best)
+{
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    // minpts above INT_MAX wraps negative in the reference's int and no point
+    // is core (src/structs.h:37, src/DBScan.cpp:552)
+    bool isc = minpts <= 0x7fffffffu && counts[p] >= minpts;
+    core[p] = isc ? 1 : 0;
+    parent[p] = p;
+    best[p] = kNone;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_uf_fused(GridView g, const uint32_t *__restrict__ item_cell,
+               const uint32_t *__restrict__ item_o0, uint32_t nitems, double T,
+               const uint8_t *__restrict__ core, const uint32_t *__restrict__ rank,
+               uint32_t *parent)
+{
+    uint32_t it = blockIdx.x * kWavesPerBlock + wave_id_in_block();
+    if (it >= nitems) return;
+    uint32_t c = item_cell[it], o0 = item_o0[it];
+    uint32_t nb = g.cell_start[c + 1] - o0;
+    nb = nb < kOrigins ? nb : kOrigins;
+    double ox[kOrigins], oy[kOrigins];
+    uint32_t rp[kOrigins];
+    uint32_t any = 0;
+#pragma unroll
+    for (int b = 0; b < kOrigins; b++) {
+        uint32_t ob = o0 + (b < (int)nb ? b : nb - 1);
+        bool isc = b < (int)nb && core[ob];
+        ox[b] = isc ? g.xs[ob] : __longlong_as_double(0x7ff8000000000000ll);
+        oy[b] = g.ys[ob];
+        rp[b] = isc ? uf_find(parent, ob) : kNone;
+        any |= isc;
+    }
+    if (!any) return;
+    uint32_t qmax = o0 + nb;
+    window_blocks(g, c, true, qmax, [&](uint32_t q, bool v, double qx, double qy) {
+        bool qc = v && core[q];
+        if (!ballot(qc)) return;
+        uint32_t pq = qc ? ld_par(parent + q) : 0u;
+#pragma unroll
+        for (int b = 0; b < kOrigins; b++) {
+            if (rp[b] == kNone) continue;
+            bool e = qc && dist2_exact(ox[b], oy[b], qx, qy) <= T && pq != rp[b];
+            uint64_t m = ballot(e);
+            if (m) {
+                if (e) uf_unite(parent, rank, q, o0 + b);
+                rp[b] = uf_find(parent, o0 + b);
+            }
+        }
+    });
+}
+
+__global__ void k_compress(uint32_t *parent, const uint8_t *core, uint32_t n)
+{
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n || !core[p]) return;
+    st_par(parent + p, uf_find(parent, p));
+}
+
+// border: best[p] = smallest root rank among core neighbours of non-core p
+__global__ void __launch_bounds__(kBlock)
+    k_border_fused(GridView g, const uint32_t *__restrict__ item_cell,
+                   const uint32_t *__restrict__ item_o0, uint32_t nitems, double T,
+                   const uint8_t *__restrict__ core, const uint32_t *__restrict__ rank,
+                   const uint32_t *__restrict__ parent, uint32_t *__restrict__ best)
+{
+    uint32_t it = blockIdx.x * kWavesPerBlock + wave_id_in_block();
+    if (it >= nitems) return;
+    uint32_t c = item_cell[it], o0 = item_o0[it];
+    uint32_t nb = g.cell_start[c + 1] - o0;
+    nb = nb < kOrigins ? nb : kOrigins;
+    double ox[kOrigins], oy[kOrigins];
+    uint32_t bst[kOrigins];
+    uint32_t any = 0;
+#pragma unroll
+    for (int b = 0; b < kOrigins; b++) {
+        uint32_t ob = o0 + (b < (int)nb ? b : nb - 1);
+        bool nc = b < (int)nb && !core[ob];
+        ox[b] = nc ? g.xs[ob] : __longlong_as_double(0x7ff8000000000000ll);
+        oy[b] = g.ys[ob];
+        bst[b] = kNone;
+        any |= nc;
+    }
+    if (!any) return;
+    window_blocks(g, c, false, 0, [&](uint32_t q, bool v, double qx, double qy) {
+        bool qc = v && core[q];
+        if (!ballot(qc)) return;
+        uint32_t rr = qc ? rank[parent[q]] : kNone;
+#pragma unroll
+        for (int b = 0; b < kOrigins; b++) {
+            uint32_t cand = (qc && dist2_exact(ox[b], oy[b], qx, qy) <= T) ? rr : kNone;
+            if (!ballot(cand != kNone)) continue;
+            for (int o = kWave / 2; o > 0; o >>= 1) {
+                uint32_t t = __shfl_xor(cand, o);
+                cand = t < cand ? t : cand;
+            }
+            bst[b] = cand < bst[b] ? cand : bst[b];
+        }
+    });
+    uint32_t mine = kNone;
+#pragma unroll
+    for (int b = 0; b < kOrigins; b++)
+        if (lane_id() == b) mine = bst[b];
+    if (lane_id() < (int)nb && !core[o0 + lane_id()]) best[o0 + lane_id()] = mine;
+}
+
+// --------------------------------------------- union-find over a CSR table
+// one wave per row; rows [p0, p1) of position space, ids are positions
+__global__ void __launch_bounds__(kBlock)
+    k_uf_table(const uint64_t *__restrict__ offsets, uint64_t off_base,
+               const uint32_t *__restrict__ ids, uint32_t p0, uint32_t p1,
+               const uint8_t *__restrict__ core, const uint32_t *__restrict__ rank,
+               uint32_t *parent)
+{
+    uint32_t p = p0 + blockIdx.x * kWavesPerBlock + wave_id_in_block();
+    if (p >= p1 || !core[p]) return;
+    uint64_t a = offsets[p] - off_base, e = offsets[p + 1] - off_base;
+    uint32_t rp = uf_find(parent, p);
+    for (uint64_t j0 = a; j0 < e; j0 += kWave) {
+        uint64_t j = j0 + lane_id();
+        uint32_t q = j < e ? ids[j] : kNone;
+        bool qc = q < p && core[q];
+        uint32_t pq = qc ? ld_par(parent + q) : 0u;
+        bool need = qc && pq != rp;
+        if (ballot(need)) {
+            if (need) uf_unite(parent, rank, q, p);
+            rp = uf_find(parent, p);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_border_table(const uint64_t *__restrict__ offsets, uint64_t off_base,
+                   const uint32_t *__restrict__ ids, uint32_t p0, uint32_t p1,
+                   const uint8_t *__restrict__ core, const uint32_t *__restrict__ rank,
+                   const uint32_t *__restrict__ parent, uint32_t *__restrict__ best)
+{
+    uint32_t p = p0 + blockIdx.x * kWavesPerBlock + wave_id_in_block();
+    if (p >= p1 || core[p]) return;
+    uint64_t a = offsets[p] - off_base, e = offsets[p + 1] - off_base;
+    uint32_t bst = kNone;
+    for (uint64_t j0 = a; j0 < e; j0 += kWave) {
+        uint64_t j = j0 + lane_id();
+        uint32_t q = j < e ? ids[j] : kNone;
+        uint32_t cand = (q != kNone && core[q]) ? rank[parent[q]] : kNone;
+        bst = cand < bst ? cand : bst;
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        uint32_t t = __shfl_xor(bst, o);
+        bst = t < bst ? t : bst;
+    }
+    if (lane_id() == 0) best[p] = bst;
+}
+
+// ------------------------------------------------------------ numbering
+__global__ void k_root_flags(const uint8_t *core, const uint32_t *parent, const uint32_t *rank,
+                             uint32_t n, uint32_t *flag_by_rank)
+{
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    flag_by_rank[rank[p]] = (core[p] && parent[p] == p) ? 1u : 0u;
+}
+
+__global__ void k_labels(const uint8_t *core, const uint32_t *parent, const uint32_t *rank,
+                         const uint32_t *best, const uint32_t *cid_by_rank,
+                         const uint32_t *mapping, uint32_t n, uint32_t *out)
+{
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    uint32_t r = rank[p];
+    uint32_t lbl;
+    if (core[p]) {
+        lbl = cid_by_rank[rank[parent[p]]] + 1;
+    } else {
+        uint32_t b = best[p];
+        lbl = (b != kNone && b < r) ? cid_by_rank[b] + 1 : 0u;
+    }
+    out[mapping[r]] = lbl;
+}
+
+__global__ void k_perm_counts(const uint32_t *cnt, const uint32_t *pos_rank,
+                              const uint32_t *mapping, uint32_t n, uint32_t *cc, uint32_t *cop)
+{
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    uint32_t ci = mapping[pos_rank[p]];
+    cop[p] = ci;
+    cc[ci] = cnt[p];
+}
+
+Band band_for(const Grid &G, double T)
+{
+    // |s32 - s| <= w^2 * 2^-14 for coordinates within 3w of the reference
+    // origin (float32 conversion, subtraction, product and sum each add at
+    // most a few ulps of (3w)^2); the band used is that bound, widened to
+    // the float grid around T.
+    double band = G.w * G.w * std::ldexp(1.0, -14);
+    Band b;
+    double lo = T - band, hi = T + band;
+    b.lo = lo > 0 ? std::nextafter((float)lo, 0.0f) : -1.0f;
+    if (b.lo > lo) b.lo = std::nextafter(b.lo, -INFINITY);
+    b.hi = std::nextafter((float)hi, INFINITY);
+    if (T < 0) {
+        b.lo = -1.0f;
+        b.hi = -1.0f;
+    }
+    return b;
+}
+
+struct EvTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+    hipStream_t s;
+    explicit EvTimer(hipStream_t st) : s(st)
+    {
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+    }
+    ~EvTimer()
+    {
+        if (a) hipEventDestroy(a);
+        if (b) hipEventDestroy(b);
+    }
+    void start() { hipEventRecord(a, s); }
+    float stop()
+    {
+        hipEventRecord(b, s);
+        hipEventSynchronize(b);
+        float ms = 0;
+        hipEventElapsedTime(&ms, a, b);
+        return ms;
+    }
+};
+
+} // namespace
+
+int build_items(Context &c, const Grid &G, const char *tag, Items &I)
+{
+    uint32_t *cnt, *excl;
+    std::string t(tag);
+    VDB_TRY(c.get((t + ".icnt").c_str(), G.ncell, &cnt));
+    VDB_TRY(c.get((t + ".iexcl").c_str(), G.ncell, &excl));
+    k_item_counts<<<ceil_div(G.ncell, kBlock), kBlock, 0, c.stream()>>>(G.cell_start, G.ncell,
+                                                                        cnt);
+    VDB_HIP(hipGetLastError());
+    uint32_t total = 0;
+    VDB_TRY(compact_count(c, cnt, excl, G.ncell, &total));
+    I.n = total;
+    VDB_TRY(c.get((t + ".icell").c_str(), total, &I.cell));
+    VDB_TRY(c.get((t + ".io0").c_str(), total, &I.o0));
+    k_item_fill<<<ceil_div(G.ncell, kBlock), kBlock, 0, c.stream()>>>(G.cell_start, excl,
+                                                                      G.ncell, I.cell, I.o0);
+    VDB_HIP(hipGetLastError());
+    return kOk;
+}
+
+int count_neighbors(Context &c, const Grid &G, const Items &I, double eps, int precision,
+                    uint32_t *counts, uint64_t *recheck_dev)
+{
+    double T = threshold_for_eps(eps);
+    if (I.n == 0) return kOk;
+    unsigned blocks = ceil_div(I.n, kWavesPerBlock);
+    if (precision == 0) {
+        k_count<0><<<blocks, kBlock, 0, c.stream()>>>(view(G), I.cell, I.o0, I.n, T, Band{},
+                                                      counts, nullptr);
+    } else {
+        k_count<1><<<blocks, kBlock, 0, c.stream()>>>(view(G), I.cell, I.o0, I.n, T,
+                                                      band_for(G, T), counts,
+                                                      (unsigned long long *)recheck_dev);
+    }
+    VDB_HIP(hipGetLastError());
+    return kOk;
+}
+
+static int finish_labels(Context &c, const Points &P, const Grid &G, const uint8_t *core,
+                         const uint32_t *parent, const uint32_t *best, uint32_t *out)
+{
+    uint32_t n = P.n;
+    unsigned g = ceil_div(n, kBlock);
+    uint32_t *flag, *cid;
+    VDB_TRY(c.get("lab.flag", n, &flag));
+    VDB_TRY(c.get("lab.cid", n, &cid));
+    k_root_flags<<<g, kBlock, 0, c.stream()>>>(core, parent, G.pos_rank, n, flag);
+    VDB_HIP(hipGetLastError());
+    VDB_TRY(scan_exclusive_u32(c, flag, cid, n, nullptr));
+    k_labels<<<g, kBlock, 0, c.stream()>>>(core, parent, G.pos_rank, best, cid, P.mapping, n,
+                                           out);
+    VDB_HIP(hipGetLastError());
+    return kOk;
+}
+
+int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, double eps,
+                    uint32_t minpts, const uint32_t *counts, int mode, size_t table_budget,
+                    uint32_t *out, PhaseTimes *pt)
+{
+    uint32_t n = P.n;
+    double T = threshold_for_eps(eps);
+    unsigned g = ceil_div(n, kBlock);
+    uint8_t *core;
+    uint32_t *parent, *best;
+    VDB_TRY(c.get("cv.core", n, &core));
+    VDB_TRY(c.get("cv.parent", n, &parent));
+    VDB_TRY(c.get("cv.best", n, &best));
+    EvTimer tm(c.stream());
+    k_core_init<<<g, kBlock, 0, c.stream()>>>(counts, n, minpts, core, parent, best);
+    VDB_HIP(hipGetLastError());
+    unsigned blocks = ceil_div(I.n, kWavesPerBlock);
+    if (mode == 0) {
+        if (pt) tm.start();
+        if (I.n)
+            k_uf_fused<<<blocks, kBlock, 0, c.stream()>>>(view(G), I.cell, I.o0, I.n, T, core,
+                                                          G.pos_rank, parent);
+        VDB_HIP(hipGetLastError());
+        k_compress<<<g, kBlock, 0, c.stream()>>>(parent, core, n);
+        VDB_HIP(hipGetLastError());
+        if (pt) pt->uf += tm.stop(), tm.start();
+        if (I.n)
+            k_border_fused<<<blocks, kBlock, 0, c.stream()>>>(view(G), I.cell, I.o0, I.n, T,
+                                                              core, G.pos_rank, parent, best);
+        VDB_HIP(hipGetLastError());
+        if (pt) pt->border += tm.stop();
+    } else {
+        // table mode: offsets over positions, then tiles of whole cells
+        uint64_t *offsets;
+        VDB_TRY(c.get("cv.offsets", (size_t)n + 1, &offsets));
+        if (pt) tm.start();
+        VDB_TRY(scan_exclusive_u32_u64(c, counts, offsets, n, offsets + n));
+        if (pt) pt->scan += tm.stop();
+        std::vector<uint64_t> hoff(n + 1);
+        std::vector<uint32_t> hstart(G.ncell + 1), hio0(I.n);
+        VDB_HIP(hipMemcpyAsync(hoff.data(), offsets, sizeof(uint64_t) * (n + 1),
+                               hipMemcpyDeviceToHost, c.stream()));
+        VDB_HIP(hipMemcpyAsync(hstart.data(), G.cell_start, sizeof(uint32_t) * (G.ncell + 1),
+                               hipMemcpyDeviceToHost, c.stream()));
+        VDB_HIP(hipMemcpyAsync(hio0.data(), I.o0, sizeof(uint32_t) * I.n,
+                               hipMemcpyDeviceToHost, c.stream()));
+        VDB_HIP(hipStreamSynchronize(c.stream()));
+        uint64_t P_total = hoff[n];
+        if (pt) pt->pairs = P_total;
+        uint64_t cap = table_budget / sizeof(uint32_t);
+        // largest single cell must fit
+        uint32_t *ids = nullptr, *ovf = nullptr;
+        VDB_TRY(c.get("cv.ovf", 1, &ovf));
+        VDB_HIP(hipMemsetAsync(ovf, 0, sizeof(uint32_t), c.stream()));
+        uint64_t want = P_total < cap ? P_total : cap;
+        VDB_TRY(c.get("cv.ids", want ? want : 1, &ids));
+        // tiles are runs of items (items are ordered by cell, origins ascending)
+        uint32_t ntiles = 0;
+        uint32_t i0 = 0;
+        while (i0 < I.n) {
+            uint32_t i1 = i0;
+            uint64_t base = hoff[hio0[i0]];
+            while (i1 < I.n) {
+                uint32_t oend = (i1 + 1 < I.n) ? hio0[i1 + 1] : n;
+                if (hoff[oend] - base > cap && i1 > i0) break;
+                i1++;
+            }
+            uint32_t pend = i1 < I.n ? hio0[i1] : n;
+            if (hoff[pend] - base > cap) {
+                set_error("neighbour table tile does not fit the table budget");
+                return kErrNoMemory;
+            }
+            if (pt) tm.start();
+            k_fill<0><<<ceil_div(i1 - i0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
+                view(G), I.cell, I.o0, i0, i1, T, Band{}, offsets, base, nullptr, nullptr, ids,
+                nullptr, ovf);
+            VDB_HIP(hipGetLastError());
+            if (pt) pt->fill += tm.stop(), tm.start();
+            uint32_t p0 = hio0[i0];
+            k_uf_table<<<ceil_div(pend - p0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
+                offsets, base, ids, p0, pend, core, G.pos_rank, parent);
+            VDB_HIP(hipGetLastError());
+            if (pt) pt->uf += tm.stop();
+            ntiles++;
+            i0 = i1;
+        }
+        if (pt) tm.start();
+        k_compress<<<g, kBlock, 0, c.stream()>>>(parent, core, n);
+        VDB_HIP(hipGetLastError());
+        if (pt) pt->uf += tm.stop(), tm.start();
+        // border rows are short (fewer than minpts entries): re-read them tile by tile
+        i0 = 0;
+        while (i0 < I.n) {
+            uint32_t i1 = i0;
+            uint64_t base = hoff[hio0[i0]];
+            while (i1 < I.n) {
+                uint32_t oend = (i1 + 1 < I.n) ? hio0[i1 + 1] : n;
+                if (hoff[oend] - base > cap && i1 > i0) break;
+                i1++;
+            }
+            uint32_t pend = i1 < I.n ? hio0[i1] : n;
+            uint32_t p0 = hio0[i0];
+            if (ntiles > 1) {
+                // the table no longer holds this tile: rebuild it
+                k_fill<0><<<ceil_div(i1 - i0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
+                    view(G), I.cell, I.o0, i0, i1, T, Band{}, offsets, base, nullptr, nullptr,
+                    ids, nullptr, ovf);
+                VDB_HIP(hipGetLastError());
+            }
+            k_border_table<<<ceil_div(pend - p0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
+                offsets, base, ids, p0, pend, core, G.pos_rank, parent, best);
+            VDB_HIP(hipGetLastError());
+            i0 = i1;
+        }
+        if (pt) pt->border += tm.stop(), pt->tiles += ntiles;
+        uint32_t hov = 0;
+        VDB_HIP(hipMemcpyAsync(&hov, ovf, sizeof hov, hipMemcpyDeviceToHost, c.stream()));
+        VDB_HIP(hipStreamSynchronize(c.stream()));
+        if (hov) {
+            set_error("neighbour table fill disagreed with the count pass");
+            return kErrDevice;
+        }
+    }
+    if (pt) tm.start();
+    VDB_TRY(finish_labels(c, P, G, core, parent, best, out));
+    if (pt) pt->labels += tm.stop();
+    return kOk;
+}
+
+namespace {
+__global__ void k_pos_to_caller(const uint32_t *v, const uint32_t *pos_rank,
+                                const uint32_t *mapping, uint32_t n, uint32_t *out)
+{
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    out[mapping[pos_rank[p]]] = v[p];
+}
+} // namespace
+
+int pos_to_caller(Context &c, const Points &P, const Grid &G, const uint32_t *by_pos,
+                  uint32_t *by_caller)
+{
+    k_pos_to_caller<<<ceil_div(P.n, kBlock), kBlock, 0, c.stream()>>>(by_pos, G.pos_rank,
+                                                                     P.mapping, P.n, by_caller);
+    VDB_HIP(hipGetLastError());
+    return kOk;
+}
+
+int neighbor_table(Context &c, const Points &P, const Grid &G, const Items &I, double eps,
+                   int precision, uint64_t *offsets, uint32_t *ids, float *d2,
+                   uint64_t capacity, uint64_t *total, PhaseTimes *pt)
+{
+    uint32_t n = P.n;
+    double T = threshold_for_eps(eps);
+    uint32_t *counts, *counts_caller, *caller_of_pos;
+    VDB_TRY(c.get("nt.counts", n, &counts));
+    VDB_TRY(c.get("nt.counts_caller", n, &counts_caller));
+    VDB_TRY(c.get("nt.caller_of_pos", n, &caller_of_pos));
+    unsigned long long *rc;
+    VDB_TRY(c.get("nt.recheck", 1, &rc));
+    VDB_HIP(hipMemsetAsync(rc, 0, sizeof(unsigned long long), c.stream()));
+    EvTimer tm(c.stream());
+    if (pt) tm.start();
+    VDB_TRY(count_neighbors(c, G, I, eps, precision, counts, (uint64_t *)rc));
+    if (pt) pt->count += tm.stop();
+    // caller index of each position, and counts in caller order
+    k_perm_counts<<<ceil_div(n, kBlock), kBlock, 0, c.stream()>>>(counts, G.pos_rank, P.mapping, n,
+                                                            counts_caller, caller_of_pos);
+    VDB_HIP(hipGetLastError());
+    if (pt) tm.start();
+    VDB_TRY(scan_exclusive_u32_u64(c, counts_caller, offsets, n, offsets + n));
+    if (pt) pt->scan += tm.stop();
+    uint64_t tot = 0;
+    VDB_HIP(hipMemcpyAsync(&tot, offsets + n, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           c.stream()));
+    VDB_HIP(hipStreamSynchronize(c.stream()));
+    *total = tot;
+    if (pt) {
+        pt->pairs = tot;
+        unsigned long long r = 0;
+        VDB_HIP(hipMemcpy(&r, rc, sizeof r, hipMemcpyDeviceToHost));
+        pt->recheck = r;
+    }
+    if (tot > capacity) return kOk; // caller sizes its buffers from *total
+    if (I.n == 0) return kOk;
+    if (pt) tm.start();
+    uint32_t *ovf = nullptr;
+    VDB_TRY(c.get("nt.ovf", 1, &ovf));
+    VDB_HIP(hipMemsetAsync(ovf, 0, sizeof(uint32_t), c.stream()));
+    unsigned blocks = ceil_div(I.n, kWavesPerBlock);
+    if (precision == 0)
+        k_fill<0><<<blocks, kBlock, 0, c.stream()>>>(view(G), I.cell, I.o0, 0, I.n, T, Band{},
+                                                     offsets, 0, caller_of_pos, caller_of_pos,
+                                                     ids, d2, ovf);
+    else
+        k_fill<1><<<blocks, kBlock, 0, c.stream()>>>(view(G), I.cell, I.o0, 0, I.n, T,
+                                                     band_for(G, T), offsets, 0, caller_of_pos,
+                                                     caller_of_pos, ids, d2, ovf);
+    VDB_HIP(hipGetLastError());
+    if (pt) pt->fill += tm.stop();
+    uint32_t hov = 0;
+    VDB_HIP(hipMemcpyAsync(&hov, ovf, sizeof hov, hipMemcpyDeviceToHost, c.stream()));
+    VDB_HIP(hipStreamSynchronize(c.stream()));
+    if (hov) {
+        set_error("neighbour table fill disagreed with the count pass");
+        return kErrDevice;
+    }
+    return kOk;
+}
+
+} // namespace vdb

@@ -1,0 +1,106 @@
+// Shared definitions for the VariantDBSCAN MI355X library (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+namespace vdb {
+
+constexpr int kWave = 64;
+
+// ------------------------------------------------------------------ errors
+// Every library entry point returns an int status; 0 = success.
+enum Status : int {
+    kOk = 0,
+    kErrInvalid = 1,     // bad argument (NULL pointer, MBBsize <= 0, ...)
+    kErrNonFinite = 2,   // NaN/inf coordinate
+    kErrDevice = 3,      // HIP runtime failure
+    kErrNoMemory = 4,    // device or host allocation failed
+    kErrUnsupported = 5, // request outside what this build handles
+};
+
+void set_error(const std::string &msg);
+const char *last_error();
+
+#define VDB_HIP(call)                                                          \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) {                                                \
+            ::vdb::set_error(std::string(#call) + ": " + hipGetErrorString(e_) + \
+                             " @" + __FILE__ + ":" + std::to_string(__LINE__)); \
+            return e_ == hipErrorOutOfMemory ? ::vdb::kErrNoMemory             \
+                                             : ::vdb::kErrDevice;              \
+        }                                                                      \
+    } while (0)
+
+#define VDB_TRY(expr)                                                          \
+    do {                                                                       \
+        int s_ = (expr);                                                       \
+        if (s_ != ::vdb::kOk) return s_;                                       \
+    } while (0)
+
+// --------------------------------------------------------- exact predicate
+// The reference accepts q as an eps-neighbour of p when
+//   sqrt((px-qx)*(px-qx) + (py-qy)*(py-qy)) <= eps        (src/DBScan.cpp:687,699-702)
+// evaluated in binary64 with every operation rounded separately. sqrt_rn is
+// monotone, so this equals  s <= T(eps)  with s the unrounded-free sum below
+// and T(eps) the largest double whose rounded square root is <= eps
+// (computed on the host, threshold_for_eps). The explicit _rn intrinsics keep
+// the compiler from contracting the products into an FMA.
+__device__ __forceinline__ double dist2_exact(double px, double py, double qx,
+                                              double qy)
+{
+    double dx = __dsub_rn(px, qx);
+    double dy = __dsub_rn(py, qy);
+    return __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
+}
+
+// Exact lower / upper bounds of dist2_exact(p, q) over every q inside a box
+// [bx0,bx1]x[by0,by1] (bounds taken from actual point coordinates). Rounding
+// is monotone in each operand, so evaluating the formula at the extreme
+// coordinate differences bounds every pair inside the box.
+__device__ __forceinline__ double box_dist2_min(double px, double py, double bx0,
+                                                double bx1, double by0, double by1)
+{
+    double gx = 0.0, gy = 0.0;
+    if (px < bx0) gx = __dsub_rn(bx0, px);
+    else if (px > bx1) gx = __dsub_rn(px, bx1);
+    if (py < by0) gy = __dsub_rn(by0, py);
+    else if (py > by1) gy = __dsub_rn(py, by1);
+    return __dadd_rn(__dmul_rn(gx, gx), __dmul_rn(gy, gy));
+}
+
+__device__ __forceinline__ double box_dist2_max(double px, double py, double bx0,
+                                                double bx1, double by0, double by1)
+{
+    double gx = fmax(__dsub_rn(px, bx0), __dsub_rn(bx1, px));
+    double gy = fmax(__dsub_rn(py, by0), __dsub_rn(by1, py));
+    return __dadd_rn(__dmul_rn(gx, gx), __dmul_rn(gy, gy));
+}
+
+// --------------------------------------------------------------- wave ops
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ int popc64(uint64_t m) { return __popcll(m); }
+
+// number of set bits of m strictly below this lane
+__device__ __forceinline__ int mask_rank(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+}
+
+__device__ __forceinline__ int wave_id_in_block() { return threadIdx.x / kWave; }
+
+template <typename T> __device__ __forceinline__ T uniform(T v)
+{
+    // value of lane 0 (caller guarantees the value is wave-uniform)
+    return __shfl(v, 0);
+}
+
+inline unsigned ceil_div(uint64_t a, uint64_t b) { return (unsigned)((a + b - 1) / b); }
+
+} // namespace vdb

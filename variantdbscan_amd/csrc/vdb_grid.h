@@ -1,0 +1,55 @@
+// Point layout in HBM.
+//
+//   caller order  : x[i], y[i]                      (as handed in)
+//   rank order    : xb[r], yb[r], mapping[r]         r = position in the reference's
+//                                                    binned visit order
+//   cell order    : xs[p], ys[p], pos_rank[p]        p = position after a stable sort by
+//                                                    grid cell; inside a cell, ranks ascend
+//
+// A Grid is the cell-order view for one cell side. Only non-empty cells are
+// stored (sorted keys + CSR starts), so the footprint is O(n) whatever eps is.
+#pragma once
+
+#include "vdb_context.h"
+
+namespace vdb {
+
+// T(eps) = max{ s >= 0 : sqrt_rn(s) <= eps }, -1 when nothing qualifies.
+double threshold_for_eps(double eps);
+
+struct Points {
+    uint32_t n = 0;
+    const double *x = nullptr, *y = nullptr; // caller order (device)
+    double *xb = nullptr, *yb = nullptr;     // rank order
+    uint32_t *mapping = nullptr;             // rank -> caller index
+    double ext[4] = {0, 0, 0, 0};            // min x, max x, min y, max y
+};
+
+// Reference binning (src/main.cpp:459-605) on the device. Fills the rank
+// order of P. Returns kErrUnsupported when a bin is out of range.
+int build_rank_order(Context &c, Points &P, const char *tag);
+
+struct Grid {
+    double w = 0, gx0 = 0, gy0 = 0;
+    uint32_t ncx = 0, ncy = 0; // key = cx*ncy + cy
+    uint32_t ncell = 0;        // non-empty cells
+    int R = 1;                 // window radius in cells
+    double *xs = nullptr, *ys = nullptr;
+    uint32_t *pos_rank = nullptr;   // [n]
+    uint32_t *rank_pos = nullptr;   // [n] inverse of pos_rank
+    uint32_t *cell_of_pos = nullptr;// [n]
+    uint32_t *cell_key = nullptr;   // [ncell]
+    uint32_t *cell_start = nullptr; // [ncell+1]
+    double *cell_box = nullptr;     // [ncell*4] x0,x1,y0,y1 of the cell's points
+    uint32_t *row_lo = nullptr;     // [ncell*(2R+1)] first cell of window row
+    uint32_t *row_hi = nullptr;     // [ncell*(2R+1)] one past the last
+    uint32_t max_cell = 0;          // largest cell population
+};
+
+// Build a grid of cell side >= w_min over the rank-ordered points. If the key
+// space would exceed 32 bits the side is enlarged; *enlarged reports it.
+// tag prefixes the pool slot names so several grids can coexist.
+int build_grid(Context &c, const Points &P, double w_min, int R, const char *tag,
+               Grid &G, bool *enlarged);
+
+} // namespace vdb
