@@ -1,12 +1,883 @@
+// Batched variants from one fine grid and one truncated, distance-sorted
+// neighbour table.
+//
+// Levels k = 0..K-1 are the distinct eps values in ascending order (T_k their
+// thresholds), j = 0..J-1 the distinct minpts values, L = max minpts.
+//
+//  1. Fine grid of side d <= eps_0/sqrt(2): every cell is checked to be a
+//     clique at eps_0 (its point bbox diagonal passes the exact predicate),
+//     so all core points of a cell are mutually connected at every level.
+//  2. Truncated table: for each point of a cell holding fewer than L points,
+//     the L nearest neighbours within eps_{K-1}, sorted by (s, id). A smaller
+//     eps is a prefix cut of that list and a larger minpts a filter on its
+//     length; points of cells holding >= L points have >= L neighbours within
+//     eps_0 and are core in every variant.
+//  3. Core level kc_j(p) = first k with count_k(p) >= m_j.
+//  4. Cell pairs within eps_{K-1} (bbox distance), with the first level at
+//     which any pair may connect (kmin) and the first at which every pair is
+//     connected (kfull).
+//  5. For each j, union-find over cells carried across k = 0..K-1 (components
+//     only merge as eps grows); a pair leaves the worklist once connected.
+//     After each level the component roots give the labels of variant (k,j):
+//     core points by the smallest rank of their component, border points by
+//     the smallest such root among the core entries of their list prefix.
 #include "vdb_batch.h"
+#include "vdb_prims.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
 
 namespace vdb {
 
-int cluster_batch(Context &, const Points &, const double *, const uint32_t *, uint32_t, int,
-                  uint32_t *, vdb_stats *, bool, BatchResult *)
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWPB = kBlock / kWave;
+constexpr uint32_t kNone = 0xffffffffu;
+constexpr int kMaxList = 64;
+constexpr int kMaxLevels = 64;
+constexpr int kMaxMinpts = 64;
+// pairs whose cells hold at most this many point pairs are tested by one thread
+constexpr uint32_t kSmallPairWork = 16;
+
+struct FG {
+    const double *xs, *ys, *box;
+    const uint32_t *cell_start, *cell_of_pos, *row_lo, *row_hi, *pos_rank;
+    int W;
+};
+
+FG fview(const Grid &G)
 {
-    set_error("batched engine not built yet");
-    return kErrUnsupported;
+    return FG{G.xs, G.ys, G.cell_box, G.cell_start, G.cell_of_pos, G.row_lo, G.row_hi,
+              G.pos_rank, 2 * G.R + 1};
+}
+
+__device__ __forceinline__ uint32_t ld_u32(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_u32(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// union-find over cells: parents point to smaller cell indices
+__device__ uint32_t cf_find(uint32_t *parent, uint32_t x)
+{
+    while (true) {
+        uint32_t p = ld_u32(parent + x);
+        if (p == x) return x;
+        uint32_t gp = ld_u32(parent + p);
+        if (gp == p) return p;
+        st_u32(parent + x, gp);
+        x = gp;
+    }
+}
+
+__device__ void cf_unite(uint32_t *parent, uint32_t a, uint32_t b)
+{
+    while (true) {
+        a = cf_find(parent, a);
+        b = cf_find(parent, b);
+        if (a == b) return;
+        if (a > b) {
+            uint32_t t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS(parent + b, b, a) == b) return;
+    }
+}
+
+// ------------------------------------------------------------- cliques
+__global__ void k_clique_check(const double *box, uint32_t ncell, double T0, uint32_t *bad)
+{
+    uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= ncell) return;
+    const double *b = box + 4 * (size_t)c;
+    double dx = __dsub_rn(b[1], b[0]), dy = __dsub_rn(b[3], b[2]);
+    double s = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
+    if (!(s <= T0)) atomicAdd(bad, 1u);
+}
+
+// ------------------------------------------------------ list membership
+__global__ void k_list_flags(const uint32_t *cell_start, const uint32_t *cell_of_pos,
+                             uint32_t n, uint32_t L, uint32_t *flag)
+{
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    uint32_t c = cell_of_pos[p];
+    flag[p] = (cell_start[c + 1] - cell_start[c] < L) ? 1u : 0u;
+}
+
+__global__ void k_list_index(const uint32_t *flag, const uint32_t *excl, uint32_t n,
+                             uint32_t *list_pos, uint32_t *lst_idx)
+{
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    if (flag[p]) {
+        list_pos[excl[p]] = p;
+        lst_idx[p] = excl[p];
+    } else {
+        lst_idx[p] = kNone;
+    }
+}
+
+// ------------------------------------------------------ truncated table
+// One wave per listed point. Lane e < L holds entry e of the sorted list.
+__device__ __forceinline__ bool before(double s1, uint32_t i1, double s2, uint32_t i2)
+{
+    return s1 < s2 || (s1 == s2 && i1 < i2);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_knn(FG g, const uint32_t *__restrict__ list_pos, uint32_t nlist, int L, double Tmax,
+          double T0, double *__restrict__ lst_s, uint32_t *__restrict__ lst_id,
+          uint32_t *__restrict__ lst_n)
+{
+    uint32_t i = blockIdx.x * kWPB + wave_id_in_block();
+    if (i >= nlist) return;
+    int lane = lane_id();
+    uint32_t p = list_pos[i];
+    double px = g.xs[p], py = g.ys[p];
+    uint32_t c = g.cell_of_pos[p];
+    double es = INFINITY;  // entry s of this lane
+    uint32_t eid = kNone;  // entry id
+    int cnt = 0;           // entries held (uniform)
+    double kth = INFINITY; // L-th smallest s once the list is full (uniform)
+    uint32_t kid = kNone;
+    double thr = Tmax;     // candidates must have s <= thr
+    bool done = false;
+    int R = g.W / 2;
+    for (int t = 0; t < g.W && !done; t++) {
+        int d = (t & 1) ? -((t + 1) >> 1) : (t >> 1); // 0, -1, +1, -2, +2, ...
+        int r = d + R;
+        uint32_t lo = g.row_lo[(size_t)c * g.W + r], hi = g.row_hi[(size_t)c * g.W + r];
+        for (uint32_t cb = lo; cb < hi && !done; cb += kWave) {
+            uint32_t C = cb + lane;
+            double S = INFINITY;
+            if (C < hi) {
+                const double *b = g.box + 4 * (size_t)C;
+                S = box_dist2_min(px, py, b[0], b[1], b[2], b[3]);
+            }
+            uint64_t m = ballot(S <= thr);
+            while (m && !done) {
+                int bit = __builtin_ctzll(m);
+                m &= m - 1;
+                double SC = __shfl(S, bit);
+                if (SC > thr) continue;
+                uint32_t Cc = cb + bit;
+                uint32_t a = g.cell_start[Cc], e = g.cell_start[Cc + 1];
+                for (uint32_t q0 = a; q0 < e; q0 += kWave) {
+                    uint32_t q = q0 + lane;
+                    double s = INFINITY;
+                    if (q < e) s = dist2_exact(px, py, g.xs[q], g.ys[q]);
+                    bool hit = s <= thr && (cnt < L ? true : before(s, q, kth, kid));
+                    uint64_t hm = ballot(hit);
+                    while (hm) {
+                        int hb = __builtin_ctzll(hm);
+                        hm &= hm - 1;
+                        double cs = __shfl(s, hb);
+                        uint32_t cq = q0 + hb;
+                        if (cnt == L && !before(cs, cq, kth, kid)) continue;
+                        bool lt = lane < L && before(es, eid, cs, cq);
+                        int pos = popc64(ballot(lt));
+                        double ps = __shfl_up(es, 1);
+                        uint32_t pid = __shfl_up(eid, 1);
+                        if (lane == pos) {
+                            es = cs;
+                            eid = cq;
+                        } else if (lane > pos && lane < L) {
+                            es = ps;
+                            eid = pid;
+                        }
+                        if (cnt < L) cnt++;
+                        if (cnt == L) {
+                            kth = __shfl(es, L - 1);
+                            kid = __shfl(eid, L - 1);
+                            thr = kth < Tmax ? kth : Tmax;
+                        }
+                    }
+                }
+                // L neighbours within eps_0: core in every variant, list unused
+                if (cnt == L && kth <= T0) done = true;
+            }
+        }
+    }
+    if (lane < L) {
+        lst_s[(size_t)i * L + lane] = es;
+        lst_id[(size_t)i * L + lane] = eid;
+    }
+    if (lane == 0) lst_n[i] = (uint32_t)cnt;
+}
+
+// kc[p*J+j] = first level k with count_k(p) >= m_j (K = never)
+__global__ void k_core_levels(const uint32_t *lst_idx, const double *lst_s,
+                              const uint32_t *lst_n, int L, const double *Tk, int K,
+                              const uint32_t *mj, int J, uint32_t n, uint8_t *kc)
+{
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    uint32_t i = lst_idx[p];
+    int ck[kMaxLevels];
+    for (int k = 0; k < K; k++) ck[k] = L;
+    if (i != kNone) {
+        int m = (int)lst_n[i];
+        const double *s = lst_s + (size_t)i * L;
+        // count_k = entries with s <= T_k (lists hold exact binary64 s, sorted)
+        int e = 0;
+        for (int k = 0; k < K; k++) {
+            while (e < m && s[e] <= Tk[k]) e++;
+            ck[k] = e;
+        }
+    }
+    for (int j = 0; j < J; j++) {
+        uint32_t need = mj[j];
+        int lev = K;
+        if (need <= 0x7fffffffu) {
+            for (int k = 0; k < K; k++)
+                if ((uint32_t)ck[k] >= need) {
+                    lev = k;
+                    break;
+                }
+        }
+        kc[(size_t)p * J + j] = (uint8_t)lev;
+    }
+}
+
+// per cell, per j, per k: smallest rank of a point with kc_j <= k (cells list
+// their points in ascending rank, so it is the first such point), and the
+// first level at which the cell holds a core point
+__global__ void __launch_bounds__(kBlock)
+    k_cell_core(const uint32_t *cell_start, uint32_t ncell, const uint8_t *kc, int J, int K,
+                const uint32_t *pos_rank, uint32_t *cmr, uint8_t *cellkc)
+{
+    uint32_t c = blockIdx.x * kWPB + wave_id_in_block();
+    if (c >= ncell) return;
+    int lane = lane_id();
+    uint32_t a = cell_start[c], e = cell_start[c + 1];
+    for (int j = 0; j < J; j++) {
+        // first position (in the cell) with kc <= k, for every k; kc values are
+        // levels, so track the running minimum level while scanning in order
+        uint32_t first[kMaxLevels];
+        for (int k = 0; k < K; k++) first[k] = kNone;
+        int minlev = K; // levels [minlev, K) already resolved
+        for (uint32_t q0 = a; q0 < e && minlev > 0; q0 += kWave) {
+            uint32_t q = q0 + lane;
+            int lv = q < e ? (int)kc[(size_t)q * J + j] : K;
+            for (int k = 0; k < minlev; k++) {
+                uint64_t m = ballot(lv <= k);
+                if (m) first[k] = q0 + __builtin_ctzll(m);
+            }
+            // once level k is resolved every level above it is too
+            int nm = minlev;
+            for (int k = 0; k < minlev; k++)
+                if (first[k] != kNone) {
+                    nm = k;
+                    break;
+                }
+            for (int k = nm; k < minlev; k++)
+                if (first[k] == kNone) first[k] = first[nm];
+            minlev = nm;
+        }
+        if (lane == 0) {
+            for (int k = 0; k < K; k++)
+                cmr[((size_t)c * J + j) * K + k] = first[k] == kNone ? kNone : pos_rank[first[k]];
+            cellkc[(size_t)c * J + j] = (uint8_t)minlev;
+        }
+    }
+}
+
+// ------------------------------------------------------------ cell pairs
+__device__ __forceinline__ void pair_levels(const double *A, const double *B, const double *Tk,
+                                            int K, int *kmin, int *kfull)
+{
+    double gx = 0.0, gy = 0.0;
+    if (B[0] > A[1]) gx = __dsub_rn(B[0], A[1]);
+    else if (A[0] > B[1]) gx = __dsub_rn(A[0], B[1]);
+    if (B[2] > A[3]) gy = __dsub_rn(B[2], A[3]);
+    else if (A[2] > B[3]) gy = __dsub_rn(A[2], B[3]);
+    double smin = __dadd_rn(__dmul_rn(gx, gx), __dmul_rn(gy, gy));
+    double hx = fmax(__dsub_rn(A[1], B[0]), __dsub_rn(B[1], A[0]));
+    double hy = fmax(__dsub_rn(A[3], B[2]), __dsub_rn(B[3], A[2]));
+    double smax = __dadd_rn(__dmul_rn(hx, hx), __dmul_rn(hy, hy));
+    int a = K, b = K;
+    for (int k = K - 1; k >= 0; k--) {
+        if (smin <= Tk[k]) a = k;
+        if (smax <= Tk[k]) b = k;
+    }
+    *kmin = a;
+    *kfull = b;
+}
+
+template <bool WRITE>
+__global__ void k_pairs(const double *box, const uint32_t *row_lo, const uint32_t *row_hi, int W,
+                        uint32_t ncell, const double *Tk, int K, uint32_t *cnt,
+                        const uint64_t *off, uint32_t *pa, uint32_t *pb, uint16_t *pk)
+{
+    uint32_t A = blockIdx.x * kBlock + threadIdx.x;
+    if (A >= ncell) return;
+    const double *bA = box + 4 * (size_t)A;
+    uint32_t k = 0;
+    uint64_t o = WRITE ? off[A] : 0;
+    for (int r = 0; r < W; r++) {
+        uint32_t lo = row_lo[(size_t)A * W + r], hi = row_hi[(size_t)A * W + r];
+        if (lo <= A) lo = A + 1;
+        for (uint32_t B = lo; B < hi; B++) {
+            int kmin, kfull;
+            pair_levels(bA, box + 4 * (size_t)B, Tk, K, &kmin, &kfull);
+            if (kmin >= K) continue;
+            if (WRITE) {
+                pa[o + k] = A;
+                pb[o + k] = B;
+                pk[o + k] = (uint16_t)(kmin | (kfull << 8));
+            }
+            k++;
+        }
+    }
+    if (!WRITE) cnt[A] = k;
+}
+
+__global__ void k_iota(uint32_t *a, uint32_t n)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
+__global__ void k_init_parent(uint32_t *parent, uint32_t n)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) parent[i] = i;
+}
+
+// Append v to list[] for every lane with take set: one atomic per wave.
+__device__ __forceinline__ void wave_append(bool take, uint32_t v, uint32_t *list,
+                                            uint32_t *count)
+{
+    uint64_t m = ballot(take);
+    if (!m) return;
+    int leader = __builtin_ctzll(m);
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(count, (uint32_t)popc64(m));
+    base = __shfl(base, leader);
+    if (take) list[base + mask_rank(m)] = v;
+}
+
+// ------------------------------------------------- per-level pair sweep
+struct LevelArgs {
+    const double *xs, *ys, *box;
+    const uint32_t *cell_start, *cell_key;
+    uint32_t ncy;
+    const uint8_t *kc;
+    const uint8_t *cellkc;
+    const uint32_t *pa, *pb;
+    const uint16_t *pk;
+    int J, j, k;
+    double T;
+};
+
+__device__ bool small_pair_connected(const LevelArgs &a, uint32_t A, uint32_t B)
+{
+    uint32_t a0 = a.cell_start[A], a1 = a.cell_start[A + 1];
+    uint32_t b0 = a.cell_start[B], b1 = a.cell_start[B + 1];
+    for (uint32_t p = a0; p < a1; p++) {
+        if (a.kc[(size_t)p * a.J + a.j] > a.k) continue;
+        double px = a.xs[p], py = a.ys[p];
+        for (uint32_t q = b0; q < b1; q++) {
+            if (a.kc[(size_t)q * a.J + a.j] > a.k) continue;
+            if (dist2_exact(px, py, a.xs[q], a.ys[q]) <= a.T) return true;
+        }
+    }
+    return false;
+}
+
+// Thread per listed pair: settle it, keep it, or hand it to the wave test.
+// Tests of grid-adjacent cells go to `near`, the rest to `far`; the near
+// tests run first so that most far pairs are already connected when theirs run.
+__global__ void k_sweep(LevelArgs a, const uint32_t *list_in, uint32_t nin, uint32_t *parent,
+                        uint32_t *list_out, uint32_t *nout, uint32_t *near, uint32_t *nnear,
+                        uint32_t *far, uint32_t *nfar)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    bool valid = i < nin;
+    uint32_t pid = valid ? list_in[i] : 0u;
+    bool keep = false, tnear = false, tfar = false;
+    if (valid) {
+        uint32_t A = a.pa[pid], B = a.pb[pid];
+        uint16_t kk = a.pk[pid];
+        int kmin = kk & 0xff, kfull = kk >> 8;
+        if (kmin > a.k || a.cellkc[(size_t)A * a.J + a.j] > a.k ||
+            a.cellkc[(size_t)B * a.J + a.j] > a.k) {
+            keep = true; // not connectable yet
+        } else if (cf_find(parent, A) == cf_find(parent, B)) {
+            keep = false; // connected now, hence at every later level
+        } else if (kfull <= a.k) {
+            cf_unite(parent, A, B);
+        } else {
+            uint32_t na = a.cell_start[A + 1] - a.cell_start[A];
+            uint32_t nb = a.cell_start[B + 1] - a.cell_start[B];
+            if ((uint64_t)na * nb <= kSmallPairWork) {
+                if (small_pair_connected(a, A, B)) cf_unite(parent, A, B);
+                else keep = true;
+            } else {
+                uint32_t ka = a.cell_key[A], kb = a.cell_key[B];
+                int64_t dx = (int64_t)(ka / a.ncy) - (int64_t)(kb / a.ncy);
+                int64_t dy = (int64_t)(ka % a.ncy) - (int64_t)(kb % a.ncy);
+                bool adj = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1;
+                tnear = adj;
+                tfar = !adj;
+            }
+        }
+    }
+    wave_append(keep, pid, list_out, nout);
+    wave_append(tnear, pid, near, nnear);
+    wave_append(tfar, pid, far, nfar);
+}
+
+// wave per test pair: any core-core pair within eps_k connects the two cells
+__global__ void __launch_bounds__(kBlock)
+    k_test(LevelArgs a, const uint32_t *tests, uint32_t ntest, uint32_t *parent,
+           uint32_t *list_out, uint32_t *nout)
+{
+    uint32_t w = blockIdx.x * kWPB + wave_id_in_block();
+    if (w >= ntest) return;
+    uint32_t pid = tests[w];
+    uint32_t A = a.pa[pid], B = a.pb[pid];
+    uint32_t na = a.cell_start[A + 1] - a.cell_start[A];
+    uint32_t nb = a.cell_start[B + 1] - a.cell_start[B];
+    if (na > nb) { // lanes sweep the larger cell
+        uint32_t t = A;
+        A = B;
+        B = t;
+    }
+    bool found = cf_find(parent, A) == cf_find(parent, B);
+    if (!found) {
+        uint32_t a0 = a.cell_start[A], a1 = a.cell_start[A + 1];
+        uint32_t b0 = a.cell_start[B], b1 = a.cell_start[B + 1];
+        const double *bB = a.box + 4 * (size_t)B;
+        double bx0 = bB[0], bx1 = bB[1], by0 = bB[2], by1 = bB[3];
+        for (uint32_t p = a0; p < a1 && !found; p++) {
+            if (a.kc[(size_t)p * a.J + a.j] > a.k) continue;
+            double px = a.xs[p], py = a.ys[p];
+            // no point of B can be within eps of p
+            if (box_dist2_min(px, py, bx0, bx1, by0, by1) > a.T) continue;
+            for (uint32_t q0 = b0; q0 < b1; q0 += kWave) {
+                uint32_t q = q0 + lane_id();
+                bool hit = q < b1 && a.kc[(size_t)q * a.J + a.j] <= a.k &&
+                           dist2_exact(px, py, a.xs[q], a.ys[q]) <= a.T;
+                if (ballot(hit)) {
+                    found = true;
+                    break;
+                }
+            }
+        }
+        if (lane_id() == 0) {
+            if (found) cf_unite(parent, A, B);
+            else list_out[atomicAdd(nout, 1u)] = pid;
+        }
+    }
+}
+
+// ---------------------------------------------------------- snapshots
+__global__ void k_cell_roots(uint32_t *parent, uint32_t ncell)
+{
+    uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= ncell) return;
+    uint32_t x = c, p = ld_u32(parent + x);
+    while (p != x) { // read-only walk; only true roots are stored
+        x = p;
+        p = ld_u32(parent + x);
+    }
+    st_u32(parent + c, x);
+}
+
+// compmin[root] = smallest core rank of the component. Lanes of a wave that
+// share a root (the common case inside a large cluster) reduce first, so a
+// giant component takes one atomic per wave rather than one per cell.
+__global__ void k_comp_min(const uint32_t *parent, const uint32_t *cmr, const uint8_t *cellkc,
+                           uint32_t ncell, int J, int j, int K, int k, uint32_t *compmin)
+{
+    uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    bool valid = c < ncell && cellkc[(size_t)c * J + j] <= k;
+    uint32_t root = valid ? parent[c] : kNone;
+    uint32_t v = valid ? cmr[((size_t)c * J + j) * K + k] : kNone;
+    uint64_t todo = ballot(valid);
+    while (todo) {
+        int leader = __builtin_ctzll(todo);
+        uint32_t r = __shfl(root, leader);
+        bool same = valid && root == r;
+        todo &= ~ballot(same);
+        uint32_t m = same ? v : kNone;
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            uint32_t t = __shfl_xor(m, o);
+            m = t < m ? t : m;
+        }
+        if (lane_id() == leader && m < ld_u32(compmin + r)) atomicMin(compmin + r, m);
+    }
+}
+
+__global__ void k_root_flags_cells(const uint32_t *parent, const uint32_t *compmin,
+                                   const uint8_t *cellkc, uint32_t ncell, int J, int j, int k,
+                                   uint32_t *flag)
+{
+    uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= ncell) return;
+    if (cellkc[(size_t)c * J + j] > k || parent[c] != c) return;
+    flag[compmin[c]] = 1u;
+}
+
+__global__ void k_batch_labels(const uint32_t *cell_of_pos, const uint32_t *parent,
+                               const uint32_t *compmin, const uint32_t *cid,
+                               const uint8_t *kc, int J, int j, int k, double T,
+                               const uint32_t *lst_idx, const double *lst_s,
+                               const uint32_t *lst_id, const uint32_t *lst_n, int L,
+                               const uint32_t *pos_rank, const uint32_t *mapping, uint32_t n,
+                               uint32_t *out)
+{
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    uint32_t r = pos_rank[p];
+    uint32_t lbl = 0;
+    if (kc[(size_t)p * J + j] <= k) {
+        lbl = cid[compmin[parent[cell_of_pos[p]]]] + 1;
+    } else {
+        // non-core: count_k(p) < m_j <= L, so the list holds every eps_k
+        // neighbour of p, as the prefix with s <= T_k
+        uint32_t i = lst_idx[p];
+        uint32_t best = kNone;
+        if (i != kNone) {
+            uint32_t m = lst_n[i];
+            const double *s = lst_s + (size_t)i * L;
+            const uint32_t *id = lst_id + (size_t)i * L;
+            for (uint32_t e = 0; e < m && s[e] <= T; e++) {
+                uint32_t q = id[e];
+                if (kc[(size_t)q * J + j] > k) continue;
+                uint32_t rr = compmin[parent[cell_of_pos[q]]];
+                best = rr < best ? rr : best;
+            }
+        }
+        lbl = (best != kNone && best < r) ? cid[best] + 1 : 0u;
+    }
+    out[mapping[r]] = lbl;
+}
+
+// variants with no neighbour at all (eps < 0 or NaN): core iff minpts == 0,
+// and then every point is its own cluster, numbered in binned order
+__global__ void k_isolated_labels(const uint32_t *mapping, uint32_t n, bool all_core,
+                                  uint32_t *out)
+{
+    uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    out[mapping[r]] = all_core ? r + 1 : 0u;
+}
+
+struct Ev {
+    hipEvent_t a, b;
+    hipStream_t s;
+    explicit Ev(hipStream_t st) : s(st)
+    {
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        hipEventRecord(a, s);
+    }
+    ~Ev()
+    {
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+    }
+    double ms()
+    {
+        hipEventRecord(b, s);
+        hipEventSynchronize(b);
+        float m = 0;
+        hipEventElapsedTime(&m, a, b);
+        return m;
+    }
+};
+
+int read_u32(Context &c, const uint32_t *d, uint32_t *h)
+{
+    void *hs;
+    VDB_TRY(c.host_scratch(sizeof(uint32_t), &hs));
+    VDB_HIP(hipMemcpyAsync(hs, d, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream()));
+    VDB_HIP(hipStreamSynchronize(c.stream()));
+    *h = *(uint32_t *)hs;
+    return kOk;
+}
+
+} // namespace
+
+bool batchable(double eps, uint32_t minpts)
+{
+    bool mp_ok = minpts <= kBatchMaxMinpts || minpts > 0x7fffffffu;
+    bool eps_ok = (eps > 0.0 && std::isfinite(eps)) || threshold_for_eps(eps) < 0;
+    return mp_ok && eps_ok;
+}
+
+int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t *minpts,
+                  const uint32_t *rows, uint32_t nvar, uint32_t *out, vdb_stats *st,
+                  bool verbose, BatchResult *res)
+{
+    // every distance decision of this engine is binary64
+    uint32_t n = P.n;
+    // ---- levels and minpts values
+    std::vector<double> lev;
+    std::vector<uint32_t> mps;
+    uint32_t L = 0;
+    for (uint32_t v = 0; v < nvar; v++) {
+        double T = threshold_for_eps(eps[v]);
+        if (T < 0) continue; // isolated-point variants are handled below
+        if (!(eps[v] > 0.0) || !std::isfinite(eps[v])) {
+            set_error("batched engine needs finite eps > 0");
+            return kErrUnsupported;
+        }
+        lev.push_back(eps[v]);
+        mps.push_back(minpts[v]);
+        if (minpts[v] <= 0x7fffffffu) L = std::max(L, minpts[v]);
+    }
+    std::sort(lev.begin(), lev.end());
+    lev.erase(std::unique(lev.begin(), lev.end()), lev.end());
+    std::sort(mps.begin(), mps.end());
+    mps.erase(std::unique(mps.begin(), mps.end()), mps.end());
+    int K = (int)lev.size(), J = (int)mps.size();
+    if (L > (uint32_t)kMaxList || K > kMaxLevels - 1 || J > kMaxMinpts) {
+        set_error("batched engine handles minpts <= 64 and at most 63 eps levels");
+        return kErrUnsupported;
+    }
+    if (L == 0) L = 1;
+    if (K > 0 && lev.back() / lev.front() > kBatchEpsRatio) {
+        set_error("eps range too wide for one fine grid");
+        return kErrUnsupported;
+    }
+    Ev total(c.stream());
+    // isolated-point variants
+    for (uint32_t v = 0; v < nvar; v++) {
+        if (threshold_for_eps(eps[v]) >= 0) continue;
+        bool all_core = minpts[v] == 0;
+        k_isolated_labels<<<ceil_div(n, kBlock), kBlock, 0, c.stream()>>>(
+            P.mapping, n, all_core, out + (size_t)rows[v] * n);
+        VDB_HIP(hipGetLastError());
+    }
+    if (K == 0) return kOk;
+    std::vector<double> Th(K);
+    for (int k = 0; k < K; k++) Th[k] = threshold_for_eps(lev[k]);
+
+    // ---- fine grid with clique cells at eps_0
+    Ev tprep(c.stream());
+    double d = lev[0] / std::sqrt(2.0) * (1.0 - std::ldexp(1.0, -20));
+    int R = (int)std::ceil(lev[K - 1] * (1.0 + std::ldexp(1.0, -30)) / d);
+    Grid G;
+    bool enlarged = false;
+    VDB_TRY(build_grid(c, P, d, R, "gb", G, &enlarged));
+    if (enlarged) {
+        set_error("fine grid would exceed 32-bit cell keys");
+        return kErrUnsupported;
+    }
+    uint32_t *bad;
+    VDB_TRY(c.get("b.bad", 1, &bad));
+    VDB_HIP(hipMemsetAsync(bad, 0, sizeof(uint32_t), c.stream()));
+    k_clique_check<<<ceil_div(G.ncell, kBlock), kBlock, 0, c.stream()>>>(G.cell_box, G.ncell,
+                                                                         Th[0], bad);
+    VDB_HIP(hipGetLastError());
+    uint32_t hbad = 0;
+    VDB_TRY(read_u32(c, bad, &hbad));
+    if (hbad) {
+        set_error("fine-grid cells are not cliques at the smallest eps");
+        return kErrUnsupported;
+    }
+    double *dTk;
+    uint32_t *dmj;
+    VDB_TRY(c.get("b.Tk", K, &dTk));
+    VDB_TRY(c.get("b.mj", J, &dmj));
+    VDB_HIP(hipMemcpyAsync(dTk, Th.data(), sizeof(double) * K, hipMemcpyHostToDevice,
+                           c.stream()));
+    VDB_HIP(hipMemcpyAsync(dmj, mps.data(), sizeof(uint32_t) * J, hipMemcpyHostToDevice,
+                           c.stream()));
+    if (st) st->ms_prepare += tprep.ms();
+
+    // ---- truncated distance-sorted table
+    Ev tknn(c.stream());
+    uint32_t *flag, *excl, *list_pos, *lst_idx;
+    VDB_TRY(c.get("b.flag", n, &flag));
+    VDB_TRY(c.get("b.excl", n, &excl));
+    VDB_TRY(c.get("b.lst_idx", n, &lst_idx));
+    unsigned gn = ceil_div(n, kBlock);
+    k_list_flags<<<gn, kBlock, 0, c.stream()>>>(G.cell_start, G.cell_of_pos, n, L, flag);
+    VDB_HIP(hipGetLastError());
+    uint32_t nlist = 0;
+    VDB_TRY(compact_count(c, flag, excl, n, &nlist));
+    VDB_TRY(c.get("b.list_pos", nlist ? nlist : 1, &list_pos));
+    k_list_index<<<gn, kBlock, 0, c.stream()>>>(flag, excl, n, list_pos, lst_idx);
+    VDB_HIP(hipGetLastError());
+    double *lst_s;
+    uint32_t *lst_id, *lst_n;
+    size_t lsz = (size_t)(nlist ? nlist : 1) * L;
+    VDB_TRY(c.get("b.lst_s", lsz, &lst_s));
+    VDB_TRY(c.get("b.lst_id", lsz, &lst_id));
+    VDB_TRY(c.get("b.lst_n", nlist ? nlist : 1, &lst_n));
+    if (nlist)
+        k_knn<<<ceil_div(nlist, kWPB), kBlock, 0, c.stream()>>>(fview(G), list_pos, nlist,
+                                                                (int)L, Th[K - 1], Th[0], lst_s,
+                                                                lst_id, lst_n);
+    VDB_HIP(hipGetLastError());
+    uint8_t *kc, *cellkc;
+    uint32_t *cmr;
+    VDB_TRY(c.get("b.kc", (size_t)n * J, &kc));
+    k_core_levels<<<gn, kBlock, 0, c.stream()>>>(lst_idx, lst_s, lst_n, (int)L, dTk, K, dmj, J,
+                                                 n, kc);
+    VDB_HIP(hipGetLastError());
+    VDB_TRY(c.get("b.cmr", (size_t)G.ncell * J * K, &cmr));
+    VDB_TRY(c.get("b.cellkc", (size_t)G.ncell * J, &cellkc));
+    k_cell_core<<<ceil_div(G.ncell, kWPB), kBlock, 0, c.stream()>>>(G.cell_start, G.ncell, kc, J,
+                                                                    K, G.pos_rank, cmr, cellkc);
+    VDB_HIP(hipGetLastError());
+    if (st) st->ms_count += tknn.ms();
+
+    // ---- cell pairs within eps_max
+    Ev tpairs(c.stream());
+    uint32_t *pcnt;
+    uint64_t *poff;
+    VDB_TRY(c.get("b.pcnt", G.ncell, &pcnt));
+    VDB_TRY(c.get("b.poff", (size_t)G.ncell + 1, &poff));
+    unsigned gc = ceil_div(G.ncell, kBlock);
+    int W = 2 * R + 1;
+    k_pairs<false><<<gc, kBlock, 0, c.stream()>>>(G.cell_box, G.row_lo, G.row_hi, W, G.ncell,
+                                                  dTk, K, pcnt, nullptr, nullptr, nullptr,
+                                                  nullptr);
+    VDB_HIP(hipGetLastError());
+    VDB_TRY(scan_exclusive_u32_u64(c, pcnt, poff, G.ncell, poff + G.ncell));
+    uint64_t npairs64 = 0;
+    {
+        void *hs;
+        VDB_TRY(c.host_scratch(8, &hs));
+        VDB_HIP(hipMemcpyAsync(hs, poff + G.ncell, 8, hipMemcpyDeviceToHost, c.stream()));
+        VDB_HIP(hipStreamSynchronize(c.stream()));
+        npairs64 = *(uint64_t *)hs;
+    }
+    if (npairs64 >= 0xffffffffull) {
+        set_error("too many cell pairs");
+        return kErrUnsupported;
+    }
+    uint32_t npairs = (uint32_t)npairs64;
+    uint32_t *pa, *pb, *la, *lb, *tests, *tests2, *cnts;
+    uint16_t *pk;
+    size_t psz = npairs ? npairs : 1;
+    VDB_TRY(c.get("b.pa", psz, &pa));
+    VDB_TRY(c.get("b.pb", psz, &pb));
+    VDB_TRY(c.get("b.pk", psz, &pk));
+    VDB_TRY(c.get("b.la", psz, &la));
+    VDB_TRY(c.get("b.lb", psz, &lb));
+    VDB_TRY(c.get("b.tests", psz, &tests));
+    VDB_TRY(c.get("b.tests2", psz, &tests2));
+    VDB_TRY(c.get("b.cnts", 4, &cnts));
+    k_pairs<true><<<gc, kBlock, 0, c.stream()>>>(G.cell_box, G.row_lo, G.row_hi, W, G.ncell,
+                                                 dTk, K, nullptr, poff, pa, pb, pk);
+    VDB_HIP(hipGetLastError());
+    if (st) st->ms_prepare += tpairs.ms();
+
+    // ---- variants of each (k, j)
+    std::map<std::pair<int, int>, std::vector<uint32_t>> want;
+    for (uint32_t v = 0; v < nvar; v++) {
+        if (threshold_for_eps(eps[v]) < 0) continue;
+        int k = (int)(std::lower_bound(lev.begin(), lev.end(), eps[v]) - lev.begin());
+        int j = (int)(std::lower_bound(mps.begin(), mps.end(), minpts[v]) - mps.begin());
+        want[{k, j}].push_back(v);
+    }
+    uint32_t *parent, *compmin, *rflag, *cid;
+    VDB_TRY(c.get("b.parent", G.ncell, &parent));
+    VDB_TRY(c.get("b.compmin", G.ncell, &compmin));
+    VDB_TRY(c.get("b.rflag", n, &rflag));
+    VDB_TRY(c.get("b.cid", n, &cid));
+    Ev tuf(c.stream());
+    double ms_labels = 0;
+    for (int j = 0; j < J; j++) {
+        // minpts values only reached by variants of lower levels still need
+        // the sweep up to their highest requested level
+        int kmax = -1;
+        for (auto &w : want)
+            if (w.first.second == j) kmax = std::max(kmax, w.first.first);
+        if (kmax < 0) continue;
+        k_init_parent<<<gc, kBlock, 0, c.stream()>>>(parent, G.ncell);
+        VDB_HIP(hipGetLastError());
+        uint32_t nin = npairs;
+        uint32_t *lin = la, *lout = lb;
+        if (nin) k_iota<<<ceil_div(nin, kBlock), kBlock, 0, c.stream()>>>(lin, nin);
+        VDB_HIP(hipGetLastError());
+        for (int k = 0; k <= kmax; k++) {
+            LevelArgs a{G.xs,       G.ys,   G.cell_box, G.cell_start, G.cell_key, G.ncy, kc,
+                        cellkc,     pa,     pb,         pk,           J,          j,     k,
+                        Th[k]};
+            VDB_HIP(hipMemsetAsync(cnts, 0, 4 * sizeof(uint32_t), c.stream()));
+            if (nin) {
+                k_sweep<<<ceil_div(nin, kBlock), kBlock, 0, c.stream()>>>(
+                    a, lin, nin, parent, lout, cnts, tests, cnts + 1, tests2, cnts + 2);
+                VDB_HIP(hipGetLastError());
+                uint32_t h3[3];
+                {
+                    void *hs;
+                    VDB_TRY(c.host_scratch(3 * sizeof(uint32_t), &hs));
+                    VDB_HIP(hipMemcpyAsync(hs, cnts, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                           c.stream()));
+                    VDB_HIP(hipStreamSynchronize(c.stream()));
+                    memcpy(h3, hs, sizeof h3);
+                }
+                if (h3[1]) {
+                    k_test<<<ceil_div(h3[1], kWPB), kBlock, 0, c.stream()>>>(a, tests, h3[1],
+                                                                            parent, lout, cnts);
+                    VDB_HIP(hipGetLastError());
+                }
+                if (h3[2]) {
+                    k_test<<<ceil_div(h3[2], kWPB), kBlock, 0, c.stream()>>>(a, tests2, h3[2],
+                                                                            parent, lout, cnts);
+                    VDB_HIP(hipGetLastError());
+                }
+                VDB_TRY(read_u32(c, cnts, &nin));
+                std::swap(lin, lout);
+            }
+            auto it = want.find({k, j});
+            if (it == want.end()) continue;
+            Ev tl(c.stream());
+            k_cell_roots<<<gc, kBlock, 0, c.stream()>>>(parent, G.ncell);
+            VDB_HIP(hipGetLastError());
+            VDB_HIP(hipMemsetAsync(compmin, 0xff, sizeof(uint32_t) * G.ncell, c.stream()));
+            k_comp_min<<<gc, kBlock, 0, c.stream()>>>(parent, cmr, cellkc, G.ncell, J, j, K, k,
+                                                      compmin);
+            VDB_HIP(hipGetLastError());
+            VDB_HIP(hipMemsetAsync(rflag, 0, sizeof(uint32_t) * n, c.stream()));
+            k_root_flags_cells<<<gc, kBlock, 0, c.stream()>>>(parent, compmin, cellkc, G.ncell,
+                                                              J, j, k, rflag);
+            VDB_HIP(hipGetLastError());
+            VDB_TRY(scan_exclusive_u32(c, rflag, cid, n, nullptr));
+            for (uint32_t v : it->second) {
+                k_batch_labels<<<gn, kBlock, 0, c.stream()>>>(
+                    G.cell_of_pos, parent, compmin, cid, kc, J, j, k, Th[k], lst_idx, lst_s,
+                    lst_id, lst_n, (int)L, G.pos_rank, P.mapping, n, out + (size_t)rows[v] * n);
+                VDB_HIP(hipGetLastError());
+            }
+            ms_labels += tl.ms();
+        }
+    }
+    double ms_uf_all = tuf.ms();
+    if (st) {
+        st->ms_uf += ms_uf_all - ms_labels;
+        st->ms_labels += ms_labels;
+    }
+    if (res) {
+        res->levels = (uint32_t)K;
+        res->minpts_values = (uint32_t)J;
+        res->list_len = L;
+        res->listed_points = nlist;
+        res->cell_pairs = npairs;
+    }
+    if (verbose)
+        printf("\n[vdbscan] batch: %d eps levels x %d minpts, fine grid %u cells (R=%d), "
+               "%u listed points (L=%u), %u cell pairs, %.3f ms",
+               K, J, G.ncell, R, nlist, L, npairs, total.ms());
+    return kOk;
 }
 
 } // namespace vdb

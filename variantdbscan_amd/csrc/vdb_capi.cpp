@@ -77,11 +77,16 @@ size_t default_budget(const vdb_options *opt)
 
 // per-variant paths: one grid per distinct eps, counts shared by its variants
 int run_per_variant(Context &c, Points &P, const double *eps, const uint32_t *minpts,
-                    uint32_t nvar, uint32_t *labels_dev, int mode, int precision,
-                    size_t budget, vdb_stats *st, bool verbose)
+                    const std::vector<uint32_t> &which, uint32_t *labels_dev, int mode,
+                    int precision, size_t budget, vdb_stats *st, bool verbose)
 {
-    std::map<double, std::vector<uint32_t>> by_eps;
-    for (uint32_t v = 0; v < nvar; v++) by_eps[eps[v]].push_back(v);
+    // group by eps (NaN eps compare unequal to themselves: key them by bits)
+    std::map<uint64_t, std::vector<uint32_t>> by_eps;
+    for (uint32_t v : which) {
+        uint64_t bits;
+        memcpy(&bits, &eps[v], 8);
+        by_eps[bits].push_back(v);
+    }
     uint32_t *counts;
     VDB_TRY(c.get("pv.counts", P.n, &counts));
     unsigned long long *rc;
@@ -89,7 +94,7 @@ int run_per_variant(Context &c, Points &P, const double *eps, const uint32_t *mi
     VDB_HIP(hipMemsetAsync(rc, 0, sizeof(unsigned long long), c.stream()));
     PhaseTimes pt;
     for (auto &kv : by_eps) {
-        double e = kv.first;
+        double e = eps[kv.second[0]];
         Grid G;
         Items I;
         bool enl = false;
@@ -153,19 +158,58 @@ int cluster_impl(const double *x, const double *y, uint32_t n, const double *eps
     uint32_t *out = labels;
     if (!(flags & VDB_DEVICE_PTRS)) VDB_TRY(c.get("out.labels", (size_t)n * nvar, &out));
     int used = path;
+    // variants left for the per-variant engine (all of them unless batched)
+    std::vector<uint32_t> rest;
     if (path == VDB_PATH_AUTO || path == VDB_PATH_BATCH) {
-        BatchResult br;
-        int s = cluster_batch(c, P, eps, minpts, nvar, precision, out, st, verbose, &br);
-        if (s == kOk) {
-            used = VDB_PATH_BATCH;
-        } else if (s == kErrUnsupported && path == VDB_PATH_AUTO) {
-            used = VDB_PATH_FUSED;
-        } else {
-            return s;
+        // batchable variants, grouped so that max eps / min eps stays within
+        // the fine grid's window; each group is one batched run
+        std::vector<uint32_t> bat;
+        for (uint32_t v = 0; v < nvar; v++)
+            (batchable(eps[v], minpts[v]) ? bat : rest).push_back(v);
+        std::sort(bat.begin(), bat.end(), [&](uint32_t a, uint32_t b) {
+            bool ia = threshold_for_eps(eps[a]) < 0, ib = threshold_for_eps(eps[b]) < 0;
+            if (ia != ib) return ia;
+            return eps[a] < eps[b] || (eps[a] == eps[b] && a < b);
+        });
+        size_t g0 = 0;
+        bool any_batched = false;
+        while (g0 < bat.size()) {
+            size_t g1 = g0;
+            double lo = -1.0;
+            while (g1 < bat.size()) {
+                double e = eps[bat[g1]];
+                if (threshold_for_eps(e) >= 0) {
+                    if (lo < 0) lo = e;
+                    else if (e / lo > kBatchEpsRatio) break;
+                }
+                g1++;
+            }
+            std::vector<double> ge;
+            std::vector<uint32_t> gm, gr;
+            for (size_t i = g0; i < g1; i++) {
+                ge.push_back(eps[bat[i]]);
+                gm.push_back(minpts[bat[i]]);
+                gr.push_back(bat[i]);
+            }
+            BatchResult br;
+            int s = cluster_batch(c, P, ge.data(), gm.data(), gr.data(), (uint32_t)ge.size(), out,
+                                  st, verbose, &br);
+            if (s == kOk) {
+                any_batched = true;
+            } else if (s == kErrUnsupported) {
+                rest.insert(rest.end(), gr.begin(), gr.end());
+            } else {
+                return s;
+            }
+            g0 = g1;
         }
+        used = any_batched ? VDB_PATH_BATCH : VDB_PATH_FUSED;
+    } else {
+        for (uint32_t v = 0; v < nvar; v++) rest.push_back(v);
     }
-    if (used == VDB_PATH_FUSED || used == VDB_PATH_TABLE) {
-        VDB_TRY(run_per_variant(c, P, eps, minpts, nvar, out, used == VDB_PATH_TABLE ? 1 : 0,
+    if (!rest.empty()) {
+        std::sort(rest.begin(), rest.end());
+        VDB_TRY(run_per_variant(c, P, eps, minpts, rest, out, path == VDB_PATH_TABLE ? 1 : 0,
                                 precision, default_budget(opt), st, verbose));
     }
     if (!(flags & VDB_DEVICE_PTRS)) {

@@ -323,11 +323,25 @@ __global__ void __launch_bounds__(kBlock)
     });
 }
 
+// Root of x without path-halving stores. Used once every union is done:
+// concurrent threads then only ever store true roots, so no thread can put
+// back an intermediate ancestor over a root another thread just wrote (which
+// a halving find could, from a stale read of its path).
+__device__ uint32_t uf_root(const uint32_t *parent, uint32_t x)
+{
+    uint32_t p = ld_par(parent + x);
+    while (p != x) {
+        x = p;
+        p = ld_par(parent + x);
+    }
+    return x;
+}
+
 __global__ void k_compress(uint32_t *parent, const uint8_t *core, uint32_t n)
 {
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= n || !core[p]) return;
-    st_par(parent + p, uf_find(parent, p));
+    st_par(parent + p, uf_root(parent, p));
 }
 
 // border: best[p] = smallest root rank among core neighbours of non-core p

@@ -107,7 +107,7 @@ __global__ void k_cell_tail(uint32_t *cell_start, uint32_t ncell, uint32_t n)
 
 // bbox of each cell from its points; one wave per cell
 __global__ void k_cell_box(const double *xs, const double *ys, const uint32_t *cell_start,
-                           uint32_t ncell, double *box, uint32_t *max_pop)
+                           uint32_t ncell, double *box)
 {
     uint32_t c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     if (c >= ncell) return;
@@ -132,7 +132,28 @@ __global__ void k_cell_box(const double *xs, const double *ys, const uint32_t *c
         box[4 * (size_t)c + 1] = x1;
         box[4 * (size_t)c + 2] = y0;
         box[4 * (size_t)c + 3] = y1;
-        atomicMax(max_pop, b - a);
+    }
+}
+
+// largest cell population: block-level maximum, one atomic per block
+__global__ void __launch_bounds__(kBlock)
+    k_max_pop(const uint32_t *cell_start, uint32_t ncell, uint32_t *max_pop)
+{
+    uint32_t m = 0;
+    for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < ncell; c += gridDim.x * kBlock) {
+        uint32_t p = cell_start[c + 1] - cell_start[c];
+        m = p > m ? p : m;
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        uint32_t t = __shfl_xor(m, o);
+        m = t > m ? t : m;
+    }
+    __shared__ uint32_t sm[kBlock / kWave];
+    if (lane_id() == 0) sm[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / kWave; w++) m = sm[w] > m ? sm[w] : m;
+        atomicMax(max_pop, m);
     }
 }
 
@@ -310,7 +331,10 @@ int build_grid(Context &c, const Points &P, double w_min, int R, const char *tag
     VDB_HIP(hipMemsetAsync(maxpop, 0, sizeof(uint32_t), c.stream()));
     VDB_TRY(c.get(slot(tag, "cell_box").c_str(), (size_t)ncell * 4, &G.cell_box));
     k_cell_box<<<ceil_div(ncell, kBlock / kWave), kBlock, 0, c.stream()>>>(
-        G.xs, G.ys, G.cell_start, ncell, G.cell_box, maxpop);
+        G.xs, G.ys, G.cell_start, ncell, G.cell_box);
+    VDB_HIP(hipGetLastError());
+    k_max_pop<<<std::min<unsigned>(ceil_div(ncell, kBlock), 512), kBlock, 0, c.stream()>>>(
+        G.cell_start, ncell, maxpop);
     VDB_HIP(hipGetLastError());
 
     size_t W = 2 * (size_t)R + 1;
