@@ -1,0 +1,277 @@
+#!/usr/bin/env python3
+"""Benchmark of the VariantDBSCAN hot path on one MI355X.
+
+Headline (`value`): variant-clusterings per second, with exact labels, of the
+batched engine on 2,000,000 points x 64 variants (eps 0.2..0.9 step 0.1 x
+minpts {4,6,8,12,16,24,32,48}); points and labels resident in HBM; one step =
+one call clustering all 64 variants.
+
+Also measured in the same run (each in its own timed region):
+  * `roofline`: the CSR neighbour-table build (count, scan, fill) at eps 0.5 on
+    the same points, the table kernels' achieved algorithmic GB/s against
+    8 TB/s; the dominant table kernel is the fill.
+  * `plain_fp64`: the plain per-variant FP64 path (grid, count/scan/fill
+    table, union-find over it, labels), one variant per eps level, scaled to
+    variants/s; `ratio_vs_plain` = value / that.
+  * `cpu_baseline`: the reference library (compiled from its sources into
+    oracle/_ref) on a 100,000-point random subsample of the same points, all
+    64 variants in one call, on the host's cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+       [--no-table] [--no-plain] [--only batch|table]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "variant-clusterings/sec at exact labels, 2M pts×64 variants; neighbor-table HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def table_bytes(n, pairs, k_cuts=1, with_dist=False):
+    """Algorithmic bytes of one CSR neighbour-table build:
+    16N read points + 4N*K counts + 16N scan (counts in, int64 offsets out)
+    + 24N fill (points + offsets re-read) + P*(4 [+4]) ids (and distances)."""
+    return 16 * n + 4 * n * k_cuts + 16 * n + 24 * n + pairs * (4 + (4 if with_dist else 0))
+
+
+def fill_bytes(n, pairs):
+    """Algorithmic bytes of the fill kernel alone: 24N in + 4P ids out."""
+    return 24 * n + 4 * pairs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-table", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--only", choices=["batch", "table"], default=None)
+    ap.add_argument("--table-steps", type=int, default=2)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a 1 GiB device copy in the table leg (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import variantdbscan_amd as vd
+    from variantdbscan_amd import datasets
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    L = vd._lib.load()
+    stream = torch.cuda.current_stream()
+    L.vdb_set_stream(stream.cuda_stream)
+
+    x, y = datasets.aniso_2m()
+    n = len(x)
+    E, M = datasets.BATCH64_EPS, datasets.BATCH64_MINPTS
+    eps = np.array([e for e in E for m in M])
+    mp = np.array([m for e in E for m in M], dtype=np.uint32)
+    V = len(eps)
+    dx = torch.from_numpy(x).to(dev)
+    dy = torch.from_numpy(y).to(dev)
+    out = {}
+
+    # ------------------------------------------------------------ batch
+    if args.only in (None, "batch"):
+        labels = torch.empty(V * n, dtype=torch.int32, device=dev)
+        log("batch: warmup %d" % args.warmup)
+        for _ in range(args.warmup):
+            vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, eps, mp, labels.data_ptr(),
+                              path=vd.PATH_BATCH)
+        torch.cuda.synchronize()
+        log("batch: timing %d steps" % args.steps)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        phases = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            st = vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, eps, mp, labels.data_ptr(),
+                                   path=vd.PATH_BATCH)
+            phases.append(st)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms_step = ev0.elapsed_time(ev1) / args.steps
+        assert all(p["path_used"] == vd.PATH_BATCH for p in phases)
+        value = V / (ms_step / 1e3)
+        avg = {k: float(np.mean([p[k] for p in phases])) for k in
+               ("ms_total", "ms_prepare", "ms_count", "ms_uf", "ms_labels")}
+        out.update(value=value, ms_per_step=ms_step, wall_s=wall, batch_phases_ms=avg)
+        log("batch: %.3f ms/step -> %.1f variants/s" % (ms_step, value))
+
+        # spot check against the per-variant engine on two variants
+        chk = [0, V - 1]
+        ref = torch.empty(len(chk) * n, dtype=torch.int32, device=dev)
+        vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, eps[chk], mp[chk], ref.data_ptr(),
+                          path=vd.PATH_FUSED)
+        lab = labels.view(V, n)[chk]
+        out["labels_match_per_variant_engine"] = bool(torch.equal(lab, ref.view(len(chk), n)))
+        del labels, ref
+        L.vdb_release()
+        torch.cuda.empty_cache()
+
+    # --------------------------------------------- neighbour-table roofline
+    if args.only in (None, "table") and not args.no_table:
+        # the table leg reads the device time of each kernel phase from HIP
+        # events the library records on this stream
+        if args.calibrate_copy:
+            # plain copy of known size: 1 GiB read + 1 GiB written
+            ca = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+            cb = torch.empty_like(ca)
+            cb.copy_(ca)
+            torch.cuda.synchronize()
+            del ca, cb
+        lab1 = torch.empty(n, dtype=torch.int32, device=dev)
+        tphases = []
+        for i in range(args.table_steps):
+            st = vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, [0.5], [8], lab1.data_ptr(),
+                                   path=vd.PATH_TABLE)
+            tphases.append(st)
+            log("table: step %d count %.2f scan %.2f fill %.2f ms, %d pairs, %d tile(s)" %
+                (i, st["ms_count"], st["ms_scan"], st["ms_fill"], st["pairs"], st["tiles"]))
+        torch.cuda.synchronize()
+        t = tphases[-1] if len(tphases) == 1 else tphases[1]
+        P = int(t["pairs"])
+        fill_ms = float(np.mean([p["ms_fill"] for p in tphases[1:] or tphases]))
+        cnt_ms = float(np.mean([p["ms_count"] for p in tphases[1:] or tphases]))
+        scan_ms = float(np.mean([p["ms_scan"] for p in tphases[1:] or tphases]))
+        fb = fill_bytes(n, P)
+        achieved = fb / (fill_ms / 1e3) / 1e9
+        tb = table_bytes(n, P)
+        tab_gbs = tb / ((cnt_ms + scan_ms + fill_ms) / 1e3) / 1e9
+        # HBM bytes per launch from the calibrated FETCH_SIZE/WRITE_SIZE passes
+        # recorded under profiles/ for this kernel and workload
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", "hbm_counters_k_fill.json")
+        if os.path.exists(tf):
+            traffic = json.load(open(tf)).get("traffic_bytes")
+        out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                           "traffic_source": "profiles/hbm_counters_k_fill.md",
+                           "kernel": "k_fill<0> (CSR neighbour-table fill, FP64)",
+                           "workload": "2M points, eps 0.5, %d pairs, %d tile(s)" %
+                           (P, t["tiles"]),
+                           "algorithmic_bytes": fb, "ms_per_launch": fill_ms}
+        out["table_build"] = {"count_ms": cnt_ms, "scan_ms": scan_ms, "fill_ms": fill_ms,
+                              "pairs": P, "algorithmic_bytes": tb, "achieved_gbs": tab_gbs,
+                              "frac": tab_gbs / HBM_PEAK_GBS}
+        del lab1
+        L.vdb_release()
+        torch.cuda.empty_cache()
+
+    # ------------------------------------------------ plain per-variant path
+    if args.only is None and not args.no_plain:
+        # one variant per eps level (minpts 8, the cost is set by eps)
+        lab1 = torch.empty(n, dtype=torch.int32, device=dev)
+        tot_ms = 0.0
+        per = []
+        # largest eps first: its table sizes the pooled buffer the others reuse
+        for e in sorted(E, reverse=True):
+            torch.cuda.synchronize()
+            ea = torch.cuda.Event(enable_timing=True)
+            eb = torch.cuda.Event(enable_timing=True)
+            ea.record(stream)
+            vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, [e], [8], lab1.data_ptr(),
+                              path=vd.PATH_TABLE)
+            eb.record(stream)
+            torch.cuda.synchronize()
+            ms = ea.elapsed_time(eb)
+            per.append(ms)
+            tot_ms += ms
+            log("plain fp64 table path eps %.1f: %.1f ms" % (e, ms))
+        L.vdb_release()
+        torch.cuda.empty_cache()
+        plain_vps = len(E) / (tot_ms / 1e3)
+        per = per[::-1]
+        out["plain_fp64"] = {"variants_per_s": plain_vps, "ms_per_eps_level": per,
+                             "sample": "minpts 8 at each of the 8 eps levels"}
+        if "value" in out:
+            out["ratio_vs_plain"] = out["value"] / plain_vps
+        del lab1
+
+    # ----------------------------------------------------------- CPU leg
+    if args.only is None and not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = 16 if oracle.ref_available(16) else 8 if oracle.ref_available(8) else 1
+        if oracle.ref_available(threads):
+            rng = np.random.default_rng(0)
+            idx = np.sort(rng.choice(n, 100_000, replace=False))
+            xs, ys = x[idx], y[idx]
+            log("cpu: reference on %d points x %d variants, %d threads" % (len(xs), V, threads))
+            _, sec = oracle.run_reference(xs, ys, eps, mp, threads=threads, timeout=600)
+            # the GPU engine on the same sample, for a like-for-like line
+            gx = torch.from_numpy(xs).to(dev)
+            gy = torch.from_numpy(ys).to(dev)
+            gl = torch.empty(V * len(xs), dtype=torch.int32, device=dev)
+            vd.cluster_device(gx.data_ptr(), gy.data_ptr(), len(xs), eps, mp, gl.data_ptr(),
+                              path=vd.PATH_BATCH)
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(3):
+                vd.cluster_device(gx.data_ptr(), gy.data_ptr(), len(xs), eps, mp, gl.data_ptr(),
+                                  path=vd.PATH_BATCH)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            gms = e0.elapsed_time(e1) / 3
+            out["cpu_baseline"] = {
+                "value": V / sec, "unit": "variant-clusterings/s", "cores": threads,
+                "kind": "reference",
+                "sample": "100,000-point random subsample of the 2M points, all 64 variants "
+                          "in one libVDBSCAN call (reference OpenMP R-tree build, "
+                          "NSEARCHTHREADS=%d, parallel across variants)" % threads,
+                "seconds": sec, "gpu_same_sample_variants_per_s": V / (gms / 1e3)}
+        else:
+            out["cpu_baseline"] = None
+
+    line = {
+        "metric": METRIC,
+        "value": out.get("value"),
+        "unit": "variant-clusterings/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": out.get("ms_per_step"),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator, variantdbscan_amd/datasets.py)",
+        "config": {"workload": "aniso_2m: 2,000,000 points (60% from 200 anisotropic blobs, "
+                               "40% uniform) x 64 variants (eps 0.2..0.9 x minpts 4..48), "
+                               "batched engine, inputs and labels in HBM",
+                   "n_points": n, "variants": V, "eps": E, "minpts": M},
+        "roofline": out.get("roofline"),
+        "cpu_baseline": out.get("cpu_baseline"),
+    }
+    for k in ("ratio_vs_plain", "plain_fp64", "table_build", "batch_phases_ms",
+              "labels_match_per_variant_engine", "wall_s"):
+        if k in out:
+            line[k] = out[k]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -43,6 +43,8 @@ constexpr int kMaxLevels = 64;
 constexpr int kMaxMinpts = 64;
 // pairs whose cells hold at most this many point pairs are tested by one thread
 constexpr uint32_t kSmallPairWork = 16;
+// grid of the worklist kernels, whose sizes live on the device
+constexpr unsigned kPersistentBlocks = 1024;
 
 struct FG {
     const double *xs, *ys, *box;
@@ -395,57 +397,97 @@ __device__ bool small_pair_connected(const LevelArgs &a, uint32_t A, uint32_t B)
     return false;
 }
 
-// Thread per listed pair: settle it, keep it, or hand it to the wave test.
-// Tests of grid-adjacent cells go to `near`, the rest to `far`; the near
-// tests run first so that most far pairs are already connected when theirs run.
-__global__ void k_sweep(LevelArgs a, const uint32_t *list_in, uint32_t nin, uint32_t *parent,
-                        uint32_t *list_out, uint32_t *nout, uint32_t *near, uint32_t *nnear,
-                        uint32_t *far, uint32_t *nfar)
+// Eligibility level of every pair for minpts j: the first level at which
+// both cells hold a core point and some pair of their boxes is within eps.
+// K marks pairs that never become eligible.
+__global__ void k_pair_elig(const uint32_t *pa, const uint32_t *pb, const uint16_t *pk,
+                            const uint8_t *cellkc, int J, int j, int K, uint32_t npairs,
+                            uint32_t *key, uint32_t *val)
 {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    bool valid = i < nin;
-    uint32_t pid = valid ? list_in[i] : 0u;
-    bool keep = false, tnear = false, tfar = false;
-    if (valid) {
-        uint32_t A = a.pa[pid], B = a.pb[pid];
-        uint16_t kk = a.pk[pid];
-        int kmin = kk & 0xff, kfull = kk >> 8;
-        if (kmin > a.k || a.cellkc[(size_t)A * a.J + a.j] > a.k ||
-            a.cellkc[(size_t)B * a.J + a.j] > a.k) {
-            keep = true; // not connectable yet
-        } else if (cf_find(parent, A) == cf_find(parent, B)) {
-            keep = false; // connected now, hence at every later level
-        } else if (kfull <= a.k) {
-            cf_unite(parent, A, B);
-        } else {
-            uint32_t na = a.cell_start[A + 1] - a.cell_start[A];
-            uint32_t nb = a.cell_start[B + 1] - a.cell_start[B];
-            if ((uint64_t)na * nb <= kSmallPairWork) {
-                if (small_pair_connected(a, A, B)) cf_unite(parent, A, B);
-                else keep = true;
+    if (i >= npairs) return;
+    int e = pk[i] & 0xff;
+    int ca = cellkc[(size_t)pa[i] * J + j], cb = cellkc[(size_t)pb[i] * J + j];
+    e = e > ca ? e : ca;
+    e = e > cb ? e : cb;
+    key[i] = (uint32_t)(e < K ? e : K);
+    val[i] = i;
+}
+
+// bstart[k] = first index of sorted keys with key >= k, for k = 0..K+1
+__global__ void k_bucket_starts(const uint32_t *key, uint32_t n, int K, uint32_t *bstart)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i > n) return;
+    uint32_t prev = i == 0 ? 0u : key[i - 1] + 1;
+    uint32_t cur = i == n ? (uint32_t)K + 1 : key[i];
+    if (i == 0) prev = 0;
+    for (uint32_t k = prev; k <= cur && k <= (uint32_t)K + 1; k++) bstart[k] = i;
+}
+
+// counters: cnt[0], cnt[1] sizes of the two carried lists, cnt[2] near
+// tests, cnt[3] far tests
+__global__ void k_level_reset(uint32_t *cnt, int out_idx, int clear_in, int in_idx)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        cnt[out_idx] = 0;
+        cnt[2] = 0;
+        cnt[3] = 0;
+        if (clear_in) cnt[in_idx] = 0;
+    }
+}
+
+// Grid-stride over the carried pairs (count on the device) followed by the
+// pairs that become eligible at this level. Each eligible pair is settled
+// (already connected; fully inside eps; small and tested here), kept for the
+// next level, or handed to a wave test. Tests of grid-adjacent cells go to
+// `near`, the rest to `far`; the near tests run first so most far pairs are
+// already connected when theirs run.
+__global__ void __launch_bounds__(kBlock)
+    k_sweep(LevelArgs a, const uint32_t *carried, const uint32_t *ncarried,
+            const uint32_t *fresh, uint32_t nfresh, uint32_t *parent, uint32_t *keep,
+            uint32_t *nkeep, uint32_t *near, uint32_t *nnear, uint32_t *far, uint32_t *nfar)
+{
+    uint32_t nc = *ncarried;
+    uint32_t total = nc + nfresh;
+    for (uint32_t base = blockIdx.x * kBlock; base < total; base += gridDim.x * kBlock) {
+        uint32_t i = base + threadIdx.x;
+        bool valid = i < total;
+        uint32_t pid = valid ? (i < nc ? carried[i] : fresh[i - nc]) : 0u;
+        bool kp = false, tnear = false, tfar = false;
+        if (valid) {
+            uint32_t A = a.pa[pid], B = a.pb[pid];
+            int kfull = a.pk[pid] >> 8;
+            if (cf_find(parent, A) == cf_find(parent, B)) {
+                // connected now, hence at every later level
+            } else if (kfull <= a.k) {
+                cf_unite(parent, A, B);
             } else {
-                uint32_t ka = a.cell_key[A], kb = a.cell_key[B];
-                int64_t dx = (int64_t)(ka / a.ncy) - (int64_t)(kb / a.ncy);
-                int64_t dy = (int64_t)(ka % a.ncy) - (int64_t)(kb % a.ncy);
-                bool adj = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1;
-                tnear = adj;
-                tfar = !adj;
+                uint32_t na = a.cell_start[A + 1] - a.cell_start[A];
+                uint32_t nb = a.cell_start[B + 1] - a.cell_start[B];
+                if ((uint64_t)na * nb <= kSmallPairWork) {
+                    if (small_pair_connected(a, A, B)) cf_unite(parent, A, B);
+                    else kp = true;
+                } else {
+                    uint32_t ka = a.cell_key[A], kb = a.cell_key[B];
+                    int64_t dx = (int64_t)(ka / a.ncy) - (int64_t)(kb / a.ncy);
+                    int64_t dy = (int64_t)(ka % a.ncy) - (int64_t)(kb % a.ncy);
+                    bool adj = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1;
+                    tnear = adj;
+                    tfar = !adj;
+                }
             }
         }
+        wave_append(kp, pid, keep, nkeep);
+        wave_append(tnear, pid, near, nnear);
+        wave_append(tfar, pid, far, nfar);
     }
-    wave_append(keep, pid, list_out, nout);
-    wave_append(tnear, pid, near, nnear);
-    wave_append(tfar, pid, far, nfar);
 }
 
 // wave per test pair: any core-core pair within eps_k connects the two cells
-__global__ void __launch_bounds__(kBlock)
-    k_test(LevelArgs a, const uint32_t *tests, uint32_t ntest, uint32_t *parent,
-           uint32_t *list_out, uint32_t *nout)
+__device__ void test_pair(const LevelArgs &a, uint32_t pid, uint32_t *parent,
+                          uint32_t *list_out, uint32_t *nout)
 {
-    uint32_t w = blockIdx.x * kWPB + wave_id_in_block();
-    if (w >= ntest) return;
-    uint32_t pid = tests[w];
     uint32_t A = a.pa[pid], B = a.pb[pid];
     uint32_t na = a.cell_start[A + 1] - a.cell_start[A];
     uint32_t nb = a.cell_start[B + 1] - a.cell_start[B];
@@ -480,6 +522,15 @@ __global__ void __launch_bounds__(kBlock)
             else list_out[atomicAdd(nout, 1u)] = pid;
         }
     }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_test(LevelArgs a, const uint32_t *tests, const uint32_t *ntests, uint32_t *parent,
+           uint32_t *list_out, uint32_t *nout)
+{
+    uint32_t nt = *ntests;
+    for (uint32_t w = blockIdx.x * kWPB + wave_id_in_block(); w < nt; w += gridDim.x * kWPB)
+        test_pair(a, tests[w], parent, list_out, nout);
 }
 
 // ---------------------------------------------------------- snapshots
@@ -774,6 +825,12 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_TRY(c.get("b.tests", psz, &tests));
     VDB_TRY(c.get("b.tests2", psz, &tests2));
     VDB_TRY(c.get("b.cnts", 4, &cnts));
+    uint32_t *ekey, *eval, *stk, *stv, *bstart;
+    VDB_TRY(c.get("b.ekey", psz, &ekey));
+    VDB_TRY(c.get("b.eval", psz, &eval));
+    VDB_TRY(c.get("b.stk", psz, &stk));
+    VDB_TRY(c.get("b.stv", psz, &stv));
+    VDB_TRY(c.get("b.bstart", (size_t)K + 2, &bstart));
     k_pairs<true><<<gc, kBlock, 0, c.stream()>>>(G.cell_box, G.row_lo, G.row_hi, W, G.ncell,
                                                  dTk, K, nullptr, poff, pa, pb, pk);
     VDB_HIP(hipGetLastError());
@@ -803,41 +860,51 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         if (kmax < 0) continue;
         k_init_parent<<<gc, kBlock, 0, c.stream()>>>(parent, G.ncell);
         VDB_HIP(hipGetLastError());
-        uint32_t nin = npairs;
+        // pairs bucketed by the level at which they become eligible for this j
+        std::vector<uint32_t> bst(K + 2, 0);
+        uint32_t *elig = nullptr;
+        if (npairs) {
+            k_pair_elig<<<ceil_div(npairs, kBlock), kBlock, 0, c.stream()>>>(
+                pa, pb, pk, cellkc, J, j, K, npairs, ekey, eval);
+            VDB_HIP(hipGetLastError());
+            uint32_t *sk, *sv;
+            VDB_TRY(radix_sort_pairs(c, ekey, eval, stk, stv, npairs, bits_for(K + 1), &sk, &sv));
+            k_bucket_starts<<<ceil_div(npairs + 1, kBlock), kBlock, 0, c.stream()>>>(sk, npairs, K,
+                                                                                   bstart);
+            VDB_HIP(hipGetLastError());
+            void *hs;
+            VDB_TRY(c.host_scratch(sizeof(uint32_t) * (K + 2), &hs));
+            VDB_HIP(hipMemcpyAsync(hs, bstart, sizeof(uint32_t) * (K + 2), hipMemcpyDeviceToHost,
+                                   c.stream()));
+            VDB_HIP(hipStreamSynchronize(c.stream()));
+            memcpy(bst.data(), hs, sizeof(uint32_t) * (K + 2));
+            elig = sv;
+        }
+        int in_idx = 0;
         uint32_t *lin = la, *lout = lb;
-        if (nin) k_iota<<<ceil_div(nin, kBlock), kBlock, 0, c.stream()>>>(lin, nin);
-        VDB_HIP(hipGetLastError());
         for (int k = 0; k <= kmax; k++) {
             LevelArgs a{G.xs,       G.ys,   G.cell_box, G.cell_start, G.cell_key, G.ncy, kc,
                         cellkc,     pa,     pb,         pk,           J,          j,     k,
                         Th[k]};
-            VDB_HIP(hipMemsetAsync(cnts, 0, 4 * sizeof(uint32_t), c.stream()));
-            if (nin) {
-                k_sweep<<<ceil_div(nin, kBlock), kBlock, 0, c.stream()>>>(
-                    a, lin, nin, parent, lout, cnts, tests, cnts + 1, tests2, cnts + 2);
-                VDB_HIP(hipGetLastError());
-                uint32_t h3[3];
-                {
-                    void *hs;
-                    VDB_TRY(c.host_scratch(3 * sizeof(uint32_t), &hs));
-                    VDB_HIP(hipMemcpyAsync(hs, cnts, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                           c.stream()));
-                    VDB_HIP(hipStreamSynchronize(c.stream()));
-                    memcpy(h3, hs, sizeof h3);
-                }
-                if (h3[1]) {
-                    k_test<<<ceil_div(h3[1], kWPB), kBlock, 0, c.stream()>>>(a, tests, h3[1],
-                                                                            parent, lout, cnts);
-                    VDB_HIP(hipGetLastError());
-                }
-                if (h3[2]) {
-                    k_test<<<ceil_div(h3[2], kWPB), kBlock, 0, c.stream()>>>(a, tests2, h3[2],
-                                                                            parent, lout, cnts);
-                    VDB_HIP(hipGetLastError());
-                }
-                VDB_TRY(read_u32(c, cnts, &nin));
-                std::swap(lin, lout);
-            }
+            int out_idx = 1 - in_idx;
+            k_level_reset<<<1, 64, 0, c.stream()>>>(cnts, out_idx, k == 0, in_idx);
+            VDB_HIP(hipGetLastError());
+            uint32_t nfresh = npairs ? bst[k + 1] - bst[k] : 0;
+            const uint32_t *fresh = npairs ? elig + bst[k] : nullptr;
+            // the carried count is on the device: launch a bounded grid that
+            // strides over whatever the list holds
+            k_sweep<<<kPersistentBlocks, kBlock, 0, c.stream()>>>(
+                a, lin, cnts + in_idx, fresh, nfresh, parent, lout, cnts + out_idx, tests,
+                cnts + 2, tests2, cnts + 3);
+            VDB_HIP(hipGetLastError());
+            k_test<<<kPersistentBlocks, kBlock, 0, c.stream()>>>(a, tests, cnts + 2, parent, lout,
+                                                                 cnts + out_idx);
+            VDB_HIP(hipGetLastError());
+            k_test<<<kPersistentBlocks, kBlock, 0, c.stream()>>>(a, tests2, cnts + 3, parent,
+                                                                 lout, cnts + out_idx);
+            VDB_HIP(hipGetLastError());
+            std::swap(lin, lout);
+            in_idx = out_idx;
             auto it = want.find({k, j});
             if (it == want.end()) continue;
             Ev tl(c.stream());

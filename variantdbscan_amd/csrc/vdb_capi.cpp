@@ -66,11 +66,14 @@ int upload_points(Context &c, const double *x, const double *y, uint32_t n, uint
     return kOk;
 }
 
-size_t default_budget(const vdb_options *opt)
+// Table budget: free device memory plus the pooled table slot (reused by the
+// next table), minus headroom for the other buffers of the call.
+size_t default_budget(Context &c, const vdb_options *opt)
 {
     if (opt && opt->table_budget_bytes) return (size_t)opt->table_budget_bytes;
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) return (size_t)1 << 32;
+    fr += c.slot_bytes("cv.ids");
     size_t keep = (size_t)8 << 30;
     return fr > 2 * keep ? fr - keep : fr / 2;
 }
@@ -210,7 +213,7 @@ int cluster_impl(const double *x, const double *y, uint32_t n, const double *eps
     if (!rest.empty()) {
         std::sort(rest.begin(), rest.end());
         VDB_TRY(run_per_variant(c, P, eps, minpts, rest, out, path == VDB_PATH_TABLE ? 1 : 0,
-                                precision, default_budget(opt), st, verbose));
+                                precision, default_budget(c, opt), st, verbose));
     }
     if (!(flags & VDB_DEVICE_PTRS)) {
         VDB_HIP(hipMemcpyAsync(labels, out, sizeof(uint32_t) * (size_t)n * nvar,

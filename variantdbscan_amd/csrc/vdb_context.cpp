@@ -77,6 +77,12 @@ size_t Context::pooled_bytes() const
     return s;
 }
 
+size_t Context::slot_bytes(const char *name) const
+{
+    auto it = pool_.find(name);
+    return it == pool_.end() ? 0 : it->second.bytes;
+}
+
 int Context::host_scratch(size_t bytes, void **out)
 {
     if (host_bytes_ < bytes) {

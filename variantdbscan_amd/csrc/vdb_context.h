@@ -40,6 +40,7 @@ public:
     void release(const char *name);
     void release_all();
     size_t pooled_bytes() const;
+    size_t slot_bytes(const char *name) const;
 
     // Pinned host staging for small device->host reads.
     int host_scratch(size_t bytes, void **out);
