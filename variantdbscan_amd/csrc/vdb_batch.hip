@@ -136,6 +136,43 @@ __device__ __forceinline__ bool before(double s1, uint32_t i1, double s2, uint32
     return s1 < s2 || (s1 == s2 && i1 < i2);
 }
 
+// Insert the candidates of one 64-lane block that pass `hit` into the sorted
+// lane-resident list (entry e in lane e < L), keeping the L smallest (s, id).
+__device__ __forceinline__ void knn_block(double s, uint32_t q0, bool hit, int L, double Tmax,
+                                          double &es, uint32_t &eid, int &cnt, double &kth,
+                                          uint32_t &kid, double &thr)
+{
+    int lane = lane_id();
+    uint64_t hm = ballot(hit);
+    while (hm) {
+        int hb = __builtin_ctzll(hm);
+        hm &= hm - 1;
+        double cs = __shfl(s, hb);
+        uint32_t cq = q0 + hb;
+        if (cnt == L && !before(cs, cq, kth, kid)) continue;
+        bool lt = lane < L && before(es, eid, cs, cq);
+        int pos = popc64(ballot(lt));
+        double ps = __shfl_up(es, 1);
+        uint32_t pid = __shfl_up(eid, 1);
+        if (lane == pos) {
+            es = cs;
+            eid = cq;
+        } else if (lane > pos && lane < L) {
+            es = ps;
+            eid = pid;
+        }
+        if (cnt < L) cnt++;
+        if (cnt == L) {
+            kth = __shfl(es, L - 1);
+            kid = __shfl(eid, L - 1);
+            thr = kth < Tmax ? kth : Tmax;
+        }
+    }
+}
+
+// rows holding at most this many points are swept whole, without per-cell pruning
+constexpr uint32_t kKnnRowSweep = 128;
+
 __global__ void __launch_bounds__(kBlock)
     k_knn(FG g, const uint32_t *__restrict__ list_pos, uint32_t nlist, int L, double Tmax,
           double T0, double *__restrict__ lst_s, uint32_t *__restrict__ lst_id,
@@ -159,6 +196,20 @@ __global__ void __launch_bounds__(kBlock)
         int d = (t & 1) ? -((t + 1) >> 1) : (t >> 1); // 0, -1, +1, -2, +2, ...
         int r = d + R;
         uint32_t lo = g.row_lo[(size_t)c * g.W + r], hi = g.row_hi[(size_t)c * g.W + r];
+        if (lo >= hi) continue;
+        uint32_t ra = g.cell_start[lo], re = g.cell_start[hi];
+        if (re - ra <= kKnnRowSweep) {
+            // sparse row: every point of the row's cells, contiguous in memory
+            for (uint32_t q0 = ra; q0 < re; q0 += kWave) {
+                uint32_t q = q0 + lane;
+                double s = INFINITY;
+                if (q < re) s = dist2_exact(px, py, g.xs[q], g.ys[q]);
+                bool hit = s <= thr && (cnt < L ? true : before(s, q, kth, kid));
+                knn_block(s, q0, hit, L, Tmax, es, eid, cnt, kth, kid, thr);
+            }
+            if (cnt == L && kth <= T0) done = true;
+            continue;
+        }
         for (uint32_t cb = lo; cb < hi && !done; cb += kWave) {
             uint32_t C = cb + lane;
             double S = INFINITY;
@@ -179,31 +230,7 @@ __global__ void __launch_bounds__(kBlock)
                     double s = INFINITY;
                     if (q < e) s = dist2_exact(px, py, g.xs[q], g.ys[q]);
                     bool hit = s <= thr && (cnt < L ? true : before(s, q, kth, kid));
-                    uint64_t hm = ballot(hit);
-                    while (hm) {
-                        int hb = __builtin_ctzll(hm);
-                        hm &= hm - 1;
-                        double cs = __shfl(s, hb);
-                        uint32_t cq = q0 + hb;
-                        if (cnt == L && !before(cs, cq, kth, kid)) continue;
-                        bool lt = lane < L && before(es, eid, cs, cq);
-                        int pos = popc64(ballot(lt));
-                        double ps = __shfl_up(es, 1);
-                        uint32_t pid = __shfl_up(eid, 1);
-                        if (lane == pos) {
-                            es = cs;
-                            eid = cq;
-                        } else if (lane > pos && lane < L) {
-                            es = ps;
-                            eid = pid;
-                        }
-                        if (cnt < L) cnt++;
-                        if (cnt == L) {
-                            kth = __shfl(es, L - 1);
-                            kid = __shfl(eid, L - 1);
-                            thr = kth < Tmax ? kth : Tmax;
-                        }
-                    }
+                    knn_block(s, q0, hit, L, Tmax, es, eid, cnt, kth, kid, thr);
                 }
                 // L neighbours within eps_0: core in every variant, list unused
                 if (cnt == L && kth <= T0) done = true;
@@ -253,7 +280,7 @@ __global__ void k_core_levels(const uint32_t *lst_idx, const double *lst_s,
 
 // per cell, per j, per k: smallest rank of a point with kc_j <= k (cells list
 // their points in ascending rank, so it is the first such point), and the
-// first level at which the cell holds a core point
+// first level at which the cell holds a core point. Lane k tracks level k.
 __global__ void __launch_bounds__(kBlock)
     k_cell_core(const uint32_t *cell_start, uint32_t ncell, const uint8_t *kc, int J, int K,
                 const uint32_t *pos_rank, uint32_t *cmr, uint8_t *cellkc)
@@ -263,34 +290,22 @@ __global__ void __launch_bounds__(kBlock)
     int lane = lane_id();
     uint32_t a = cell_start[c], e = cell_start[c + 1];
     for (int j = 0; j < J; j++) {
-        // first position (in the cell) with kc <= k, for every k; kc values are
-        // levels, so track the running minimum level while scanning in order
-        uint32_t first[kMaxLevels];
-        for (int k = 0; k < K; k++) first[k] = kNone;
-        int minlev = K; // levels [minlev, K) already resolved
-        for (uint32_t q0 = a; q0 < e && minlev > 0; q0 += kWave) {
+        uint32_t first = kNone; // level `lane`
+        bool all = false;
+        for (uint32_t q0 = a; q0 < e && !all; q0 += kWave) {
             uint32_t q = q0 + lane;
             int lv = q < e ? (int)kc[(size_t)q * J + j] : K;
-            for (int k = 0; k < minlev; k++) {
+            for (int k = 0; k < K; k++) {
                 uint64_t m = ballot(lv <= k);
-                if (m) first[k] = q0 + __builtin_ctzll(m);
+                if (lane == k && first == kNone && m) first = q0 + __builtin_ctzll(m);
             }
-            // once level k is resolved every level above it is too
-            int nm = minlev;
-            for (int k = 0; k < minlev; k++)
-                if (first[k] != kNone) {
-                    nm = k;
-                    break;
-                }
-            for (int k = nm; k < minlev; k++)
-                if (first[k] == kNone) first[k] = first[nm];
-            minlev = nm;
+            // level 0 found: every level is resolved
+            all = __shfl(first, 0) != kNone;
         }
-        if (lane == 0) {
-            for (int k = 0; k < K; k++)
-                cmr[((size_t)c * J + j) * K + k] = first[k] == kNone ? kNone : pos_rank[first[k]];
-            cellkc[(size_t)c * J + j] = (uint8_t)minlev;
-        }
+        if (lane < K)
+            cmr[((size_t)c * J + j) * K + lane] = first == kNone ? kNone : pos_rank[first];
+        uint64_t have = ballot(lane < K && first != kNone);
+        if (lane == 0) cellkc[(size_t)c * J + j] = (uint8_t)(have ? __builtin_ctzll(have) : K);
     }
 }
 
@@ -370,6 +385,11 @@ __device__ __forceinline__ void wave_append(bool take, uint32_t v, uint32_t *lis
 }
 
 // ------------------------------------------------- per-level pair sweep
+// a cell pair with its levels: k = kmin | kfull << 8
+struct PairRec {
+    uint32_t a, b, k;
+};
+
 struct LevelArgs {
     const double *xs, *ys, *box;
     const uint32_t *cell_start, *cell_key;
@@ -400,18 +420,45 @@ __device__ bool small_pair_connected(const LevelArgs &a, uint32_t A, uint32_t B)
 // Eligibility level of every pair for minpts j: the first level at which
 // both cells hold a core point and some pair of their boxes is within eps.
 // K marks pairs that never become eligible.
+// For pairs of small cells (at most kSmallPairWork point pairs) the level is
+// exact instead: the first level at which some core-core pair is within eps,
+//   min over (p, q) of max(level(s(p,q)), kc_j(p), kc_j(q)),
+// and the value carries bit 31 so the sweep unites them without a test.
 __global__ void k_pair_elig(const uint32_t *pa, const uint32_t *pb, const uint16_t *pk,
-                            const uint8_t *cellkc, int J, int j, int K, uint32_t npairs,
-                            uint32_t *key, uint32_t *val)
+                            const uint8_t *cellkc, const uint32_t *cell_start, const double *xs,
+                            const double *ys, const uint8_t *kc, const double *Tk, int J, int j,
+                            int K, uint32_t npairs, uint32_t *key, uint32_t *val)
 {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= npairs) return;
+    uint32_t A = pa[i], B = pb[i];
     int e = pk[i] & 0xff;
-    int ca = cellkc[(size_t)pa[i] * J + j], cb = cellkc[(size_t)pb[i] * J + j];
+    int ca = cellkc[(size_t)A * J + j], cb = cellkc[(size_t)B * J + j];
     e = e > ca ? e : ca;
     e = e > cb ? e : cb;
+    uint32_t flag = 0;
+    uint32_t a0 = cell_start[A], a1 = cell_start[A + 1];
+    uint32_t b0 = cell_start[B], b1 = cell_start[B + 1];
+    if (e < K && (uint64_t)(a1 - a0) * (b1 - b0) <= kSmallPairWork) {
+        int best = K;
+        for (uint32_t p = a0; p < a1; p++) {
+            int kp = kc[(size_t)p * J + j];
+            if (kp >= best) continue;
+            double px = xs[p], py = ys[p];
+            for (uint32_t q = b0; q < b1; q++) {
+                int lv = kc[(size_t)q * J + j];
+                lv = lv > kp ? lv : kp;
+                if (lv >= best) continue;
+                double sq = dist2_exact(px, py, xs[q], ys[q]);
+                while (lv < best && sq > Tk[lv]) lv++;
+                if (lv < best) best = lv;
+            }
+        }
+        e = best;
+        flag = 0x80000000u;
+    }
     key[i] = (uint32_t)(e < K ? e : K);
-    val[i] = i;
+    val[i] = i | flag;
 }
 
 // bstart[k] = first index of sorted keys with key >= k, for k = 0..K+1
@@ -423,6 +470,28 @@ __global__ void k_bucket_starts(const uint32_t *key, uint32_t n, int K, uint32_t
     uint32_t cur = i == n ? (uint32_t)K + 1 : key[i];
     if (i == 0) prev = 0;
     for (uint32_t k = prev; k <= cur && k <= (uint32_t)K + 1; k++) bstart[k] = i;
+}
+
+__global__ void k_pack_pairs(const uint32_t *order, const uint32_t *pa, const uint32_t *pb,
+                             const uint16_t *pk, uint32_t n, PairRec *rec)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t v = order[i] & 0x7fffffffu;
+    uint32_t exact = order[i] >> 31;
+    rec[i] = PairRec{pa[v], pb[v], (uint32_t)pk[v] | (exact << 16)};
+}
+
+__device__ __forceinline__ void wave_append_rec(bool take, const PairRec &v, PairRec *list,
+                                                uint32_t *count)
+{
+    uint64_t m = ballot(take);
+    if (!m) return;
+    int leader = __builtin_ctzll(m);
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(count, (uint32_t)popc64(m));
+    base = __shfl(base, leader);
+    if (take) list[base + mask_rank(m)] = v;
 }
 
 // counters: cnt[0], cnt[1] sizes of the two carried lists, cnt[2] near
@@ -444,23 +513,26 @@ __global__ void k_level_reset(uint32_t *cnt, int out_idx, int clear_in, int in_i
 // `near`, the rest to `far`; the near tests run first so most far pairs are
 // already connected when theirs run.
 __global__ void __launch_bounds__(kBlock)
-    k_sweep(LevelArgs a, const uint32_t *carried, const uint32_t *ncarried,
-            const uint32_t *fresh, uint32_t nfresh, uint32_t *parent, uint32_t *keep,
-            uint32_t *nkeep, uint32_t *near, uint32_t *nnear, uint32_t *far, uint32_t *nfar)
+    k_sweep(LevelArgs a, const PairRec *carried, const uint32_t *ncarried,
+            const PairRec *fresh, uint32_t nfresh, uint32_t *parent, PairRec *keep,
+            uint32_t *nkeep, PairRec *near, uint32_t *nnear, PairRec *far, uint32_t *nfar)
 {
     uint32_t nc = *ncarried;
     uint32_t total = nc + nfresh;
     for (uint32_t base = blockIdx.x * kBlock; base < total; base += gridDim.x * kBlock) {
         uint32_t i = base + threadIdx.x;
         bool valid = i < total;
-        uint32_t pid = valid ? (i < nc ? carried[i] : fresh[i - nc]) : 0u;
+        PairRec pr{0, 0, 0};
+        if (valid) pr = i < nc ? carried[i] : fresh[i - nc];
         bool kp = false, tnear = false, tfar = false;
         if (valid) {
-            uint32_t A = a.pa[pid], B = a.pb[pid];
-            int kfull = a.pk[pid] >> 8;
+            uint32_t A = pr.a, B = pr.b;
+            int kfull = (int)((pr.k >> 8) & 0xff);
+            bool exact = (pr.k >> 16) & 1u;
             if (cf_find(parent, A) == cf_find(parent, B)) {
                 // connected now, hence at every later level
-            } else if (kfull <= a.k) {
+            } else if (exact || kfull <= a.k) {
+                // exact pairs arrive at the level they connect
                 cf_unite(parent, A, B);
             } else {
                 uint32_t na = a.cell_start[A + 1] - a.cell_start[A];
@@ -478,17 +550,17 @@ __global__ void __launch_bounds__(kBlock)
                 }
             }
         }
-        wave_append(kp, pid, keep, nkeep);
-        wave_append(tnear, pid, near, nnear);
-        wave_append(tfar, pid, far, nfar);
+        wave_append_rec(kp, pr, keep, nkeep);
+        wave_append_rec(tnear, pr, near, nnear);
+        wave_append_rec(tfar, pr, far, nfar);
     }
 }
 
 // wave per test pair: any core-core pair within eps_k connects the two cells
-__device__ void test_pair(const LevelArgs &a, uint32_t pid, uint32_t *parent,
-                          uint32_t *list_out, uint32_t *nout)
+__device__ void test_pair(const LevelArgs &a, const PairRec &pr, uint32_t *parent,
+                          PairRec *list_out, uint32_t *nout)
 {
-    uint32_t A = a.pa[pid], B = a.pb[pid];
+    uint32_t A = pr.a, B = pr.b;
     uint32_t na = a.cell_start[A + 1] - a.cell_start[A];
     uint32_t nb = a.cell_start[B + 1] - a.cell_start[B];
     if (na > nb) { // lanes sweep the larger cell
@@ -519,14 +591,14 @@ __device__ void test_pair(const LevelArgs &a, uint32_t pid, uint32_t *parent,
         }
         if (lane_id() == 0) {
             if (found) cf_unite(parent, A, B);
-            else list_out[atomicAdd(nout, 1u)] = pid;
+            else list_out[atomicAdd(nout, 1u)] = pr;
         }
     }
 }
 
 __global__ void __launch_bounds__(kBlock)
-    k_test(LevelArgs a, const uint32_t *tests, const uint32_t *ntests, uint32_t *parent,
-           uint32_t *list_out, uint32_t *nout)
+    k_test(LevelArgs a, const PairRec *tests, const uint32_t *ntests, uint32_t *parent,
+           PairRec *list_out, uint32_t *nout)
 {
     uint32_t nt = *ntests;
     for (uint32_t w = blockIdx.x * kWPB + wave_id_in_block(); w < nt; w += gridDim.x * kWPB)
@@ -614,6 +686,104 @@ __global__ void k_batch_labels(const uint32_t *cell_of_pos, const uint32_t *pare
         lbl = (best != kNone && best < r) ? cid[best] + 1 : 0u;
     }
     out[mapping[r]] = lbl;
+}
+
+// root rank of every cell at level k (NONE if the cell holds no core point)
+__global__ void k_snapshot_roots(const uint32_t *parent, const uint32_t *compmin,
+                                 const uint8_t *cellkc, uint32_t ncell, int J, int j, int k,
+                                 uint32_t *rootrank)
+{
+    uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= ncell) return;
+    rootrank[c] = cellkc[(size_t)c * J + j] <= k ? compmin[parent[c]] : kNone;
+}
+
+// Labels of up to kLabelSlots variants sharing minpts j, from their level
+// snapshots. Thread per caller index: the point's static data are read once
+// and every output row is written coalesced.
+constexpr int kLabelSlots = 8;
+struct LabelSlots {
+    int n;
+    int k[kLabelSlots];
+    double T[kLabelSlots];
+    const uint32_t *rootrank[kLabelSlots]; // per cell
+    const uint32_t *cid[kLabelSlots];      // per rank
+    uint32_t *out[kLabelSlots];            // output row (caller order)
+};
+
+__global__ void __launch_bounds__(kBlock)
+    k_labels_j(LabelSlots S, const uint32_t *pos_of_caller, const uint32_t *cell_of_pos,
+               const uint32_t *pos_rank, const uint8_t *kc, int J, int j,
+               const uint32_t *lst_idx, const double *lst_s, const uint32_t *lst_id,
+               const uint32_t *lst_n, int L, uint32_t n)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t p = pos_of_caller[i];
+    uint32_t r = pos_rank[p];
+    uint32_t C = cell_of_pos[p];
+    int kcp = kc[(size_t)p * J + j];
+    uint32_t best[kLabelSlots];
+    bool any_border = false;
+    double Tmax = -1.0;
+#pragma unroll
+    for (int t = 0; t < kLabelSlots; t++) {
+        best[t] = kNone;
+        if (t < S.n && kcp > S.k[t]) {
+            any_border = true;
+            Tmax = S.T[t] > Tmax ? S.T[t] : Tmax;
+        }
+    }
+    if (any_border) {
+        // non-core at level k: count_k(p) < m_j <= L, so its list holds every
+        // eps_k neighbour as the prefix with s <= T_k
+        uint32_t li = lst_idx[p];
+        if (li != kNone) {
+            uint32_t m = lst_n[li];
+            const double *sv = lst_s + (size_t)li * L;
+            const uint32_t *iv = lst_id + (size_t)li * L;
+            for (uint32_t e = 0; e < m; e++) {
+                double se = sv[e];
+                if (se > Tmax) break;
+                uint32_t q = iv[e];
+                int kq = kc[(size_t)q * J + j];
+                uint32_t cq = cell_of_pos[q];
+#pragma unroll
+                for (int t = 0; t < kLabelSlots; t++) {
+                    if (t < S.n && kcp > S.k[t] && se <= S.T[t] && kq <= S.k[t]) {
+                        uint32_t rr = S.rootrank[t][cq];
+                        best[t] = rr < best[t] ? rr : best[t];
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < kLabelSlots; t++) {
+        if (t >= S.n) break;
+        uint32_t lbl;
+        if (kcp <= S.k[t]) {
+            lbl = S.cid[t][S.rootrank[t][C]] + 1;
+        } else {
+            uint32_t b = best[t];
+            lbl = (b != kNone && b < r) ? S.cid[t][b] + 1 : 0u;
+        }
+        S.out[t][i] = lbl;
+    }
+}
+
+__global__ void k_invert_perm(const uint32_t *pos_rank, const uint32_t *mapping, uint32_t n,
+                              uint32_t *pos_of_caller)
+{
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    pos_of_caller[mapping[pos_rank[p]]] = p;
+}
+
+__global__ void k_fill_u32(uint32_t *a, uint32_t v, uint32_t n)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) a[i] = v;
 }
 
 // variants with no neighbour at all (eps < 0 or NaN): core iff minpts == 0,
@@ -814,12 +984,14 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         return kErrUnsupported;
     }
     uint32_t npairs = (uint32_t)npairs64;
-    uint32_t *pa, *pb, *la, *lb, *tests, *tests2, *cnts;
+    uint32_t *pa, *pb, *cnts;
     uint16_t *pk;
+    PairRec *rec, *la, *lb, *tests, *tests2;
     size_t psz = npairs ? npairs : 1;
     VDB_TRY(c.get("b.pa", psz, &pa));
     VDB_TRY(c.get("b.pb", psz, &pb));
     VDB_TRY(c.get("b.pk", psz, &pk));
+    VDB_TRY(c.get("b.rec", psz, &rec));
     VDB_TRY(c.get("b.la", psz, &la));
     VDB_TRY(c.get("b.lb", psz, &lb));
     VDB_TRY(c.get("b.tests", psz, &tests));
@@ -834,6 +1006,10 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     k_pairs<true><<<gc, kBlock, 0, c.stream()>>>(G.cell_box, G.row_lo, G.row_hi, W, G.ncell,
                                                  dTk, K, nullptr, poff, pa, pb, pk);
     VDB_HIP(hipGetLastError());
+    uint32_t *pos_of_caller;
+    VDB_TRY(c.get("b.poc", n, &pos_of_caller));
+    k_invert_perm<<<gn, kBlock, 0, c.stream()>>>(G.pos_rank, P.mapping, n, pos_of_caller);
+    VDB_HIP(hipGetLastError());
     if (st) st->ms_prepare += tpairs.ms();
 
     // ---- variants of each (k, j)
@@ -844,44 +1020,53 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         int j = (int)(std::lower_bound(mps.begin(), mps.end(), minpts[v]) - mps.begin());
         want[{k, j}].push_back(v);
     }
-    uint32_t *parent, *compmin, *rflag, *cid;
+    uint32_t *parent, *compmin, *rflag;
     VDB_TRY(c.get("b.parent", G.ncell, &parent));
     VDB_TRY(c.get("b.compmin", G.ncell, &compmin));
     VDB_TRY(c.get("b.rflag", n, &rflag));
-    VDB_TRY(c.get("b.cid", n, &cid));
+    // per-level snapshots of one j: cell root ranks and cluster ids by rank
+    uint32_t *snap_root, *snap_cid;
+    VDB_TRY(c.get("b.snap_root", (size_t)K * G.ncell, &snap_root));
+    VDB_TRY(c.get("b.snap_cid", (size_t)K * n, &snap_cid));
     Ev tuf(c.stream());
     double ms_labels = 0;
     for (int j = 0; j < J; j++) {
-        // minpts values only reached by variants of lower levels still need
-        // the sweep up to their highest requested level
         int kmax = -1;
         for (auto &w : want)
             if (w.first.second == j) kmax = std::max(kmax, w.first.first);
         if (kmax < 0) continue;
         k_init_parent<<<gc, kBlock, 0, c.stream()>>>(parent, G.ncell);
         VDB_HIP(hipGetLastError());
-        // pairs bucketed by the level at which they become eligible for this j
+        // pair records bucketed by the level at which they become eligible
         std::vector<uint32_t> bst(K + 2, 0);
-        uint32_t *elig = nullptr;
         if (npairs) {
             k_pair_elig<<<ceil_div(npairs, kBlock), kBlock, 0, c.stream()>>>(
-                pa, pb, pk, cellkc, J, j, K, npairs, ekey, eval);
+                pa, pb, pk, cellkc, G.cell_start, G.xs, G.ys, kc, dTk, J, j, K, npairs, ekey,
+                eval);
             VDB_HIP(hipGetLastError());
             uint32_t *sk, *sv;
             VDB_TRY(radix_sort_pairs(c, ekey, eval, stk, stv, npairs, bits_for(K + 1), &sk, &sv));
             k_bucket_starts<<<ceil_div(npairs + 1, kBlock), kBlock, 0, c.stream()>>>(sk, npairs, K,
                                                                                    bstart);
             VDB_HIP(hipGetLastError());
-            void *hs;
-            VDB_TRY(c.host_scratch(sizeof(uint32_t) * (K + 2), &hs));
-            VDB_HIP(hipMemcpyAsync(hs, bstart, sizeof(uint32_t) * (K + 2), hipMemcpyDeviceToHost,
-                                   c.stream()));
-            VDB_HIP(hipStreamSynchronize(c.stream()));
-            memcpy(bst.data(), hs, sizeof(uint32_t) * (K + 2));
-            elig = sv;
+            uint32_t nelig = 0;
+            {
+                void *hs;
+                VDB_TRY(c.host_scratch(sizeof(uint32_t) * (K + 2), &hs));
+                VDB_HIP(hipMemcpyAsync(hs, bstart, sizeof(uint32_t) * (K + 2),
+                                       hipMemcpyDeviceToHost, c.stream()));
+                VDB_HIP(hipStreamSynchronize(c.stream()));
+                memcpy(bst.data(), hs, sizeof(uint32_t) * (K + 2));
+                nelig = bst[K];
+            }
+            if (nelig)
+                k_pack_pairs<<<ceil_div(nelig, kBlock), kBlock, 0, c.stream()>>>(sv, pa, pb, pk,
+                                                                               nelig, rec);
+            VDB_HIP(hipGetLastError());
         }
         int in_idx = 0;
-        uint32_t *lin = la, *lout = lb;
+        PairRec *lin = la, *lout = lb;
+        std::vector<int> snap_k;
         for (int k = 0; k <= kmax; k++) {
             LevelArgs a{G.xs,       G.ys,   G.cell_box, G.cell_start, G.cell_key, G.ncy, kc,
                         cellkc,     pa,     pb,         pk,           J,          j,     k,
@@ -890,9 +1075,8 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             k_level_reset<<<1, 64, 0, c.stream()>>>(cnts, out_idx, k == 0, in_idx);
             VDB_HIP(hipGetLastError());
             uint32_t nfresh = npairs ? bst[k + 1] - bst[k] : 0;
-            const uint32_t *fresh = npairs ? elig + bst[k] : nullptr;
-            // the carried count is on the device: launch a bounded grid that
-            // strides over whatever the list holds
+            const PairRec *fresh = rec + (npairs ? bst[k] : 0);
+            // worklist sizes stay on the device: bounded grids stride over them
             k_sweep<<<kPersistentBlocks, kBlock, 0, c.stream()>>>(
                 a, lin, cnts + in_idx, fresh, nfresh, parent, lout, cnts + out_idx, tests,
                 cnts + 2, tests2, cnts + 3);
@@ -905,28 +1089,54 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             VDB_HIP(hipGetLastError());
             std::swap(lin, lout);
             in_idx = out_idx;
-            auto it = want.find({k, j});
-            if (it == want.end()) continue;
-            Ev tl(c.stream());
+            if (want.find({k, j}) == want.end()) continue;
+            // snapshot of level k: component roots, their smallest core rank,
+            // cluster ids in ascending root rank
+            int slot = (int)snap_k.size();
+            snap_k.push_back(k);
+            uint32_t *sroot = snap_root + (size_t)slot * G.ncell;
+            uint32_t *scid = snap_cid + (size_t)slot * n;
             k_cell_roots<<<gc, kBlock, 0, c.stream()>>>(parent, G.ncell);
             VDB_HIP(hipGetLastError());
-            VDB_HIP(hipMemsetAsync(compmin, 0xff, sizeof(uint32_t) * G.ncell, c.stream()));
+            k_fill_u32<<<gc, kBlock, 0, c.stream()>>>(compmin, kNone, G.ncell);
+            VDB_HIP(hipGetLastError());
             k_comp_min<<<gc, kBlock, 0, c.stream()>>>(parent, cmr, cellkc, G.ncell, J, j, K, k,
                                                       compmin);
             VDB_HIP(hipGetLastError());
-            VDB_HIP(hipMemsetAsync(rflag, 0, sizeof(uint32_t) * n, c.stream()));
+            k_fill_u32<<<gn, kBlock, 0, c.stream()>>>(rflag, 0u, n);
+            VDB_HIP(hipGetLastError());
             k_root_flags_cells<<<gc, kBlock, 0, c.stream()>>>(parent, compmin, cellkc, G.ncell,
                                                               J, j, k, rflag);
             VDB_HIP(hipGetLastError());
-            VDB_TRY(scan_exclusive_u32(c, rflag, cid, n, nullptr));
-            for (uint32_t v : it->second) {
-                k_batch_labels<<<gn, kBlock, 0, c.stream()>>>(
-                    G.cell_of_pos, parent, compmin, cid, kc, J, j, k, Th[k], lst_idx, lst_s,
-                    lst_id, lst_n, (int)L, G.pos_rank, P.mapping, n, out + (size_t)rows[v] * n);
-                VDB_HIP(hipGetLastError());
-            }
-            ms_labels += tl.ms();
+            VDB_TRY(scan_exclusive_u32(c, rflag, scid, n, nullptr));
+            k_snapshot_roots<<<gc, kBlock, 0, c.stream()>>>(parent, compmin, cellkc, G.ncell, J,
+                                                            j, k, sroot);
+            VDB_HIP(hipGetLastError());
         }
+        // labels of every requested variant of this j, kLabelSlots rows per launch
+        Ev tl(c.stream());
+        std::vector<std::pair<int, uint32_t>> rows_j; // (slot, variant)
+        for (size_t sl = 0; sl < snap_k.size(); sl++)
+            for (uint32_t v : want[{snap_k[sl], j}]) rows_j.push_back({(int)sl, v});
+        for (size_t r0 = 0; r0 < rows_j.size(); r0 += kLabelSlots) {
+            LabelSlots S;
+            memset(&S, 0, sizeof S);
+            S.n = (int)std::min<size_t>(kLabelSlots, rows_j.size() - r0);
+            for (int t = 0; t < S.n; t++) {
+                int sl = rows_j[r0 + t].first;
+                int k = snap_k[sl];
+                S.k[t] = k;
+                S.T[t] = Th[k];
+                S.rootrank[t] = snap_root + (size_t)sl * G.ncell;
+                S.cid[t] = snap_cid + (size_t)sl * n;
+                S.out[t] = out + (size_t)rows[rows_j[r0 + t].second] * n;
+            }
+            k_labels_j<<<gn, kBlock, 0, c.stream()>>>(S, pos_of_caller, G.cell_of_pos,
+                                                      G.pos_rank, kc, J, j, lst_idx, lst_s,
+                                                      lst_id, lst_n, (int)L, n);
+            VDB_HIP(hipGetLastError());
+        }
+        ms_labels += tl.ms();
     }
     double ms_uf_all = tuf.ms();
     if (st) {

@@ -33,7 +33,7 @@ public:
     template <typename T> int get(const char *name, size_t count, T **out)
     {
         void *p = nullptr;
-        int s = get(name, count * sizeof(T) + 16, &p);
+        int s = get(name, count * sizeof(T), &p);
         *out = (T *)p;
         return s;
     }
