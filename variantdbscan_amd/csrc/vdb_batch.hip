@@ -708,18 +708,31 @@ struct LabelSlots {
     double T[kLabelSlots];
     const uint32_t *rootrank[kLabelSlots]; // per cell
     const uint32_t *cid[kLabelSlots];      // per rank
+    const uint32_t *celllab[kLabelSlots];  // per cell: label of its core points
     uint32_t *out[kLabelSlots];            // output row (caller order)
 };
 
+// label of a cell's core points at a snapshot: 1 + cluster id of its root
+__global__ void k_cell_labels(const uint32_t *rootrank, const uint32_t *cid, uint32_t ncell,
+                              uint32_t *celllab)
+{
+    uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= ncell) return;
+    uint32_t rr = rootrank[c];
+    celllab[c] = rr == kNone ? 0u : cid[rr] + 1;
+}
+
+// Thread per cell-order position: the point's data are read in sequence, the
+// labels are scattered to caller order.
 __global__ void __launch_bounds__(kBlock)
-    k_labels_j(LabelSlots S, const uint32_t *pos_of_caller, const uint32_t *cell_of_pos,
+    k_labels_j(LabelSlots S, const uint32_t *caller_of_pos, const uint32_t *cell_of_pos,
                const uint32_t *pos_rank, const uint8_t *kc, int J, int j,
                const uint32_t *lst_idx, const double *lst_s, const uint32_t *lst_id,
                const uint32_t *lst_n, int L, uint32_t n)
 {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    uint32_t p = pos_of_caller[i];
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    uint32_t i = caller_of_pos[p];
     uint32_t r = pos_rank[p];
     uint32_t C = cell_of_pos[p];
     int kcp = kc[(size_t)p * J + j];
@@ -763,7 +776,7 @@ __global__ void __launch_bounds__(kBlock)
         if (t >= S.n) break;
         uint32_t lbl;
         if (kcp <= S.k[t]) {
-            lbl = S.cid[t][S.rootrank[t][C]] + 1;
+            lbl = S.celllab[t][C];
         } else {
             uint32_t b = best[t];
             lbl = (b != kNone && b < r) ? S.cid[t][b] + 1 : 0u;
@@ -772,12 +785,12 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-__global__ void k_invert_perm(const uint32_t *pos_rank, const uint32_t *mapping, uint32_t n,
-                              uint32_t *pos_of_caller)
+__global__ void k_caller_of_pos(const uint32_t *pos_rank, const uint32_t *mapping, uint32_t n,
+                                uint32_t *caller_of_pos)
 {
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= n) return;
-    pos_of_caller[mapping[pos_rank[p]]] = p;
+    caller_of_pos[p] = mapping[pos_rank[p]];
 }
 
 __global__ void k_fill_u32(uint32_t *a, uint32_t v, uint32_t n)
@@ -1006,9 +1019,9 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     k_pairs<true><<<gc, kBlock, 0, c.stream()>>>(G.cell_box, G.row_lo, G.row_hi, W, G.ncell,
                                                  dTk, K, nullptr, poff, pa, pb, pk);
     VDB_HIP(hipGetLastError());
-    uint32_t *pos_of_caller;
-    VDB_TRY(c.get("b.poc", n, &pos_of_caller));
-    k_invert_perm<<<gn, kBlock, 0, c.stream()>>>(G.pos_rank, P.mapping, n, pos_of_caller);
+    uint32_t *caller_of_pos;
+    VDB_TRY(c.get("b.cop", n, &caller_of_pos));
+    k_caller_of_pos<<<gn, kBlock, 0, c.stream()>>>(G.pos_rank, P.mapping, n, caller_of_pos);
     VDB_HIP(hipGetLastError());
     if (st) st->ms_prepare += tpairs.ms();
 
@@ -1025,8 +1038,9 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_TRY(c.get("b.compmin", G.ncell, &compmin));
     VDB_TRY(c.get("b.rflag", n, &rflag));
     // per-level snapshots of one j: cell root ranks and cluster ids by rank
-    uint32_t *snap_root, *snap_cid;
+    uint32_t *snap_root, *snap_cid, *snap_lab;
     VDB_TRY(c.get("b.snap_root", (size_t)K * G.ncell, &snap_root));
+    VDB_TRY(c.get("b.snap_lab", (size_t)K * G.ncell, &snap_lab));
     VDB_TRY(c.get("b.snap_cid", (size_t)K * n, &snap_cid));
     Ev tuf(c.stream());
     double ms_labels = 0;
@@ -1112,6 +1126,9 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             k_snapshot_roots<<<gc, kBlock, 0, c.stream()>>>(parent, compmin, cellkc, G.ncell, J,
                                                             j, k, sroot);
             VDB_HIP(hipGetLastError());
+            k_cell_labels<<<gc, kBlock, 0, c.stream()>>>(sroot, scid, G.ncell,
+                                                         snap_lab + (size_t)slot * G.ncell);
+            VDB_HIP(hipGetLastError());
         }
         // labels of every requested variant of this j, kLabelSlots rows per launch
         Ev tl(c.stream());
@@ -1129,9 +1146,10 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
                 S.T[t] = Th[k];
                 S.rootrank[t] = snap_root + (size_t)sl * G.ncell;
                 S.cid[t] = snap_cid + (size_t)sl * n;
+                S.celllab[t] = snap_lab + (size_t)sl * G.ncell;
                 S.out[t] = out + (size_t)rows[rows_j[r0 + t].second] * n;
             }
-            k_labels_j<<<gn, kBlock, 0, c.stream()>>>(S, pos_of_caller, G.cell_of_pos,
+            k_labels_j<<<gn, kBlock, 0, c.stream()>>>(S, caller_of_pos, G.cell_of_pos,
                                                       G.pos_rank, kc, J, j, lst_idx, lst_s,
                                                       lst_id, lst_n, (int)L, n);
             VDB_HIP(hipGetLastError());

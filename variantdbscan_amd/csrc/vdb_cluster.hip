@@ -52,9 +52,44 @@ __device__ __forceinline__ void window_blocks(const GridView &g, uint32_t c, boo
         for (uint32_t q0 = a; q0 < e; q0 += kWave) {
             uint32_t q = q0 + lane_id();
             bool v = q < e;
-            double qx = v ? g.xs[q] : __longlong_as_double(0x7ff8000000000000ll);
-            double qy = v ? g.ys[q] : __longlong_as_double(0x7ff8000000000000ll);
+            // clamp instead of branching; lanes past the run see NaN (never accepted)
+            uint32_t qc = v ? q : e - 1;
+            double qx = g.xs[qc], qy = g.ys[qc];
+            qx = v ? qx : __longlong_as_double(0x7ff8000000000000ll);
+            qy = v ? qy : __longlong_as_double(0x7ff8000000000000ll);
             f(q, v, qx, qy);
+        }
+    }
+}
+
+// Like window_blocks, but hands out chunks of up to kChunk 64-candidate blocks
+// held in registers (lane l holds candidates q0 + i*64 + l). Used where the
+// consumer wants to finish one origin over a whole chunk before the next.
+constexpr int kChunk = 8;
+template <class F>
+__device__ __forceinline__ void window_chunks(const GridView &g, uint32_t c, F &&f)
+{
+    int R = g.W / 2;
+    const double qnan = __longlong_as_double(0x7ff8000000000000ll);
+    for (int r = 0; r < g.W; r++) {
+        (void)R;
+        uint32_t lo = g.row_lo[(size_t)c * g.W + r], hi = g.row_hi[(size_t)c * g.W + r];
+        if (lo >= hi) continue;
+        uint32_t a = g.cell_start[lo], e = g.cell_start[hi];
+        for (uint32_t q0 = a; q0 < e; q0 += kWave * kChunk) {
+            int nblk = (int)((e - q0 + kWave - 1) / kWave);
+            nblk = nblk < kChunk ? nblk : kChunk;
+            double qx[kChunk], qy[kChunk];
+#pragma unroll
+            for (int i = 0; i < kChunk; i++) {
+                uint32_t q = q0 + i * kWave + lane_id();
+                bool v = i < nblk && q < e;
+                uint32_t qc = v ? q : e - 1;
+                double x = g.xs[qc], y = g.ys[qc];
+                qx[i] = v ? x : qnan;
+                qy[i] = v ? y : qnan;
+            }
+            f(q0, nblk, qx, qy);
         }
     }
 }
@@ -73,6 +108,26 @@ __device__ __forceinline__ float dist2_f32(float ox, float oy, float qx, float q
     float dx = __fsub_rn(ox, qx);
     float dy = __fsub_rn(oy, qy);
     return __fmaf_rn(dx, dx, __fmul_rn(dy, dy));
+}
+
+// Wave-uniform values pinned to scalar registers, so the 8 origins of an
+// item cost no vector registers.
+__device__ __forceinline__ double sgpr_d(double v)
+{
+    uint64_t u = __double_as_longlong(v);
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ uint64_t sgpr_u64(uint64_t v)
+{
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint32_t sgpr_u32(uint32_t v)
+{
+    return __builtin_amdgcn_readfirstlane(v);
 }
 
 // ----------------------------------------------------------------- items
@@ -105,18 +160,20 @@ __global__ void __launch_bounds__(kBlock)
 {
     uint32_t it = blockIdx.x * kWavesPerBlock + wave_id_in_block();
     if (it >= nitems) return;
-    uint32_t c = item_cell[it], o0 = item_o0[it];
-    uint32_t nb = g.cell_start[c + 1] - o0;
+    uint32_t c = sgpr_u32(item_cell[it]), o0 = sgpr_u32(item_o0[it]);
+    uint32_t nb = sgpr_u32(g.cell_start[c + 1]) - o0;
     nb = nb < kOrigins ? nb : kOrigins;
+    const double qnan = __longlong_as_double(0x7ff8000000000000ll);
     double ox[kOrigins], oy[kOrigins];
     float fx[kOrigins], fy[kOrigins];
     uint32_t cnt[kOrigins];
-    double rx = g.xs[o0], ry = g.ys[o0];
+    double rx = sgpr_d(g.xs[o0]), ry = sgpr_d(g.ys[o0]);
 #pragma unroll
     for (int b = 0; b < kOrigins; b++) {
-        uint32_t ob = o0 + (b < (int)nb ? b : nb - 1);
-        ox[b] = g.xs[ob];
-        oy[b] = g.ys[ob];
+        uint32_t ob = o0 + (b < (int)nb ? b : 0);
+        // padded origins are NaN: they accept nothing
+        ox[b] = b < (int)nb ? sgpr_d(g.xs[ob]) : qnan;
+        oy[b] = sgpr_d(g.ys[ob]);
         fx[b] = (float)__dsub_rn(ox[b], rx);
         fy[b] = (float)__dsub_rn(oy[b], ry);
         cnt[b] = 0;
@@ -133,7 +190,7 @@ __global__ void __launch_bounds__(kBlock)
             for (int b = 0; b < kOrigins; b++) {
                 float s = dist2_f32(fx[b], fy[b], gx, gy);
                 bool acc = s <= band.lo;
-                uint64_t amb = ballot(v && b < (int)nb && !acc && s <= band.hi);
+                uint64_t amb = ballot(!acc && s <= band.hi);
                 if (amb) {
                     if ((amb >> lane_id()) & 1ull) acc = dist2_exact(ox[b], oy[b], qx, qy) <= T;
                     nre += popc64(amb);
@@ -153,7 +210,7 @@ __global__ void __launch_bounds__(kBlock)
 // ----------------------------------------------------------------- table
 // ids[row_off[p] + k] = idmap[q] (or q when idmap is null) for the k-th
 // accepted candidate q of origin p; rows located through rowmap.
-template <int PREC>
+template <int PREC, bool D2>
 __global__ void __launch_bounds__(kBlock)
     k_fill(GridView g, const uint32_t *__restrict__ item_cell,
            const uint32_t *__restrict__ item_o0, uint32_t it0, uint32_t it1, double T,
@@ -163,68 +220,66 @@ __global__ void __launch_bounds__(kBlock)
 {
     uint32_t it = it0 + blockIdx.x * kWavesPerBlock + wave_id_in_block();
     if (it >= it1) return;
-    uint32_t c = item_cell[it], o0 = item_o0[it];
-    uint32_t nb = g.cell_start[c + 1] - o0;
+    uint32_t c = sgpr_u32(item_cell[it]), o0 = sgpr_u32(item_o0[it]);
+    uint32_t nb = sgpr_u32(g.cell_start[c + 1]) - o0;
     nb = nb < kOrigins ? nb : kOrigins;
+    const double qnan = __longlong_as_double(0x7ff8000000000000ll);
     double ox[kOrigins], oy[kOrigins];
     float fx[kOrigins], fy[kOrigins];
     uint64_t run[kOrigins], lim[kOrigins];
-    double rx = g.xs[o0], ry = g.ys[o0];
+    double rx = sgpr_d(g.xs[o0]), ry = sgpr_d(g.ys[o0]);
 #pragma unroll
     for (int b = 0; b < kOrigins; b++) {
-        uint32_t ob = o0 + (b < (int)nb ? b : nb - 1);
-        ox[b] = g.xs[ob];
-        oy[b] = g.ys[ob];
+        uint32_t ob = o0 + (b < (int)nb ? b : 0);
+        ox[b] = b < (int)nb ? sgpr_d(g.xs[ob]) : qnan;
+        oy[b] = sgpr_d(g.ys[ob]);
         fx[b] = (float)__dsub_rn(ox[b], rx);
         fy[b] = (float)__dsub_rn(oy[b], ry);
         uint32_t row = rowmap ? rowmap[ob] : ob;
-        run[b] = offsets[row] - off_base;
-        lim[b] = offsets[row + 1] - off_base;
+        run[b] = sgpr_u64(offsets[row] - off_base);
+        lim[b] = sgpr_u64(offsets[row + 1] - off_base);
     }
     bool over = false;
-    window_blocks(g, c, false, 0, [&](uint32_t q, bool v, double qx, double qy) {
-        uint32_t id = v ? (idmap ? idmap[q] : q) : 0u;
-        float gx = 0.f, gy = 0.f;
-        if (PREC == 1) {
-            gx = (float)__dsub_rn(qx, rx);
-            gy = (float)__dsub_rn(qy, ry);
-        }
+    // origin-major over a register chunk: each origin's accepted ids of the
+    // chunk go out as one contiguous run before the next origin's
+    window_chunks(g, c, [&](uint32_t q0, int nblk, const double *qx, const double *qy) {
 #pragma unroll
         for (int b = 0; b < kOrigins; b++) {
-            if (b >= (int)nb) break;
-            double s = 0.0;
-            bool acc;
-            if (PREC == 0) {
-                s = dist2_exact(ox[b], oy[b], qx, qy);
-                acc = s <= T;
-            } else {
-                float s32 = dist2_f32(fx[b], fy[b], gx, gy);
-                acc = s32 <= band.lo;
-                bool amb = v && !acc && s32 <= band.hi;
-                if (ballot(amb)) {
-                    if (amb) acc = dist2_exact(ox[b], oy[b], qx, qy) <= T;
+#pragma unroll
+            for (int i = 0; i < kChunk; i++) {
+                if (i >= nblk) break;
+                uint32_t q = q0 + i * kWave + lane_id();
+                double s = 0.0;
+                bool acc;
+                if (PREC == 0) {
+                    s = dist2_exact(ox[b], oy[b], qx[i], qy[i]);
+                    acc = s <= T;
+                } else {
+                    float gx = (float)__dsub_rn(qx[i], rx), gy = (float)__dsub_rn(qy[i], ry);
+                    float s32 = dist2_f32(fx[b], fy[b], gx, gy);
+                    acc = s32 <= band.lo;
+                    bool amb = !acc && s32 <= band.hi;
+                    if (ballot(amb)) {
+                        if (amb) acc = dist2_exact(ox[b], oy[b], qx[i], qy[i]) <= T;
+                    }
+                    if (D2) s = dist2_exact(ox[b], oy[b], qx[i], qy[i]);
                 }
-                s = (double)s32;
-            }
-            uint64_t m = ballot(acc);
-            if (m) {
-                if (acc) {
+                uint64_t m = ballot(acc);
+                if (m) {
                     uint64_t dst = run[b] + (uint64_t)mask_rank(m);
                     // the count pass sized every row; never write past it
-                    if (dst < lim[b]) {
-                        ids[dst] = id;
-                        if (d2)
-                            d2[dst] = PREC == 0 ? (float)s
-                                                : (float)dist2_exact(ox[b], oy[b], qx, qy);
-                    } else {
-                        over = true;
+                    bool w = acc && dst < lim[b];
+                    over |= acc && !w;
+                    if (w) {
+                        ids[dst] = idmap ? idmap[q] : q;
+                        if (D2) d2[dst] = (float)s;
                     }
+                    run[b] += (uint64_t)popc64(m);
                 }
-                run[b] += (uint64_t)popc64(m);
             }
         }
     });
-    if (over) atomicOr(overflow, 1u);
+    if (ballot(over) && lane_id() == 0) atomicOr(overflow, 1u);
 }
 
 // ------------------------------------------------------------ union-find
@@ -650,7 +705,7 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
                 return kErrNoMemory;
             }
             if (pt) tm.start();
-            k_fill<0><<<ceil_div(i1 - i0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
+            k_fill<0, false><<<ceil_div(i1 - i0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
                 view(G), I.cell, I.o0, i0, i1, T, Band{}, offsets, base, nullptr, nullptr, ids,
                 nullptr, ovf);
             VDB_HIP(hipGetLastError());
@@ -681,7 +736,7 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
             uint32_t p0 = hio0[i0];
             if (ntiles > 1) {
                 // the table no longer holds this tile: rebuild it
-                k_fill<0><<<ceil_div(i1 - i0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
+                k_fill<0, false><<<ceil_div(i1 - i0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
                     view(G), I.cell, I.o0, i0, i1, T, Band{}, offsets, base, nullptr, nullptr,
                     ids, nullptr, ovf);
                 VDB_HIP(hipGetLastError());
@@ -767,14 +822,23 @@ int neighbor_table(Context &c, const Points &P, const Grid &G, const Items &I, d
     VDB_TRY(c.get("nt.ovf", 1, &ovf));
     VDB_HIP(hipMemsetAsync(ovf, 0, sizeof(uint32_t), c.stream()));
     unsigned blocks = ceil_div(I.n, kWavesPerBlock);
-    if (precision == 0)
-        k_fill<0><<<blocks, kBlock, 0, c.stream()>>>(view(G), I.cell, I.o0, 0, I.n, T, Band{},
-                                                     offsets, 0, caller_of_pos, caller_of_pos,
-                                                     ids, d2, ovf);
+    Band bd = precision ? band_for(G, T) : Band{};
+    if (precision == 0 && !d2)
+        k_fill<0, false><<<blocks, kBlock, 0, c.stream()>>>(
+            view(G), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
+            d2, ovf);
+    else if (precision == 0)
+        k_fill<0, true><<<blocks, kBlock, 0, c.stream()>>>(
+            view(G), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
+            d2, ovf);
+    else if (!d2)
+        k_fill<1, false><<<blocks, kBlock, 0, c.stream()>>>(
+            view(G), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
+            d2, ovf);
     else
-        k_fill<1><<<blocks, kBlock, 0, c.stream()>>>(view(G), I.cell, I.o0, 0, I.n, T,
-                                                     band_for(G, T), offsets, 0, caller_of_pos,
-                                                     caller_of_pos, ids, d2, ovf);
+        k_fill<1, true><<<blocks, kBlock, 0, c.stream()>>>(
+            view(G), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
+            d2, ovf);
     VDB_HIP(hipGetLastError());
     if (pt) pt->fill += tm.stop();
     uint32_t hov = 0;
