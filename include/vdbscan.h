@@ -116,6 +116,16 @@ int vdb_neighbor_table(const double *x, const double *y, uint32_t n, double eps,
                        uint64_t *offsets, uint32_t *ids, float *d2, uint64_t capacity,
                        uint64_t *total, uint32_t flags, int precision, vdb_stats *stats);
 
+/* Order-independent digest of the eps-neighbour table of vdb_neighbor_table
+ * (caller order): sum over every (row i, id j) pair, self pairs included, of
+ * splitmix64((i << 32) | j), mod 2^64, computed on the device without
+ * storing the table. *pairs = table size, *rechecked = pairs decided by the
+ * binary64 recheck (VDB_PREC_FP32_RECHECK). No counterpart in the reference:
+ * it checks tables too large to hold twice (e.g. FP32+recheck vs FP64). */
+int vdb_neighbor_table_digest(const double *x, const double *y, uint32_t n, double eps,
+                              uint32_t flags, int precision, uint64_t *digest,
+                              uint64_t *pairs, uint64_t *rechecked);
+
 #ifdef __cplusplus
 }
 #endif

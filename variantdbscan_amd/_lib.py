@@ -19,7 +19,8 @@ PATH_AUTO, PATH_FUSED, PATH_TABLE, PATH_BATCH = 0, 1, 2, 3
 PREC_FP64, PREC_FP32_RECHECK = 0, 1
 
 EXPORTED = ("libVDBSCAN", "vdb_last_error", "vdb_version", "vdb_set_stream", "vdb_release",
-            "vdb_cluster", "vdb_rank_order", "vdb_neighbor_counts", "vdb_neighbor_table")
+            "vdb_cluster", "vdb_rank_order", "vdb_neighbor_counts", "vdb_neighbor_table",
+            "vdb_neighbor_table_digest")
 
 
 class VdbOptions(ctypes.Structure):
@@ -80,6 +81,10 @@ def load():
                                      ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
                                      ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(VdbStats)]
     L.vdb_neighbor_table.restype = ctypes.c_int
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    L.vdb_neighbor_table_digest.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_double,
+                                            ctypes.c_uint32, ctypes.c_int, u64p, u64p, u64p]
+    L.vdb_neighbor_table_digest.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -176,3 +181,32 @@ def neighbor_table(x, y, eps, with_d2=False, precision=PREC_FP64):
 def last_error():
     msg = load().vdb_last_error()
     return msg.decode() if msg else ""
+
+
+def table_digest(x, y, eps, precision=PREC_FP64, device=False, n=None):
+    """(digest, pairs, rechecked) of the caller-order eps-neighbour table.
+    With device=True, x and y are raw device pointers and n the point count."""
+    L = load()
+    d, p, r = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)
+    if device:
+        _check(L.vdb_neighbor_table_digest(ctypes.c_void_p(x), ctypes.c_void_p(y), n,
+                                           float(eps), DEVICE_PTRS, precision, ctypes.byref(d),
+                                           ctypes.byref(p), ctypes.byref(r)))
+    else:
+        x, y = _f64(x), _f64(y)
+        _check(L.vdb_neighbor_table_digest(_ptr(x), _ptr(y), len(x), float(eps), 0, precision,
+                                           ctypes.byref(d), ctypes.byref(p), ctypes.byref(r)))
+    return d.value, p.value, r.value
+
+
+def digest_of_csr(offsets, ids):
+    """The same digest computed on the host from a CSR table (numpy, uint64)."""
+    offsets = np.asarray(offsets, dtype=np.int64)
+    rows = np.repeat(np.arange(len(offsets) - 1, dtype=np.uint64), np.diff(offsets))
+    z = (rows << np.uint64(32)) | np.asarray(ids, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = z + np.uint64(0x9e3779b97f4a7c15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xbf58476d1ce4e5b9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94d049bb133111eb)
+        z = z ^ (z >> np.uint64(31))
+    return int(z.sum(dtype=np.uint64))

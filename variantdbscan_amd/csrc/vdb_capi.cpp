@@ -396,6 +396,35 @@ int vdb_neighbor_table(const double *x, const double *y, uint32_t n, double eps,
     return kOk;
 }
 
+int vdb_neighbor_table_digest(const double *x, const double *y, uint32_t n, double eps,
+                              uint32_t flags, int precision, uint64_t *digest, uint64_t *pairs,
+                              uint64_t *rechecked)
+{
+    if (!digest || !pairs || !rechecked) {
+        set_error("NULL output argument");
+        return kErrInvalid;
+    }
+    *digest = *pairs = *rechecked = 0;
+    if (n == 0) return kOk;
+    if (!x || !y) {
+        set_error("NULL array argument");
+        return kErrInvalid;
+    }
+    Context *cp = nullptr;
+    VDB_TRY(get_context(&cp));
+    Context &c = *cp;
+    std::lock_guard<std::mutex> lock(c.mu);
+    Points P;
+    VDB_TRY(upload_points(c, x, y, n, flags, P));
+    VDB_TRY(build_rank_order(c, P, "pts"));
+    Grid G;
+    Items I;
+    bool enl;
+    VDB_TRY(build_grid(c, P, plain_cell_side(eps), 1, "g1", G, &enl));
+    VDB_TRY(build_items(c, G, "g1", I));
+    return table_digest(c, P, G, I, eps, precision, digest, pairs, rechecked);
+}
+
 int libVDBSCAN(double *inputx, double *inputy, unsigned int datasetSize, double *inputEpsilon,
                unsigned int *inputMinpts, unsigned int numVariants, int MBBsize,
                unsigned int *retArr, bool verbose)

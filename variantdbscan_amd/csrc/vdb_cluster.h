@@ -48,6 +48,11 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
 int pos_to_caller(Context &c, const Points &P, const Grid &G, const uint32_t *by_pos,
                   uint32_t *by_caller);
 
+// Order-independent digest of the caller-order neighbour table (sum over its
+// (row, id) pairs of a 64-bit mix, mod 2^64) without storing the table.
+int table_digest(Context &c, const Points &P, const Grid &G, const Items &I, double eps,
+                 int precision, uint64_t *digest, uint64_t *pairs, uint64_t *rechecked);
+
 // Neighbour table for external callers: rows and ids in caller order (ids
 // unsorted inside a row), optional float32 squared distances.
 int neighbor_table(Context &c, const Points &P, const Grid &G, const Items &I, double eps,

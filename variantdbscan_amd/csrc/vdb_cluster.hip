@@ -210,7 +210,18 @@ __global__ void __launch_bounds__(kBlock)
 // ----------------------------------------------------------------- table
 // ids[row_off[p] + k] = idmap[q] (or q when idmap is null) for the k-th
 // accepted candidate q of origin p; rows located through rowmap.
-template <int PREC, bool D2>
+// Order-independent digest of a neighbour table: the sum over all (row, id)
+// pairs of a 64-bit mix of row*2^32+id (splitmix64 finalizer), mod 2^64.
+__device__ __forceinline__ uint64_t pair_mix(uint64_t row, uint64_t id)
+{
+    uint64_t z = (row << 32) | id;
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+template <int PREC, bool D2, bool DIGEST = false>
 __global__ void __launch_bounds__(kBlock)
     k_fill(GridView g, const uint32_t *__restrict__ item_cell,
            const uint32_t *__restrict__ item_o0, uint32_t it0, uint32_t it1, double T,
@@ -227,6 +238,7 @@ __global__ void __launch_bounds__(kBlock)
     double ox[kOrigins], oy[kOrigins];
     float fx[kOrigins], fy[kOrigins];
     uint64_t run[kOrigins], lim[kOrigins];
+    uint32_t rowid[kOrigins];
     double rx = sgpr_d(g.xs[o0]), ry = sgpr_d(g.ys[o0]);
 #pragma unroll
     for (int b = 0; b < kOrigins; b++) {
@@ -236,10 +248,14 @@ __global__ void __launch_bounds__(kBlock)
         fx[b] = (float)__dsub_rn(ox[b], rx);
         fy[b] = (float)__dsub_rn(oy[b], ry);
         uint32_t row = rowmap ? rowmap[ob] : ob;
-        run[b] = sgpr_u64(offsets[row] - off_base);
-        lim[b] = sgpr_u64(offsets[row + 1] - off_base);
+        rowid[b] = sgpr_u32(row);
+        if (!DIGEST) {
+            run[b] = sgpr_u64(offsets[row] - off_base);
+            lim[b] = sgpr_u64(offsets[row + 1] - off_base);
+        }
     }
     bool over = false;
+    uint64_t dig = 0;
     // origin-major over a register chunk: each origin's accepted ids of the
     // chunk go out as one contiguous run before the next origin's
     window_chunks(g, c, [&](uint32_t q0, int nblk, const double *qx, const double *qy) {
@@ -264,6 +280,10 @@ __global__ void __launch_bounds__(kBlock)
                     }
                     if (D2) s = dist2_exact(ox[b], oy[b], qx[i], qy[i]);
                 }
+                if (DIGEST) {
+                    if (acc) dig += pair_mix(rowid[b], idmap ? idmap[q] : q);
+                    continue;
+                }
                 uint64_t m = ballot(acc);
                 if (m) {
                     uint64_t dst = run[b] + (uint64_t)mask_rank(m);
@@ -279,6 +299,11 @@ __global__ void __launch_bounds__(kBlock)
             }
         }
     });
+    if (DIGEST) {
+        for (int o = kWave / 2; o > 0; o >>= 1) dig += __shfl_xor(dig, o);
+        if (lane_id() == 0) atomicAdd((unsigned long long *)ids, (unsigned long long)dig);
+        return;
+    }
     if (ballot(over) && lane_id() == 0) atomicOr(overflow, 1u);
 }
 
@@ -777,6 +802,52 @@ int pos_to_caller(Context &c, const Points &P, const Grid &G, const uint32_t *by
     k_pos_to_caller<<<ceil_div(P.n, kBlock), kBlock, 0, c.stream()>>>(by_pos, G.pos_rank,
                                                                      P.mapping, P.n, by_caller);
     VDB_HIP(hipGetLastError());
+    return kOk;
+}
+
+int table_digest(Context &c, const Points &P, const Grid &G, const Items &I, double eps,
+                 int precision, uint64_t *digest, uint64_t *pairs, uint64_t *rechecked)
+{
+    uint32_t n = P.n;
+    double T = threshold_for_eps(eps);
+    uint32_t *counts, *caller_of_pos;
+    VDB_TRY(c.get("nt.counts", n, &counts));
+    VDB_TRY(c.get("nt.caller_of_pos", n, &caller_of_pos));
+    unsigned long long *acc;
+    VDB_TRY(c.get("nt.digest", 2, &acc));
+    VDB_HIP(hipMemsetAsync(acc, 0, 2 * sizeof(unsigned long long), c.stream()));
+    VDB_TRY(count_neighbors(c, G, I, eps, precision, counts, (uint64_t *)(acc + 1)));
+    uint32_t *dummy;
+    VDB_TRY(c.get("nt.counts_caller", n, &dummy));
+    k_perm_counts<<<ceil_div(n, kBlock), kBlock, 0, c.stream()>>>(counts, G.pos_rank, P.mapping,
+                                                                 n, dummy, caller_of_pos);
+    VDB_HIP(hipGetLastError());
+    uint64_t *tot;
+    VDB_TRY(c.get("nt.tot", 1, &tot));
+    uint64_t *offs;
+    VDB_TRY(c.get("nt.off", (size_t)n + 1, &offs));
+    VDB_TRY(scan_exclusive_u32_u64(c, dummy, offs, n, tot));
+    if (I.n) {
+        unsigned blocks = ceil_div(I.n, kWavesPerBlock);
+        Band bd = precision ? band_for(G, T) : Band{};
+        if (precision == 0)
+            k_fill<0, false, true><<<blocks, kBlock, 0, c.stream()>>>(
+                view(G), I.cell, I.o0, 0, I.n, T, bd, nullptr, 0, caller_of_pos, caller_of_pos,
+                (uint32_t *)acc, nullptr, nullptr);
+        else
+            k_fill<1, false, true><<<blocks, kBlock, 0, c.stream()>>>(
+                view(G), I.cell, I.o0, 0, I.n, T, bd, nullptr, 0, caller_of_pos, caller_of_pos,
+                (uint32_t *)acc, nullptr, nullptr);
+        VDB_HIP(hipGetLastError());
+    }
+    unsigned long long h[2];
+    VDB_HIP(hipMemcpyAsync(h, acc, sizeof h, hipMemcpyDeviceToHost, c.stream()));
+    uint64_t t = 0;
+    VDB_HIP(hipMemcpyAsync(&t, tot, sizeof t, hipMemcpyDeviceToHost, c.stream()));
+    VDB_HIP(hipStreamSynchronize(c.stream()));
+    *digest = h[0];
+    *rechecked = h[1];
+    *pairs = t;
     return kOk;
 }
 
