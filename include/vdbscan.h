@@ -95,6 +95,17 @@ int vdb_cluster(const double *x, const double *y, uint32_t n, const double *eps,
                 const uint32_t *minpts, uint32_t nvar, uint32_t *labels,
                 uint32_t flags, const vdb_options *opt, vdb_stats *stats);
 
+/* Many independent datasets in one call: dataset d is points
+ * [offsets[d], offsets[d+1]) of x, y (offsets has nsets+1 entries, offsets[0]
+ * = 0), every dataset clustered under the same nvar variants. Labels of
+ * dataset d start at labels + offsets[d]*nvar and are laid out as that
+ * dataset's own retArr (nvar rows of its point count). No counterpart in the
+ * reference, which takes one dataset per libVDBSCAN call. */
+int vdb_cluster_multi(const double *x, const double *y, const uint64_t *offsets,
+                      uint32_t nsets, const double *eps, const uint32_t *minpts,
+                      uint32_t nvar, uint32_t *labels, uint32_t flags,
+                      const vdb_options *opt, vdb_stats *stats);
+
 /* Reference visit order: mapping[k] = caller index of the point at binned
  * position k (src/main.cpp:459-605, 300x300 bins). */
 int vdb_rank_order(const double *x, const double *y, uint32_t n, uint32_t *mapping,

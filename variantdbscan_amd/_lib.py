@@ -20,7 +20,7 @@ PREC_FP64, PREC_FP32_RECHECK = 0, 1
 
 EXPORTED = ("libVDBSCAN", "vdb_last_error", "vdb_version", "vdb_set_stream", "vdb_release",
             "vdb_cluster", "vdb_rank_order", "vdb_neighbor_counts", "vdb_neighbor_table",
-            "vdb_neighbor_table_digest")
+            "vdb_neighbor_table_digest", "vdb_cluster_multi")
 
 
 class VdbOptions(ctypes.Structure):
@@ -85,6 +85,10 @@ def load():
     L.vdb_neighbor_table_digest.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_double,
                                             ctypes.c_uint32, ctypes.c_int, u64p, u64p, u64p]
     L.vdb_neighbor_table_digest.restype = ctypes.c_int
+    L.vdb_cluster_multi.argtypes = [vp, vp, vp, ctypes.c_uint32, vp, vp, ctypes.c_uint32, vp,
+                                    ctypes.c_uint32, ctypes.POINTER(VdbOptions),
+                                    ctypes.POINTER(VdbStats)]
+    L.vdb_cluster_multi.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -210,3 +214,41 @@ def digest_of_csr(offsets, ids):
         z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94d049bb133111eb)
         z = z ^ (z >> np.uint64(31))
     return int(z.sum(dtype=np.uint64))
+
+
+def cluster_multi(sets, eps, minpts, path=PATH_AUTO, precision=PREC_FP64, return_stats=False):
+    """Cluster a list of (x, y) datasets under the same variants in one call.
+    Returns a list of (V, n_d) uint32 label arrays."""
+    L = load()
+    eps = _f64(np.atleast_1d(eps))
+    mp = _u32(np.atleast_1d(minpts))
+    sizes = np.array([len(s[0]) for s in sets], dtype=np.uint64)
+    offs = np.zeros(len(sets) + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum(sizes)
+    x = _f64(np.concatenate([s[0] for s in sets])) if sets else np.zeros(0)
+    y = _f64(np.concatenate([s[1] for s in sets])) if sets else np.zeros(0)
+    V = len(eps)
+    out = np.zeros(int(offs[-1]) * V, dtype=np.uint32)
+    opt = VdbOptions(path, precision, 0, 0)
+    st = VdbStats()
+    _check(L.vdb_cluster_multi(_ptr(x), _ptr(y), _ptr(offs), len(sets), _ptr(eps), _ptr(mp), V,
+                               _ptr(out), 0, ctypes.byref(opt), ctypes.byref(st)))
+    res = [out[int(offs[d]) * V:int(offs[d + 1]) * V].reshape(V, int(sizes[d]))
+           for d in range(len(sets))]
+    return (res, st.as_dict()) if return_stats else res
+
+
+def cluster_multi_device(x_ptr, y_ptr, offsets, eps, minpts, labels_ptr, path=PATH_AUTO,
+                         precision=PREC_FP64):
+    """Device-resident variant of cluster_multi (offsets is a host array)."""
+    L = load()
+    eps = _f64(np.atleast_1d(eps))
+    mp = _u32(np.atleast_1d(minpts))
+    offs = np.ascontiguousarray(offsets, dtype=np.uint64)
+    opt = VdbOptions(path, precision, 0, 0)
+    st = VdbStats()
+    _check(L.vdb_cluster_multi(ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), _ptr(offs),
+                               len(offs) - 1, _ptr(eps), _ptr(mp), len(eps),
+                               ctypes.c_void_p(labels_ptr), DEVICE_PTRS, ctypes.byref(opt),
+                               ctypes.byref(st)))
+    return st.as_dict()

@@ -131,35 +131,18 @@ int run_per_variant(Context &c, Points &P, const double *eps, const uint32_t *mi
     return kOk;
 }
 
-int cluster_impl(const double *x, const double *y, uint32_t n, const double *eps,
-                 const uint32_t *minpts, uint32_t nvar, uint32_t *labels, uint32_t flags,
-                 const vdb_options *opt, vdb_stats *st)
+// Clusters the device-resident points P (x, y set) under nvar variants into
+// out (device, retArr layout); returns the path used in *used.
+int cluster_points(Context &c, Points &P, const double *eps, const uint32_t *minpts,
+                   uint32_t nvar, uint32_t *out, const vdb_options *opt, vdb_stats *st,
+                   int *used_out)
 {
-    if (st) memset(st, 0, sizeof(*st));
-    if (n == 0 || nvar == 0) return kOk;
-    if (!x || !y || !eps || !minpts || !labels) {
-        set_error("NULL array argument");
-        return kErrInvalid;
-    }
-    if ((uint64_t)n * nvar > 0xffffffffull) {
-        set_error("datasetSize*numVariants exceeds 2^32 labels");
-        return kErrInvalid;
-    }
-    Context *cp = nullptr;
-    VDB_TRY(get_context(&cp));
-    Context &c = *cp;
-    std::lock_guard<std::mutex> lock(c.mu);
-    Timer total(c.stream());
     int path = opt ? opt->path : VDB_PATH_AUTO;
     int precision = opt ? opt->precision : VDB_PREC_FP64;
     bool verbose = opt && opt->verbose;
-    Points P;
-    VDB_TRY(upload_points(c, x, y, n, flags, P));
     Timer tprep(c.stream());
     VDB_TRY(build_rank_order(c, P, "pts"));
     if (st) st->ms_prepare += tprep.ms();
-    uint32_t *out = labels;
-    if (!(flags & VDB_DEVICE_PTRS)) VDB_TRY(c.get("out.labels", (size_t)n * nvar, &out));
     int used = path;
     // variants left for the per-variant engine (all of them unless batched)
     std::vector<uint32_t> rest;
@@ -215,6 +198,35 @@ int cluster_impl(const double *x, const double *y, uint32_t n, const double *eps
         VDB_TRY(run_per_variant(c, P, eps, minpts, rest, out, path == VDB_PATH_TABLE ? 1 : 0,
                                 precision, default_budget(c, opt), st, verbose));
     }
+    *used_out = used;
+    return kOk;
+}
+
+int cluster_impl(const double *x, const double *y, uint32_t n, const double *eps,
+                 const uint32_t *minpts, uint32_t nvar, uint32_t *labels, uint32_t flags,
+                 const vdb_options *opt, vdb_stats *st)
+{
+    if (st) memset(st, 0, sizeof(*st));
+    if (n == 0 || nvar == 0) return kOk;
+    if (!x || !y || !eps || !minpts || !labels) {
+        set_error("NULL array argument");
+        return kErrInvalid;
+    }
+    if ((uint64_t)n * nvar > 0xffffffffull) {
+        set_error("datasetSize*numVariants exceeds 2^32 labels");
+        return kErrInvalid;
+    }
+    Context *cp = nullptr;
+    VDB_TRY(get_context(&cp));
+    Context &c = *cp;
+    std::lock_guard<std::mutex> lock(c.mu);
+    Timer total(c.stream());
+    Points P;
+    VDB_TRY(upload_points(c, x, y, n, flags, P));
+    uint32_t *out = labels;
+    if (!(flags & VDB_DEVICE_PTRS)) VDB_TRY(c.get("out.labels", (size_t)n * nvar, &out));
+    int used = 0;
+    VDB_TRY(cluster_points(c, P, eps, minpts, nvar, out, opt, st, &used));
     if (!(flags & VDB_DEVICE_PTRS)) {
         VDB_HIP(hipMemcpyAsync(labels, out, sizeof(uint32_t) * (size_t)n * nvar,
                                hipMemcpyDeviceToHost, c.stream()));
@@ -258,6 +270,60 @@ int vdb_cluster(const double *x, const double *y, uint32_t n, const double *eps,
                 const vdb_options *opt, vdb_stats *stats)
 {
     return cluster_impl(x, y, n, eps, minpts, nvar, labels, flags, opt, stats);
+}
+
+int vdb_cluster_multi(const double *x, const double *y, const uint64_t *offsets,
+                      uint32_t nsets, const double *eps, const uint32_t *minpts, uint32_t nvar,
+                      uint32_t *labels, uint32_t flags, const vdb_options *opt,
+                      vdb_stats *st)
+{
+    if (st) memset(st, 0, sizeof(*st));
+    if (nsets == 0 || nvar == 0) return kOk;
+    if (!x || !y || !offsets || !eps || !minpts || !labels) {
+        set_error("NULL array argument");
+        return kErrInvalid;
+    }
+    uint64_t ntot = offsets[nsets];
+    for (uint32_t d = 0; d < nsets; d++)
+        if (offsets[d + 1] < offsets[d]) {
+            set_error("dataset offsets must not decrease");
+            return kErrInvalid;
+        }
+    if (ntot * nvar > 0xffffffffull || offsets[0] != 0) {
+        set_error("offsets[0] must be 0 and total points*variants below 2^32");
+        return kErrInvalid;
+    }
+    if (ntot == 0) return kOk;
+    Context *cp = nullptr;
+    VDB_TRY(get_context(&cp));
+    Context &c = *cp;
+    std::lock_guard<std::mutex> lock(c.mu);
+    Timer total(c.stream());
+    Points all;
+    VDB_TRY(upload_points(c, x, y, (uint32_t)ntot, flags, all));
+    uint32_t *out = labels;
+    if (!(flags & VDB_DEVICE_PTRS)) VDB_TRY(c.get("out.labels", (size_t)ntot * nvar, &out));
+    int used = 0;
+    for (uint32_t d = 0; d < nsets; d++) {
+        uint32_t nd = (uint32_t)(offsets[d + 1] - offsets[d]);
+        if (nd == 0) continue;
+        Points P;
+        P.n = nd;
+        P.x = all.x + offsets[d];
+        P.y = all.y + offsets[d];
+        // dataset d's block is its own retArr: nvar rows of nd labels
+        VDB_TRY(cluster_points(c, P, eps, minpts, nvar, out + offsets[d] * nvar, opt, st,
+                               &used));
+    }
+    if (!(flags & VDB_DEVICE_PTRS))
+        VDB_HIP(hipMemcpyAsync(labels, out, sizeof(uint32_t) * (size_t)ntot * nvar,
+                               hipMemcpyDeviceToHost, c.stream()));
+    VDB_HIP(hipStreamSynchronize(c.stream()));
+    if (st) {
+        st->ms_total = total.ms();
+        st->path_used = used;
+    }
+    return kOk;
 }
 
 int vdb_rank_order(const double *x, const double *y, uint32_t n, uint32_t *mapping,

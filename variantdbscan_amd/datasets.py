@@ -47,6 +47,8 @@ def _inside(x, y):
 
 def _blob(rng, n, cx, cy, sx, sy, theta):
     """n samples of a rotated anisotropic Gaussian, redrawn until in the box."""
+    if n <= 0:
+        return np.zeros(0), np.zeros(0)
     xs, ys = [], []
     need = n
     c, s = np.cos(theta), np.sin(theta)
