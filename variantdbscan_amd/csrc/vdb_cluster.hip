@@ -27,11 +27,12 @@ struct GridView {
     const double *xs, *ys;
     const uint32_t *cell_start, *row_lo, *row_hi;
     int W;
+    uint32_t n;
 };
 
-GridView view(const Grid &G)
+GridView view(const Grid &G, uint32_t n)
 {
-    return GridView{G.xs, G.ys, G.cell_start, G.row_lo, G.row_hi, 2 * G.R + 1};
+    return GridView{G.xs, G.ys, G.cell_start, G.row_lo, G.row_hi, 2 * G.R + 1, n};
 }
 
 // Visit the window of cell c in 64-candidate blocks. With `half`, only rows
@@ -259,6 +260,14 @@ __global__ void __launch_bounds__(kBlock)
     // origin-major over a register chunk: each origin's accepted ids of the
     // chunk go out as one contiguous run before the next origin's
     window_chunks(g, c, [&](uint32_t q0, int nblk, const double *qx, const double *qy) {
+        // ids of the chunk's candidates, loaded before any store: a load
+        // between stores would make every store wait for all earlier ones
+        uint32_t qid[kChunk];
+#pragma unroll
+        for (int i = 0; i < kChunk; i++) {
+            uint32_t q = q0 + i * kWave + lane_id();
+            qid[i] = (idmap && i < nblk) ? idmap[q < g.n ? q : 0] : q;
+        }
 #pragma unroll
         for (int b = 0; b < kOrigins; b++) {
 #pragma unroll
@@ -281,7 +290,7 @@ __global__ void __launch_bounds__(kBlock)
                     if (D2) s = dist2_exact(ox[b], oy[b], qx[i], qy[i]);
                 }
                 if (DIGEST) {
-                    if (acc) dig += pair_mix(rowid[b], idmap ? idmap[q] : q);
+                    if (acc) dig += pair_mix(rowid[b], qid[i]);
                     continue;
                 }
                 uint64_t m = ballot(acc);
@@ -291,7 +300,7 @@ __global__ void __launch_bounds__(kBlock)
                     bool w = acc && dst < lim[b];
                     over |= acc && !w;
                     if (w) {
-                        ids[dst] = idmap ? idmap[q] : q;
+                        ids[dst] = qid[i];
                         if (D2) d2[dst] = (float)s;
                     }
                     run[b] += (uint64_t)popc64(m);
@@ -627,10 +636,10 @@ int count_neighbors(Context &c, const Grid &G, const Items &I, double eps, int p
     if (I.n == 0) return kOk;
     unsigned blocks = ceil_div(I.n, kWavesPerBlock);
     if (precision == 0) {
-        k_count<0><<<blocks, kBlock, 0, c.stream()>>>(view(G), I.cell, I.o0, I.n, T, Band{},
+        k_count<0><<<blocks, kBlock, 0, c.stream()>>>(view(G, G.npts), I.cell, I.o0, I.n, T, Band{},
                                                       counts, nullptr);
     } else {
-        k_count<1><<<blocks, kBlock, 0, c.stream()>>>(view(G), I.cell, I.o0, I.n, T,
+        k_count<1><<<blocks, kBlock, 0, c.stream()>>>(view(G, G.npts), I.cell, I.o0, I.n, T,
                                                       band_for(G, T), counts,
                                                       (unsigned long long *)recheck_dev);
     }
@@ -674,14 +683,14 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
     if (mode == 0) {
         if (pt) tm.start();
         if (I.n)
-            k_uf_fused<<<blocks, kBlock, 0, c.stream()>>>(view(G), I.cell, I.o0, I.n, T, core,
+            k_uf_fused<<<blocks, kBlock, 0, c.stream()>>>(view(G, G.npts), I.cell, I.o0, I.n, T, core,
                                                           G.pos_rank, parent);
         VDB_HIP(hipGetLastError());
         k_compress<<<g, kBlock, 0, c.stream()>>>(parent, core, n);
         VDB_HIP(hipGetLastError());
         if (pt) pt->uf += tm.stop(), tm.start();
         if (I.n)
-            k_border_fused<<<blocks, kBlock, 0, c.stream()>>>(view(G), I.cell, I.o0, I.n, T,
+            k_border_fused<<<blocks, kBlock, 0, c.stream()>>>(view(G, G.npts), I.cell, I.o0, I.n, T,
                                                               core, G.pos_rank, parent, best);
         VDB_HIP(hipGetLastError());
         if (pt) pt->border += tm.stop();
@@ -731,7 +740,7 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
             }
             if (pt) tm.start();
             k_fill<0, false><<<ceil_div(i1 - i0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
-                view(G), I.cell, I.o0, i0, i1, T, Band{}, offsets, base, nullptr, nullptr, ids,
+                view(G, G.npts), I.cell, I.o0, i0, i1, T, Band{}, offsets, base, nullptr, nullptr, ids,
                 nullptr, ovf);
             VDB_HIP(hipGetLastError());
             if (pt) pt->fill += tm.stop(), tm.start();
@@ -762,7 +771,7 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
             if (ntiles > 1) {
                 // the table no longer holds this tile: rebuild it
                 k_fill<0, false><<<ceil_div(i1 - i0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
-                    view(G), I.cell, I.o0, i0, i1, T, Band{}, offsets, base, nullptr, nullptr,
+                    view(G, G.npts), I.cell, I.o0, i0, i1, T, Band{}, offsets, base, nullptr, nullptr,
                     ids, nullptr, ovf);
                 VDB_HIP(hipGetLastError());
             }
@@ -832,11 +841,11 @@ int table_digest(Context &c, const Points &P, const Grid &G, const Items &I, dou
         Band bd = precision ? band_for(G, T) : Band{};
         if (precision == 0)
             k_fill<0, false, true><<<blocks, kBlock, 0, c.stream()>>>(
-                view(G), I.cell, I.o0, 0, I.n, T, bd, nullptr, 0, caller_of_pos, caller_of_pos,
+                view(G, G.npts), I.cell, I.o0, 0, I.n, T, bd, nullptr, 0, caller_of_pos, caller_of_pos,
                 (uint32_t *)acc, nullptr, nullptr);
         else
             k_fill<1, false, true><<<blocks, kBlock, 0, c.stream()>>>(
-                view(G), I.cell, I.o0, 0, I.n, T, bd, nullptr, 0, caller_of_pos, caller_of_pos,
+                view(G, G.npts), I.cell, I.o0, 0, I.n, T, bd, nullptr, 0, caller_of_pos, caller_of_pos,
                 (uint32_t *)acc, nullptr, nullptr);
         VDB_HIP(hipGetLastError());
     }
@@ -896,19 +905,19 @@ int neighbor_table(Context &c, const Points &P, const Grid &G, const Items &I, d
     Band bd = precision ? band_for(G, T) : Band{};
     if (precision == 0 && !d2)
         k_fill<0, false><<<blocks, kBlock, 0, c.stream()>>>(
-            view(G), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
+            view(G, G.npts), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
             d2, ovf);
     else if (precision == 0)
         k_fill<0, true><<<blocks, kBlock, 0, c.stream()>>>(
-            view(G), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
+            view(G, G.npts), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
             d2, ovf);
     else if (!d2)
         k_fill<1, false><<<blocks, kBlock, 0, c.stream()>>>(
-            view(G), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
+            view(G, G.npts), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
             d2, ovf);
     else
         k_fill<1, true><<<blocks, kBlock, 0, c.stream()>>>(
-            view(G), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
+            view(G, G.npts), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
             d2, ovf);
     VDB_HIP(hipGetLastError());
     if (pt) pt->fill += tm.stop();

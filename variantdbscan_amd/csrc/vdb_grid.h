@@ -44,6 +44,7 @@ struct Grid {
     uint32_t *row_lo = nullptr;     // [ncell*(2R+1)] first cell of window row
     uint32_t *row_hi = nullptr;     // [ncell*(2R+1)] one past the last
     uint32_t max_cell = 0;          // largest cell population
+    uint32_t npts = 0;              // points in the grid
 };
 
 // Build a grid of cell side >= w_min over the rank-ordered points. If the key

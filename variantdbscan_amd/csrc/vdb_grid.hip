@@ -284,6 +284,7 @@ int build_grid(Context &c, const Points &P, double w_min, int R, const char *tag
     G.ncx = (uint32_t)std::floor(spanx / w) + 2;
     G.ncy = (uint32_t)std::floor(spany / w) + 2;
     G.R = R;
+    G.npts = n;
     uint64_t nkeys = (uint64_t)G.ncx * G.ncy;
 
     uint32_t *keys, *vals, *tk, *tv, *flags, *excl;
