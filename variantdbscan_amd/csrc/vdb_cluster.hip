@@ -230,8 +230,12 @@ __global__ void __launch_bounds__(kBlock)
            const uint32_t *__restrict__ rowmap, const uint32_t *__restrict__ idmap,
            uint32_t *__restrict__ ids, float *__restrict__ d2, uint32_t *overflow)
 {
-    uint32_t it = it0 + blockIdx.x * kWavesPerBlock + wave_id_in_block();
-    if (it >= it1) return;
+    // block-uniform: do the block's 4 items exist and share one cell?
+    uint32_t ib = it0 + blockIdx.x * kWavesPerBlock;
+    bool shared = ib + kWavesPerBlock <= it1 &&
+                  item_cell[ib] == item_cell[ib + kWavesPerBlock - 1];
+    uint32_t it = ib + wave_id_in_block();
+    if (!shared && it >= it1) return;
     uint32_t c = sgpr_u32(item_cell[it]), o0 = sgpr_u32(item_o0[it]);
     uint32_t nb = sgpr_u32(g.cell_start[c + 1]) - o0;
     nb = nb < kOrigins ? nb : kOrigins;
@@ -259,21 +263,13 @@ __global__ void __launch_bounds__(kBlock)
     uint64_t dig = 0;
     // origin-major over a register chunk: each origin's accepted ids of the
     // chunk go out as one contiguous run before the next origin's
-    window_chunks(g, c, [&](uint32_t q0, int nblk, const double *qx, const double *qy) {
-        // ids of the chunk's candidates, loaded before any store: a load
-        // between stores would make every store wait for all earlier ones
-        uint32_t qid[kChunk];
-#pragma unroll
-        for (int i = 0; i < kChunk; i++) {
-            uint32_t q = q0 + i * kWave + lane_id();
-            qid[i] = (idmap && i < nblk) ? idmap[q < g.n ? q : 0] : q;
-        }
+    auto chunk = [&](uint32_t q0, int nblk, const double *qx, const double *qy,
+                     const uint32_t *qid) {
 #pragma unroll
         for (int b = 0; b < kOrigins; b++) {
 #pragma unroll
             for (int i = 0; i < kChunk; i++) {
                 if (i >= nblk) break;
-                uint32_t q = q0 + i * kWave + lane_id();
                 double s = 0.0;
                 bool acc;
                 if (PREC == 0) {
@@ -307,7 +303,55 @@ __global__ void __launch_bounds__(kBlock)
                 }
             }
         }
-    });
+    };
+    if (shared) {
+        // the block's 4 items share one cell: stage each chunk of the window
+        // in LDS once for all 32 origins
+        __shared__ double sx[kChunk * kWave], sy[kChunk * kWave];
+        __shared__ uint32_t sid[kChunk * kWave];
+        const double qnan2 = __longlong_as_double(0x7ff8000000000000ll);
+        for (int r = 0; r < g.W; r++) {
+            uint32_t lo = g.row_lo[(size_t)c * g.W + r], hi = g.row_hi[(size_t)c * g.W + r];
+            if (lo >= hi) continue;
+            uint32_t a = g.cell_start[lo], e = g.cell_start[hi];
+            for (uint32_t q0 = a; q0 < e; q0 += kWave * kChunk) {
+                int nblk = (int)((e - q0 + kWave - 1) / kWave);
+                nblk = nblk < kChunk ? nblk : kChunk;
+                for (int j = threadIdx.x; j < kChunk * kWave; j += kBlock) {
+                    uint32_t q = q0 + j;
+                    bool v = q < e;
+                    uint32_t qc = v ? q : e - 1;
+                    double x = g.xs[qc], y = g.ys[qc];
+                    sx[j] = v ? x : qnan2;
+                    sy[j] = v ? y : qnan2;
+                    sid[j] = idmap ? idmap[qc] : q;
+                }
+                __syncthreads();
+                double qx[kChunk], qy[kChunk];
+                uint32_t qid[kChunk];
+#pragma unroll
+                for (int i = 0; i < kChunk; i++) {
+                    qx[i] = sx[i * kWave + lane_id()];
+                    qy[i] = sy[i * kWave + lane_id()];
+                    qid[i] = sid[i * kWave + lane_id()];
+                }
+                __syncthreads();
+                chunk(q0, nblk, qx, qy, qid);
+            }
+        }
+    } else {
+        window_chunks(g, c, [&](uint32_t q0, int nblk, const double *qx, const double *qy) {
+            // ids of the chunk's candidates, loaded before any store: a load
+            // between stores would make every store wait for all earlier ones
+            uint32_t qid[kChunk];
+#pragma unroll
+            for (int i = 0; i < kChunk; i++) {
+                uint32_t q = q0 + i * kWave + lane_id();
+                qid[i] = (idmap && i < nblk) ? idmap[q < g.n ? q : 0] : q;
+            }
+            chunk(q0, nblk, qx, qy, qid);
+        });
+    }
     if (DIGEST) {
         for (int o = kWave / 2; o > 0; o >>= 1) dig += __shfl_xor(dig, o);
         if (lane_id() == 0) atomicAdd((unsigned long long *)ids, (unsigned long long)dig);
