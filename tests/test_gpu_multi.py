@@ -36,6 +36,45 @@ def test_multi_small_against_oracle():
             assert (got[d] == oracle.libvdbscan(x, y, EPS, MP)).all(), d
 
 
+def test_multi_edge_datasets():
+    # empty datasets between others, one-point sets, a set of identical points,
+    # all-negative coordinates, isolated-point and never-core variants
+    rng = np.random.default_rng(11)
+    x0, y0 = datasets.blobs_isotropic(4000, 12, 0.5, 0.7, 77)
+    sets = [
+        (np.zeros(0), np.zeros(0)),
+        (x0, y0),
+        (np.zeros(0), np.zeros(0)),
+        (np.array([3.0]), np.array([-4.0])),
+        (np.full(50, 2.5), np.full(50, 2.5)),
+        (-np.abs(x0[:900]) - 1.0, -np.abs(y0[:900]) - 1.0),
+        (rng.uniform(0, 2, 300), rng.uniform(0, 2, 300)),
+        (np.zeros(0), np.zeros(0)),
+    ]
+    eps = np.array([0.3, 0.3, 0.6, 1.0, -1.0, np.nan, 0.45, 0.45], dtype=np.float64)
+    mp = np.array([4, 0, 8, 16, 0, 3, 2**31 + 5, 1], dtype=np.uint32)
+    got = vd.cluster_multi(sets, eps, mp)
+    fused = vd.cluster_multi(sets, eps, mp, path=vd.PATH_FUSED)
+    for d, (x, y) in enumerate(sets):
+        assert got[d].shape == (len(eps), len(x))
+        assert (got[d] == fused[d]).all(), d
+        if len(x) == 0:
+            continue
+        assert (got[d] == vd.cluster(x, y, eps, mp)).all(), d
+        if len(x) >= 2 and not np.all(x == x[0]):
+            assert (got[d] == oracle.libvdbscan(x, y, eps, mp)).all(), d
+
+
+def test_multi_256_batch_equals_per_dataset_engine():
+    # the one-pass batched run against every dataset clustered on its own by
+    # the per-variant engine
+    sets = datasets.multi_256()
+    a = vd.cluster_multi(sets, EPS, MP)
+    b = vd.cluster_multi(sets, EPS, MP, path=vd.PATH_FUSED)
+    bad = [d for d in range(len(sets)) if not (a[d] == b[d]).all()]
+    assert not bad, bad
+
+
 def test_multi_256_uneven():
     sets = datasets.multi_256()
     ntot = sum(len(s[0]) for s in sets)

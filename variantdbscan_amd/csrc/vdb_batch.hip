@@ -393,7 +393,7 @@ struct PairRec {
 struct LevelArgs {
     const double *xs, *ys, *box;
     const uint32_t *cell_start, *cell_key;
-    uint32_t ncy;
+    const uint32_t *cell_ds, *skbase, *sncy; // dataset of a cell, its key base and ncy
     const uint8_t *kc;
     const uint8_t *cellkc;
     const uint32_t *pa, *pb;
@@ -541,9 +541,11 @@ __global__ void __launch_bounds__(kBlock)
                     if (small_pair_connected(a, A, B)) cf_unite(parent, A, B);
                     else kp = true;
                 } else {
-                    uint32_t ka = a.cell_key[A], kb = a.cell_key[B];
-                    int64_t dx = (int64_t)(ka / a.ncy) - (int64_t)(kb / a.ncy);
-                    int64_t dy = (int64_t)(ka % a.ncy) - (int64_t)(kb % a.ncy);
+                    // both cells belong to one dataset
+                    uint32_t ds = a.cell_ds[A], ncy = a.sncy[ds], k0 = a.skbase[ds];
+                    uint32_t ka = a.cell_key[A] - k0, kb = a.cell_key[B] - k0;
+                    int64_t dx = (int64_t)(ka / ncy) - (int64_t)(kb / ncy);
+                    int64_t dy = (int64_t)(ka % ncy) - (int64_t)(kb % ncy);
                     bool adj = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1;
                     tnear = adj;
                     tfar = !adj;
@@ -653,41 +655,6 @@ __global__ void k_root_flags_cells(const uint32_t *parent, const uint32_t *compm
     flag[compmin[c]] = 1u;
 }
 
-__global__ void k_batch_labels(const uint32_t *cell_of_pos, const uint32_t *parent,
-                               const uint32_t *compmin, const uint32_t *cid,
-                               const uint8_t *kc, int J, int j, int k, double T,
-                               const uint32_t *lst_idx, const double *lst_s,
-                               const uint32_t *lst_id, const uint32_t *lst_n, int L,
-                               const uint32_t *pos_rank, const uint32_t *mapping, uint32_t n,
-                               uint32_t *out)
-{
-    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= n) return;
-    uint32_t r = pos_rank[p];
-    uint32_t lbl = 0;
-    if (kc[(size_t)p * J + j] <= k) {
-        lbl = cid[compmin[parent[cell_of_pos[p]]]] + 1;
-    } else {
-        // non-core: count_k(p) < m_j <= L, so the list holds every eps_k
-        // neighbour of p, as the prefix with s <= T_k
-        uint32_t i = lst_idx[p];
-        uint32_t best = kNone;
-        if (i != kNone) {
-            uint32_t m = lst_n[i];
-            const double *s = lst_s + (size_t)i * L;
-            const uint32_t *id = lst_id + (size_t)i * L;
-            for (uint32_t e = 0; e < m && s[e] <= T; e++) {
-                uint32_t q = id[e];
-                if (kc[(size_t)q * J + j] > k) continue;
-                uint32_t rr = compmin[parent[cell_of_pos[q]]];
-                best = rr < best ? rr : best;
-            }
-        }
-        lbl = (best != kNone && best < r) ? cid[best] + 1 : 0u;
-    }
-    out[mapping[r]] = lbl;
-}
-
 // root rank of every cell at level k (NONE if the cell holds no core point)
 __global__ void k_snapshot_roots(const uint32_t *parent, const uint32_t *compmin,
                                  const uint8_t *cellkc, uint32_t ncell, int J, int j, int k,
@@ -699,34 +666,40 @@ __global__ void k_snapshot_roots(const uint32_t *parent, const uint32_t *compmin
 }
 
 // Labels of up to kLabelSlots variants sharing minpts j, from their level
-// snapshots. Thread per caller index: the point's static data are read once
-// and every output row is written coalesced.
+// snapshots. Output layout: dataset d (caller indices seg[d] .. seg[d+1]-1)
+// owns the block out + seg[d]*nrows, holding nrows rows of its n_d labels;
+// variant t writes row row[t] of it.
 constexpr int kLabelSlots = 8;
 struct LabelSlots {
     int n;
     int k[kLabelSlots];
     double T[kLabelSlots];
     const uint32_t *rootrank[kLabelSlots]; // per cell
-    const uint32_t *cid[kLabelSlots];      // per rank
+    const uint32_t *cid[kLabelSlots];      // per rank: root ranks below it
     const uint32_t *celllab[kLabelSlots];  // per cell: label of its core points
-    uint32_t *out[kLabelSlots];            // output row (caller order)
+    uint32_t row[kLabelSlots];
+    uint32_t *out;
+    uint32_t nrows;
+    const uint32_t *seg;                   // nsets+1 dataset starts
 };
 
-// label of a cell's core points at a snapshot: 1 + cluster id of its root
-__global__ void k_cell_labels(const uint32_t *rootrank, const uint32_t *cid, uint32_t ncell,
+// label of a cell's core points at a snapshot: 1 + cluster id of its root,
+// counted from the first root of the cell's dataset
+__global__ void k_cell_labels(const uint32_t *rootrank, const uint32_t *cid,
+                              const uint32_t *cell_ds, const uint32_t *seg, uint32_t ncell,
                               uint32_t *celllab)
 {
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= ncell) return;
     uint32_t rr = rootrank[c];
-    celllab[c] = rr == kNone ? 0u : cid[rr] + 1;
+    celllab[c] = rr == kNone ? 0u : cid[rr] - cid[seg[cell_ds[c]]] + 1;
 }
 
 // Thread per cell-order position: the point's data are read in sequence, the
 // labels are scattered to caller order.
 __global__ void __launch_bounds__(kBlock)
     k_labels_j(LabelSlots S, const uint32_t *caller_of_pos, const uint32_t *cell_of_pos,
-               const uint32_t *pos_rank, const uint8_t *kc, int J, int j,
+               const uint32_t *cell_ds, const uint32_t *pos_rank, const uint8_t *kc, int J, int j,
                const uint32_t *lst_idx, const double *lst_s, const uint32_t *lst_id,
                const uint32_t *lst_n, int L, uint32_t n)
 {
@@ -771,6 +744,9 @@ __global__ void __launch_bounds__(kBlock)
             }
         }
     }
+    uint32_t ds = cell_ds[C];
+    uint32_t s0 = S.seg[ds], nd = S.seg[ds + 1] - s0;
+    uint32_t *base = S.out + (size_t)s0 * S.nrows + (i - s0);
 #pragma unroll
     for (int t = 0; t < kLabelSlots; t++) {
         if (t >= S.n) break;
@@ -779,9 +755,9 @@ __global__ void __launch_bounds__(kBlock)
             lbl = S.celllab[t][C];
         } else {
             uint32_t b = best[t];
-            lbl = (b != kNone && b < r) ? S.cid[t][b] + 1 : 0u;
+            lbl = (b != kNone && b < r) ? S.cid[t][b] - S.cid[t][s0] + 1 : 0u;
         }
-        S.out[t][i] = lbl;
+        base[(size_t)S.row[t] * nd] = lbl;
     }
 }
 
@@ -800,13 +776,18 @@ __global__ void k_fill_u32(uint32_t *a, uint32_t v, uint32_t n)
 }
 
 // variants with no neighbour at all (eps < 0 or NaN): core iff minpts == 0,
-// and then every point is its own cluster, numbered in binned order
-__global__ void k_isolated_labels(const uint32_t *mapping, uint32_t n, bool all_core,
-                                  uint32_t *out)
+// and then every point is its own cluster, numbered in its dataset's binned
+// order; the row layout is that of LabelSlots
+__global__ void k_isolated_labels(const uint32_t *mapping, const uint32_t *seg, uint32_t nsets,
+                                  uint32_t n, bool all_core, uint32_t *out, uint32_t nrows,
+                                  uint32_t row)
 {
     uint32_t r = blockIdx.x * kBlock + threadIdx.x;
     if (r >= n) return;
-    out[mapping[r]] = all_core ? r + 1 : 0u;
+    uint32_t ds = seg_of(seg, nsets, r);
+    uint32_t s0 = seg[ds], nd = seg[ds + 1] - s0;
+    uint32_t i = mapping[r];
+    out[(size_t)s0 * nrows + (size_t)row * nd + (i - s0)] = all_core ? r - s0 + 1 : 0u;
 }
 
 struct Ev {
@@ -853,8 +834,8 @@ bool batchable(double eps, uint32_t minpts)
 }
 
 int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t *minpts,
-                  const uint32_t *rows, uint32_t nvar, uint32_t *out, vdb_stats *st,
-                  bool verbose, BatchResult *res)
+                  const uint32_t *rows, uint32_t nvar, uint32_t *out, uint32_t nrows,
+                  vdb_stats *st, bool verbose, BatchResult *res)
 {
     // every distance decision of this engine is binary64
     uint32_t n = P.n;
@@ -893,7 +874,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         if (threshold_for_eps(eps[v]) >= 0) continue;
         bool all_core = minpts[v] == 0;
         k_isolated_labels<<<ceil_div(n, kBlock), kBlock, 0, c.stream()>>>(
-            P.mapping, n, all_core, out + (size_t)rows[v] * n);
+            P.mapping, P.seg, P.nsets, n, all_core, out, nrows, rows[v]);
         VDB_HIP(hipGetLastError());
     }
     if (K == 0) return kOk;
@@ -1082,9 +1063,9 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         PairRec *lin = la, *lout = lb;
         std::vector<int> snap_k;
         for (int k = 0; k <= kmax; k++) {
-            LevelArgs a{G.xs,       G.ys,   G.cell_box, G.cell_start, G.cell_key, G.ncy, kc,
-                        cellkc,     pa,     pb,         pk,           J,          j,     k,
-                        Th[k]};
+            LevelArgs a{G.xs,  G.ys,  G.cell_box, G.cell_start, G.cell_key, G.cell_ds,
+                        G.skbase, G.sncy, kc,     cellkc,       pa,         pb,
+                        pk,    J,     j,          k,            Th[k]};
             int out_idx = 1 - in_idx;
             k_level_reset<<<1, 64, 0, c.stream()>>>(cnts, out_idx, k == 0, in_idx);
             VDB_HIP(hipGetLastError());
@@ -1126,7 +1107,8 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             k_snapshot_roots<<<gc, kBlock, 0, c.stream()>>>(parent, compmin, cellkc, G.ncell, J,
                                                             j, k, sroot);
             VDB_HIP(hipGetLastError());
-            k_cell_labels<<<gc, kBlock, 0, c.stream()>>>(sroot, scid, G.ncell,
+            k_cell_labels<<<gc, kBlock, 0, c.stream()>>>(sroot, scid, G.cell_ds, P.seg,
+                                                         G.ncell,
                                                          snap_lab + (size_t)slot * G.ncell);
             VDB_HIP(hipGetLastError());
         }
@@ -1147,9 +1129,12 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
                 S.rootrank[t] = snap_root + (size_t)sl * G.ncell;
                 S.cid[t] = snap_cid + (size_t)sl * n;
                 S.celllab[t] = snap_lab + (size_t)sl * G.ncell;
-                S.out[t] = out + (size_t)rows[rows_j[r0 + t].second] * n;
+                S.row[t] = rows[rows_j[r0 + t].second];
             }
-            k_labels_j<<<gn, kBlock, 0, c.stream()>>>(S, caller_of_pos, G.cell_of_pos,
+            S.out = out;
+            S.nrows = nrows;
+            S.seg = P.seg;
+            k_labels_j<<<gn, kBlock, 0, c.stream()>>>(S, caller_of_pos, G.cell_of_pos, G.cell_ds,
                                                       G.pos_rank, kc, J, j, lst_idx, lst_s,
                                                       lst_id, lst_n, (int)L, n);
             VDB_HIP(hipGetLastError());

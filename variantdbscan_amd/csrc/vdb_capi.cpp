@@ -131,17 +131,22 @@ int run_per_variant(Context &c, Points &P, const double *eps, const uint32_t *mi
     return kOk;
 }
 
-// Clusters the device-resident points P (x, y set) under nvar variants into
-// out (device, retArr layout); returns the path used in *used.
-int cluster_points(Context &c, Points &P, const double *eps, const uint32_t *minpts,
-                   uint32_t nvar, uint32_t *out, const vdb_options *opt, vdb_stats *st,
-                   int *used_out)
+// Clusters the device-resident points P (x, y set; several datasets when
+// P.nsets > 1, with P.hseg their starts) under the variants `which` of the
+// nvar given into out (device): dataset d's block out + hseg[d]*nvar is its
+// own retArr, nvar rows of n_d labels, and variant v fills row v. Variants
+// the batched engine takes run for all datasets at once; the rest, and groups
+// the data make it refuse, run dataset by dataset. Returns the path used in
+// *used_out.
+int cluster_points(Context &c, Points &P, const char *tag, const double *eps,
+                   const uint32_t *minpts, uint32_t nvar, const std::vector<uint32_t> &which,
+                   uint32_t *out, const vdb_options *opt, vdb_stats *st, int *used_out)
 {
     int path = opt ? opt->path : VDB_PATH_AUTO;
     int precision = opt ? opt->precision : VDB_PREC_FP64;
     bool verbose = opt && opt->verbose;
     Timer tprep(c.stream());
-    VDB_TRY(build_rank_order(c, P, "pts"));
+    VDB_TRY(build_rank_order(c, P, tag));
     if (st) st->ms_prepare += tprep.ms();
     int used = path;
     // variants left for the per-variant engine (all of them unless batched)
@@ -150,8 +155,7 @@ int cluster_points(Context &c, Points &P, const double *eps, const uint32_t *min
         // batchable variants, grouped so that max eps / min eps stays within
         // the fine grid's window; each group is one batched run
         std::vector<uint32_t> bat;
-        for (uint32_t v = 0; v < nvar; v++)
-            (batchable(eps[v], minpts[v]) ? bat : rest).push_back(v);
+        for (uint32_t v : which) (batchable(eps[v], minpts[v]) ? bat : rest).push_back(v);
         std::sort(bat.begin(), bat.end(), [&](uint32_t a, uint32_t b) {
             bool ia = threshold_for_eps(eps[a]) < 0, ib = threshold_for_eps(eps[b]) < 0;
             if (ia != ib) return ia;
@@ -179,7 +183,7 @@ int cluster_points(Context &c, Points &P, const double *eps, const uint32_t *min
             }
             BatchResult br;
             int s = cluster_batch(c, P, ge.data(), gm.data(), gr.data(), (uint32_t)ge.size(), out,
-                                  st, verbose, &br);
+                                  nvar, st, verbose, &br);
             if (s == kOk) {
                 any_batched = true;
             } else if (s == kErrUnsupported) {
@@ -191,15 +195,40 @@ int cluster_points(Context &c, Points &P, const double *eps, const uint32_t *min
         }
         used = any_batched ? VDB_PATH_BATCH : VDB_PATH_FUSED;
     } else {
-        for (uint32_t v = 0; v < nvar; v++) rest.push_back(v);
+        rest = which;
     }
-    if (!rest.empty()) {
-        std::sort(rest.begin(), rest.end());
+    if (rest.empty()) {
+        *used_out = used;
+        return kOk;
+    }
+    std::sort(rest.begin(), rest.end());
+    if (P.nsets > 1) {
+        // one dataset at a time: its own batched run first, then per variant
+        for (uint32_t d = 0; d < P.nsets; d++) {
+            uint32_t nd = P.hseg[d + 1] - P.hseg[d];
+            if (nd == 0) continue;
+            Points Q;
+            Q.n = nd;
+            Q.x = P.x + P.hseg[d];
+            Q.y = P.y + P.hseg[d];
+            int u = 0;
+            VDB_TRY(cluster_points(c, Q, "pts1", eps, minpts, nvar, rest,
+                                   out + (size_t)P.hseg[d] * nvar, opt, st, &u));
+            if (u == VDB_PATH_BATCH) used = u;
+        }
+    } else {
         VDB_TRY(run_per_variant(c, P, eps, minpts, rest, out, path == VDB_PATH_TABLE ? 1 : 0,
                                 precision, default_budget(c, opt), st, verbose));
     }
     *used_out = used;
     return kOk;
+}
+
+std::vector<uint32_t> all_variants(uint32_t nvar)
+{
+    std::vector<uint32_t> w(nvar);
+    for (uint32_t v = 0; v < nvar; v++) w[v] = v;
+    return w;
 }
 
 int cluster_impl(const double *x, const double *y, uint32_t n, const double *eps,
@@ -226,7 +255,8 @@ int cluster_impl(const double *x, const double *y, uint32_t n, const double *eps
     uint32_t *out = labels;
     if (!(flags & VDB_DEVICE_PTRS)) VDB_TRY(c.get("out.labels", (size_t)n * nvar, &out));
     int used = 0;
-    VDB_TRY(cluster_points(c, P, eps, minpts, nvar, out, opt, st, &used));
+    VDB_TRY(cluster_points(c, P, "pts", eps, minpts, nvar, all_variants(nvar), out, opt, st,
+                           &used));
     if (!(flags & VDB_DEVICE_PTRS)) {
         VDB_HIP(hipMemcpyAsync(labels, out, sizeof(uint32_t) * (size_t)n * nvar,
                                hipMemcpyDeviceToHost, c.stream()));
@@ -304,16 +334,24 @@ int vdb_cluster_multi(const double *x, const double *y, const uint64_t *offsets,
     uint32_t *out = labels;
     if (!(flags & VDB_DEVICE_PTRS)) VDB_TRY(c.get("out.labels", (size_t)ntot * nvar, &out));
     int used = 0;
-    for (uint32_t d = 0; d < nsets; d++) {
-        uint32_t nd = (uint32_t)(offsets[d + 1] - offsets[d]);
-        if (nd == 0) continue;
-        Points P;
-        P.n = nd;
-        P.x = all.x + offsets[d];
-        P.y = all.y + offsets[d];
-        // dataset d's block is its own retArr: nvar rows of nd labels
-        VDB_TRY(cluster_points(c, P, eps, minpts, nvar, out + offsets[d] * nvar, opt, st,
-                               &used));
+    // groups of datasets whose binned keys (90000 bins each) fit 32 bits
+    const uint32_t kGroup = 32768;
+    std::vector<uint32_t> which = all_variants(nvar);
+    for (uint32_t d0 = 0; d0 < nsets; d0 += kGroup) {
+        uint32_t ng = std::min(kGroup, nsets - d0);
+        Points G;
+        G.n = (uint32_t)(offsets[d0 + ng] - offsets[d0]);
+        if (G.n == 0) continue;
+        G.x = all.x + offsets[d0];
+        G.y = all.y + offsets[d0];
+        G.nsets = ng;
+        G.hseg.resize((size_t)ng + 1);
+        for (uint32_t d = 0; d <= ng; d++)
+            G.hseg[d] = (uint32_t)(offsets[d0 + d] - offsets[d0]);
+        int u = 0;
+        VDB_TRY(cluster_points(c, G, "pts", eps, minpts, nvar, which,
+                               out + (size_t)offsets[d0] * nvar, opt, st, &u));
+        if (u == VDB_PATH_BATCH || used == 0) used = u;
     }
     if (!(flags & VDB_DEVICE_PTRS))
         VDB_HIP(hipMemcpyAsync(labels, out, sizeof(uint32_t) * (size_t)ntot * nvar,

@@ -12,22 +12,34 @@
 
 #include "vdb_context.h"
 
+#include <vector>
+
 namespace vdb {
 
 // T(eps) = max{ s >= 0 : sqrt_rn(s) <= eps }, -1 when nothing qualifies.
 double threshold_for_eps(double eps);
 
+// Points of one or several independent datasets. Dataset d occupies
+// [seg[d], seg[d+1]) both in caller order and in rank order (the binned order
+// is computed per dataset); ranks are positions in the concatenated order.
 struct Points {
     uint32_t n = 0;
     const double *x = nullptr, *y = nullptr; // caller order (device)
     double *xb = nullptr, *yb = nullptr;     // rank order
     uint32_t *mapping = nullptr;             // rank -> caller index
-    double ext[4] = {0, 0, 0, 0};            // min x, max x, min y, max y
+    double ext[4] = {0, 0, 0, 0};            // extent of all points
+    uint32_t nsets = 1;
+    std::vector<uint32_t> hseg;              // nsets+1 (host)
+    uint32_t *seg = nullptr;                 // nsets+1 (device)
+    std::vector<double> hext;                // 4 per dataset (host)
 };
 
-// Reference binning (src/main.cpp:459-605) on the device. Fills the rank
-// order of P. Returns kErrUnsupported when a bin is out of range.
+// Reference binning (src/main.cpp:459-605) on the device, per dataset.
+// P.hseg must be set (nsets+1 host offsets); fills the rank order of P.
+// Returns kErrUnsupported when a bin is out of range.
 int build_rank_order(Context &c, Points &P, const char *tag);
+// One dataset covering all points.
+void single_segment(Points &P);
 
 struct Grid {
     double w = 0, gx0 = 0, gy0 = 0;
@@ -45,7 +57,25 @@ struct Grid {
     uint32_t *row_hi = nullptr;     // [ncell*(2R+1)] one past the last
     uint32_t max_cell = 0;          // largest cell population
     uint32_t npts = 0;              // points in the grid
+    // per dataset: grid origin, dimensions and first key (keys of dataset d
+    // are kbase[d] + cx*ncy[d] + cy)
+    uint32_t nsets = 1;
+    double *sgx0 = nullptr, *sgy0 = nullptr;
+    uint32_t *sncx = nullptr, *sncy = nullptr;
+    uint32_t *skbase = nullptr;     // nsets+1
+    uint32_t *cell_ds = nullptr;    // [ncell] dataset of each cell
 };
+
+// dataset of a position/rank (seg has nsets+1 entries, seg[0] = 0)
+__device__ __forceinline__ uint32_t seg_of(const uint32_t *seg, uint32_t nsets, uint32_t p)
+{
+    uint32_t lo = 0, hi = nsets; // largest d with seg[d] <= p
+    while (hi - lo > 1) {
+        uint32_t m = (lo + hi) >> 1;
+        if (seg[m] <= p) lo = m; else hi = m;
+    }
+    return lo;
+}
 
 // Build a grid of cell side >= w_min over the rank-ordered points. If the key
 // space would exceed 32 bits the side is enlarged; *enlarged reports it.

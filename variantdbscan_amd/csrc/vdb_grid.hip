@@ -5,6 +5,8 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <algorithm>
+#include <vector>
 
 namespace vdb {
 
@@ -36,11 +38,86 @@ struct BinParams {
     int xdeg, ydeg; // dimension has zero width: every point in bin 0
 };
 
-__global__ void k_bin_keys(const double *x, const double *y, uint32_t n, BinParams bp,
+// Extent of every dataset in two passes. Pass 1: block b of dataset d
+// (blocks bbase[d] .. bbase[d+1]-1) reduces kExtChunk points of it to
+// part[b] = {min x, max x, min y, max y} and counts non-finite coordinates.
+// Pass 2: one wave per dataset reduces that dataset's partials.
+constexpr uint32_t kExtChunk = 8 * kBlock;
+
+__device__ __forceinline__ void ext_merge(double *m, const double *o)
+{
+    m[0] = o[0] < m[0] ? o[0] : m[0];
+    m[1] = o[1] > m[1] ? o[1] : m[1];
+    m[2] = o[2] < m[2] ? o[2] : m[2];
+    m[3] = o[3] > m[3] ? o[3] : m[3];
+}
+
+__device__ __forceinline__ void ext_wave_reduce(double *m)
+{
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        double t[4];
+        for (int q = 0; q < 4; q++) t[q] = __shfl_xor(m[q], o);
+        ext_merge(m, t);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_seg_extent_part(const double *x, const double *y, const uint32_t *seg,
+                      const uint32_t *bbase, uint32_t nsets, double *part, uint32_t *nonfinite)
+{
+    uint32_t d = seg_of(bbase, nsets, blockIdx.x);
+    uint32_t a = seg[d] + (blockIdx.x - bbase[d]) * kExtChunk;
+    uint32_t e = seg[d + 1];
+    e = e - a > kExtChunk ? a + kExtChunk : e;
+    double m[4] = {DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX};
+    uint32_t bad = 0;
+    for (uint32_t i = a + threadIdx.x; i < e; i += kBlock) {
+        double u = x[i], v = y[i];
+        bad += !(isfinite(u) && isfinite(v));
+        double o[4] = {u, u, v, v};
+        ext_merge(m, o);
+    }
+    ext_wave_reduce(m);
+    for (int o = kWave / 2; o > 0; o >>= 1) bad += __shfl_xor(bad, o);
+    __shared__ double sm[kBlock / kWave][4];
+    __shared__ uint32_t sb[kBlock / kWave];
+    int w = threadIdx.x / kWave;
+    if (lane_id() == 0) {
+        for (int q = 0; q < 4; q++) sm[w][q] = m[q];
+        sb[w] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < kBlock / kWave; q++) {
+            ext_merge(m, sm[q]);
+            bad += sb[q];
+        }
+        for (int q = 0; q < 4; q++) part[4 * (size_t)blockIdx.x + q] = m[q];
+        if (bad) atomicAdd(nonfinite, bad);
+    }
+}
+
+__global__ void k_seg_extent_final(const double *part, const uint32_t *bbase, uint32_t nsets,
+                                   double *ext)
+{
+    uint32_t d = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    if (d >= nsets) return;
+    double m[4] = {DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX};
+    for (uint32_t b = bbase[d] + lane_id(); b < bbase[d + 1]; b += kWave)
+        ext_merge(m, part + 4 * (size_t)b);
+    ext_wave_reduce(m);
+    if (lane_id() == 0)
+        for (int q = 0; q < 4; q++) ext[4 * (size_t)d + q] = m[q];
+}
+
+__global__ void k_bin_keys(const double *x, const double *y, uint32_t n,
+                           const BinParams *bps, const uint32_t *seg, uint32_t nsets,
                            uint32_t *keys, uint32_t *vals, uint32_t *bad)
 {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
+    uint32_t d = seg_of(seg, nsets, i);
+    BinParams bp = bps[d];
     // floor((((x + xoffset) / x_width) * numBins)   src/main.cpp:539-540
     double fx = bp.xdeg ? 0.0 : floor(__dmul_rn(__ddiv_rn(__dadd_rn(x[i], bp.xoff), bp.xw), 300.0));
     double fy = bp.ydeg ? 0.0 : floor(__dmul_rn(__ddiv_rn(__dadd_rn(y[i], bp.yoff), bp.yw), 300.0));
@@ -49,7 +126,8 @@ __global__ void k_bin_keys(const double *x, const double *y, uint32_t n, BinPara
         fx = 0.0;
         fy = 0.0;
     }
-    keys[i] = (uint32_t)fx * 300u + (uint32_t)fy;
+    // dataset-major: each dataset's points stay contiguous, in its own binned order
+    keys[i] = d * 90000u + (uint32_t)fx * 300u + (uint32_t)fy;
     vals[i] = i;
 }
 
@@ -63,17 +141,20 @@ __global__ void k_gather_xy(const double *x, const double *y, const uint32_t *id
     oy[i] = y[j];
 }
 
-__global__ void k_cell_keys(const double *x, const double *y, uint32_t n, double gx0,
-                            double gy0, double w, uint32_t ncx, uint32_t ncy, uint32_t *keys,
-                            uint32_t *vals)
+__global__ void k_cell_keys(const double *x, const double *y, uint32_t n, const uint32_t *seg,
+                            uint32_t nsets, const double *sgx0, const double *sgy0, double w,
+                            const uint32_t *sncx, const uint32_t *sncy, const uint32_t *skbase,
+                            uint32_t *keys, uint32_t *vals)
 {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    double fx = floor(__ddiv_rn(__dsub_rn(x[i], gx0), w));
-    double fy = floor(__ddiv_rn(__dsub_rn(y[i], gy0), w));
+    uint32_t d = seg_of(seg, nsets, i);
+    uint32_t ncx = sncx[d], ncy = sncy[d];
+    double fx = floor(__ddiv_rn(__dsub_rn(x[i], sgx0[d]), w));
+    double fy = floor(__ddiv_rn(__dsub_rn(y[i], sgy0[d]), w));
     fx = fx < 0.0 ? 0.0 : (fx > (double)(ncx - 1) ? (double)(ncx - 1) : fx);
     fy = fy < 0.0 ? 0.0 : (fy > (double)(ncy - 1) ? (double)(ncy - 1) : fy);
-    keys[i] = (uint32_t)fx * ncy + (uint32_t)fy;
+    keys[i] = skbase[d] + (uint32_t)fx * ncy + (uint32_t)fy;
     vals[i] = i;
 }
 
@@ -169,26 +250,31 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *a, uint32_t 
 
 // window rows: for cell c and row offset d in [-R, R], the cells of grid row
 // cx+d whose cy lies in [cy-R, cy+R] form one contiguous run of the sorted keys
-__global__ void k_row_ranges(const uint32_t *cell_key, uint32_t ncell, uint32_t ncx,
-                             uint32_t ncy, int R, uint32_t *row_lo, uint32_t *row_hi)
+// rows stay inside the cell's own dataset grid
+__global__ void k_row_ranges(const uint32_t *cell_key, uint32_t ncell, const uint32_t *sncx,
+                             const uint32_t *sncy, const uint32_t *skbase, uint32_t nsets, int R,
+                             uint32_t *row_lo, uint32_t *row_hi, uint32_t *cell_ds)
 {
     uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     uint32_t W = 2 * R + 1;
     if (t >= ncell * W) return;
     uint32_t c = t / W;
-    int d = (int)(t % W) - R;
+    int dr = (int)(t % W) - R;
     uint32_t key = cell_key[c];
-    int64_t cx = key / ncy, cy = key % ncy;
-    int64_t rx = cx + d;
+    uint32_t ds = seg_of(skbase, nsets, key);
+    if (dr == 0) cell_ds[c] = ds;
+    uint32_t ncx = sncx[ds], ncy = sncy[ds], kb = skbase[ds];
+    uint32_t local = key - kb;
+    int64_t cx = local / ncy, cy = local % ncy;
+    int64_t rx = cx + dr;
     uint32_t lo = 0, hi = 0;
     if (rx >= 0 && rx < (int64_t)ncx) {
         int64_t y0 = cy - R < 0 ? 0 : cy - R;
         int64_t y1 = cy + R > (int64_t)ncy - 1 ? (int64_t)ncy - 1 : cy + R;
-        uint32_t k0 = (uint32_t)(rx * ncy + y0);
-        uint32_t k1 = (uint32_t)(rx * ncy + y1);
-        lo = lower_bound_u32(cell_key, ncell, k0);
-        hi = lower_bound_u32(cell_key, ncell, k1 + 1);
-        if (k1 == 0xffffffffu) hi = ncell;
+        uint64_t k0 = kb + (uint64_t)(rx * ncy + y0);
+        uint64_t k1 = kb + (uint64_t)(rx * ncy + y1);
+        lo = lower_bound_u32(cell_key, ncell, (uint32_t)k0);
+        hi = k1 >= 0xffffffffull ? ncell : lower_bound_u32(cell_key, ncell, (uint32_t)(k1 + 1));
     }
     row_lo[t] = lo;
     row_hi[t] = hi;
@@ -198,38 +284,87 @@ std::string slot(const char *tag, const char *name) { return std::string(tag) + 
 
 } // namespace
 
+void single_segment(Points &P)
+{
+    P.nsets = 1;
+    P.hseg.assign({0u, P.n});
+}
+
 int build_rank_order(Context &c, Points &P, const char *tag)
 {
     uint32_t n = P.n;
-    double *ext = nullptr;
-    VDB_TRY(c.get(slot(tag, "ext").c_str(), 4, &ext));
-    VDB_TRY(extent_f64(c, P.x, P.y, n, ext));
-    void *h = nullptr;
-    VDB_TRY(c.host_scratch(4 * sizeof(double), &h));
-    VDB_HIP(hipMemcpyAsync(h, ext, 4 * sizeof(double), hipMemcpyDeviceToHost, c.stream()));
+    if (P.hseg.size() != (size_t)P.nsets + 1) single_segment(P);
+    uint32_t ns = P.nsets;
+    VDB_TRY(c.get(slot(tag, "seg").c_str(), (size_t)ns + 1, &P.seg));
+    VDB_HIP(hipMemcpyAsync(P.seg, P.hseg.data(), sizeof(uint32_t) * (ns + 1),
+                           hipMemcpyHostToDevice, c.stream()));
+    // pass-1 blocks of each dataset
+    std::vector<uint32_t> blk(ns + 1, 0);
+    for (uint32_t d = 0; d < ns; d++)
+        blk[d + 1] = blk[d] + (uint32_t)ceil_div(P.hseg[d + 1] - P.hseg[d], kExtChunk);
+    double *ext = nullptr, *part = nullptr;
+    uint32_t *nonfin = nullptr, *bbase = nullptr;
+    VDB_TRY(c.get(slot(tag, "ext").c_str(), (size_t)ns * 4, &ext));
+    VDB_TRY(c.get(slot(tag, "extpart").c_str(), (size_t)std::max(blk[ns], 1u) * 4, &part));
+    VDB_TRY(c.get(slot(tag, "extbase").c_str(), (size_t)ns + 1, &bbase));
+    VDB_TRY(c.get(slot(tag, "nonfin").c_str(), 1, &nonfin));
+    VDB_HIP(hipMemcpyAsync(bbase, blk.data(), sizeof(uint32_t) * (ns + 1), hipMemcpyHostToDevice,
+                           c.stream()));
+    VDB_HIP(hipMemsetAsync(nonfin, 0, sizeof(uint32_t), c.stream()));
+    if (blk[ns])
+        k_seg_extent_part<<<blk[ns], kBlock, 0, c.stream()>>>(P.x, P.y, P.seg, bbase, ns, part,
+                                                             nonfin);
+    VDB_HIP(hipGetLastError());
+    k_seg_extent_final<<<ceil_div(ns, kBlock / kWave), kBlock, 0, c.stream()>>>(part, bbase, ns,
+                                                                               ext);
+    VDB_HIP(hipGetLastError());
+    P.hext.resize((size_t)ns * 4);
+    uint32_t hnf = 0;
+    VDB_HIP(hipMemcpyAsync(P.hext.data(), ext, sizeof(double) * 4 * ns, hipMemcpyDeviceToHost,
+                           c.stream()));
+    VDB_HIP(hipMemcpyAsync(&hnf, nonfin, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream()));
     VDB_HIP(hipStreamSynchronize(c.stream()));
-    memcpy(P.ext, h, 4 * sizeof(double));
-    for (int i = 0; i < 4; i++)
-        if (!std::isfinite(P.ext[i])) {
-            set_error("non-finite coordinate in input");
-            return kErrNonFinite;
+    if (hnf) {
+        set_error("non-finite coordinate in input");
+        return kErrNonFinite;
+    }
+    P.ext[0] = DBL_MAX;
+    P.ext[1] = -DBL_MAX;
+    P.ext[2] = DBL_MAX;
+    P.ext[3] = -DBL_MAX;
+    std::vector<BinParams> bps(ns);
+    for (uint32_t d = 0; d < ns; d++) {
+        const double *e = &P.hext[4 * (size_t)d];
+        if (P.hseg[d + 1] > P.hseg[d]) {
+            P.ext[0] = std::min(P.ext[0], e[0]);
+            P.ext[1] = std::max(P.ext[1], e[1]);
+            P.ext[2] = std::min(P.ext[2], e[2]);
+            P.ext[3] = std::max(P.ext[3], e[3]);
         }
-    // src/main.cpp:465-492: min from DBL_MAX, max from DBL_MIN (smallest
-    // positive double), width = (max - min) * 1.01
-    double min_x = P.ext[0] < DBL_MAX ? P.ext[0] : DBL_MAX;
-    double max_x = P.ext[1] > DBL_MIN ? P.ext[1] : DBL_MIN;
-    double min_y = P.ext[2] < DBL_MAX ? P.ext[2] : DBL_MAX;
-    double max_y = P.ext[3] > DBL_MIN ? P.ext[3] : DBL_MIN;
-    BinParams bp;
-    bp.xw = (max_x - min_x) * 1.01;
-    bp.yw = (max_y - min_y) * 1.01;
-    bp.xoff = (min_x > 0) ? -min_x : std::fabs(min_x); // :513-527
-    bp.yoff = (min_y > 0) ? -min_y : std::fabs(min_y);
-    // zero width (all coordinates equal and positive) divides 0 by 0 in the
-    // reference, which then aborts binning and reads past its mapping; here the
-    // whole dimension maps to bin 0
-    bp.xdeg = !(bp.xw > 0.0);
-    bp.ydeg = !(bp.yw > 0.0);
+        // src/main.cpp:465-492: min from DBL_MAX, max from DBL_MIN (smallest
+        // positive double), width = (max - min) * 1.01
+        double min_x = e[0] < DBL_MAX ? e[0] : DBL_MAX;
+        double max_x = e[1] > DBL_MIN ? e[1] : DBL_MIN;
+        double min_y = e[2] < DBL_MAX ? e[2] : DBL_MAX;
+        double max_y = e[3] > DBL_MIN ? e[3] : DBL_MIN;
+        BinParams &bp = bps[d];
+        bp.xw = (max_x - min_x) * 1.01;
+        bp.yw = (max_y - min_y) * 1.01;
+        bp.xoff = (min_x > 0) ? -min_x : std::fabs(min_x); // :513-527
+        bp.yoff = (min_y > 0) ? -min_y : std::fabs(min_y);
+        // zero width (all coordinates equal and positive) divides 0 by 0 in the
+        // reference, which then aborts binning and reads past its mapping; here
+        // the whole dimension maps to bin 0
+        bp.xdeg = !(bp.xw > 0.0);
+        bp.ydeg = !(bp.yw > 0.0);
+    }
+    if (P.ext[0] > P.ext[1]) {
+        P.ext[0] = P.ext[1] = P.ext[2] = P.ext[3] = 0.0;
+    }
+    BinParams *dbp;
+    VDB_TRY(c.get(slot(tag, "binparams").c_str(), ns, &dbp));
+    VDB_HIP(hipMemcpyAsync(dbp, bps.data(), sizeof(BinParams) * ns, hipMemcpyHostToDevice,
+                           c.stream()));
 
     uint32_t *keys, *vals, *tk, *tv, *bad;
     VDB_TRY(c.get(slot(tag, "bk").c_str(), n, &keys));
@@ -239,10 +374,11 @@ int build_rank_order(Context &c, Points &P, const char *tag)
     VDB_TRY(c.get(slot(tag, "bad").c_str(), 1, &bad));
     VDB_HIP(hipMemsetAsync(bad, 0, sizeof(uint32_t), c.stream()));
     unsigned g = ceil_div(n, kBlock);
-    k_bin_keys<<<g, kBlock, 0, c.stream()>>>(P.x, P.y, n, bp, keys, vals, bad);
+    k_bin_keys<<<g, kBlock, 0, c.stream()>>>(P.x, P.y, n, dbp, P.seg, ns, keys, vals, bad);
     VDB_HIP(hipGetLastError());
     uint32_t *sk, *sv;
-    VDB_TRY(radix_sort_pairs(c, keys, vals, tk, tv, n, 17, &sk, &sv));
+    VDB_TRY(radix_sort_pairs(c, keys, vals, tk, tv, n, bits_for((uint64_t)ns * 90000), &sk,
+                             &sv));
     VDB_TRY(c.get(slot(tag, "mapping").c_str(), n, &P.mapping));
     VDB_HIP(hipMemcpyAsync(P.mapping, sv, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice,
                            c.stream()));
@@ -251,6 +387,8 @@ int build_rank_order(Context &c, Points &P, const char *tag)
     k_gather_xy<<<g, kBlock, 0, c.stream()>>>(P.x, P.y, P.mapping, n, P.xb, P.yb);
     VDB_HIP(hipGetLastError());
     uint32_t hb = 0;
+    void *h = nullptr;
+    VDB_TRY(c.host_scratch(sizeof(uint32_t), &h));
     VDB_HIP(hipMemcpyAsync(h, bad, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream()));
     VDB_HIP(hipStreamSynchronize(c.stream()));
     memcpy(&hb, h, sizeof(uint32_t));
@@ -265,27 +403,73 @@ int build_grid(Context &c, const Points &P, double w_min, int R, const char *tag
                bool *enlarged)
 {
     uint32_t n = P.n;
-    double spanx = P.ext[1] - P.ext[0], spany = P.ext[3] - P.ext[2];
+    uint32_t ns = P.nsets;
+    double spanmax = 0.0;
+    for (uint32_t d = 0; d < ns; d++) {
+        if (P.hseg[d + 1] == P.hseg[d]) continue;
+        const double *e = &P.hext[4 * (size_t)d];
+        spanmax = std::fmax(spanmax, std::fmax(e[1] - e[0], e[3] - e[2]));
+    }
     double w = w_min;
     *enlarged = false;
     if (!(w > 0.0) || !std::isfinite(w)) {
-        w = std::fmax(std::fmax(spanx, spany), 1.0) * 2.0;
+        w = std::fmax(spanmax, 1.0) * 2.0;
         *enlarged = true;
     }
-    // keep both axes below 2^16 cells so keys fit 32 bits
-    double wmin_keys = std::fmax(spanx, spany) / 65000.0;
-    if (w < wmin_keys) {
-        w = wmin_keys;
+    // every axis stays below 2^16 cells (so one dataset's keys fit 32 bits);
+    // with several datasets the side then grows until all keys fit together
+    if (w < spanmax / 65000.0) {
+        w = spanmax / 65000.0;
         *enlarged = true;
+    }
+    std::vector<double> gx0(ns), gy0(ns);
+    std::vector<uint32_t> ncx(ns), ncy(ns), kbase(ns + 1);
+    for (int attempt = 0;; attempt++) {
+        uint64_t total = 0;
+        bool ok = true;
+        for (uint32_t d = 0; d < ns; d++) {
+            const double *e = &P.hext[4 * (size_t)d];
+            bool empty = P.hseg[d + 1] == P.hseg[d];
+            double sx = empty ? 0.0 : e[1] - e[0], sy = empty ? 0.0 : e[3] - e[2];
+            gx0[d] = empty ? 0.0 : e[0];
+            gy0[d] = empty ? 0.0 : e[2];
+            double fx = empty ? 1.0 : std::floor(sx / w) + 2;
+            double fy = empty ? 1.0 : std::floor(sy / w) + 2;
+            if (fx > 65536.0 || fy > 65536.0) ok = false;
+            ncx[d] = ok ? (uint32_t)fx : 1;
+            ncy[d] = ok ? (uint32_t)fy : 1;
+            kbase[d] = (uint32_t)std::min<uint64_t>(total, 0xffffffffull);
+            total += (uint64_t)ncx[d] * ncy[d];
+        }
+        kbase[ns] = (uint32_t)std::min<uint64_t>(total, 0xffffffffull);
+        if (ok && total < 0xffffffffull) break;
+        w *= ok ? std::sqrt((double)total / 4.0e9) * 1.05 : 1.5;
+        *enlarged = true;
+        if (attempt > 64) {
+            set_error("grid key space");
+            return kErrUnsupported;
+        }
     }
     G.w = w;
-    G.gx0 = P.ext[0];
-    G.gy0 = P.ext[2];
-    G.ncx = (uint32_t)std::floor(spanx / w) + 2;
-    G.ncy = (uint32_t)std::floor(spany / w) + 2;
+    G.gx0 = gx0[0];
+    G.gy0 = gy0[0];
+    G.ncx = ncx[0];
+    G.ncy = ncy[0];
     G.R = R;
     G.npts = n;
-    uint64_t nkeys = (uint64_t)G.ncx * G.ncy;
+    G.nsets = ns;
+    uint64_t nkeys = (uint64_t)kbase[ns];
+    VDB_TRY(c.get(slot(tag, "sgx0").c_str(), ns, &G.sgx0));
+    VDB_TRY(c.get(slot(tag, "sgy0").c_str(), ns, &G.sgy0));
+    VDB_TRY(c.get(slot(tag, "sncx").c_str(), ns, &G.sncx));
+    VDB_TRY(c.get(slot(tag, "sncy").c_str(), ns, &G.sncy));
+    VDB_TRY(c.get(slot(tag, "skbase").c_str(), (size_t)ns + 1, &G.skbase));
+    VDB_HIP(hipMemcpyAsync(G.sgx0, gx0.data(), 8 * ns, hipMemcpyHostToDevice, c.stream()));
+    VDB_HIP(hipMemcpyAsync(G.sgy0, gy0.data(), 8 * ns, hipMemcpyHostToDevice, c.stream()));
+    VDB_HIP(hipMemcpyAsync(G.sncx, ncx.data(), 4 * ns, hipMemcpyHostToDevice, c.stream()));
+    VDB_HIP(hipMemcpyAsync(G.sncy, ncy.data(), 4 * ns, hipMemcpyHostToDevice, c.stream()));
+    VDB_HIP(hipMemcpyAsync(G.skbase, kbase.data(), 4 * ((size_t)ns + 1), hipMemcpyHostToDevice,
+                           c.stream()));
 
     uint32_t *keys, *vals, *tk, *tv, *flags, *excl;
     VDB_TRY(c.get(slot(tag, "gk").c_str(), n, &keys));
@@ -293,8 +477,8 @@ int build_grid(Context &c, const Points &P, double w_min, int R, const char *tag
     VDB_TRY(c.get("sort.tk", n, &tk));
     VDB_TRY(c.get("sort.tv", n, &tv));
     unsigned g = ceil_div(n, kBlock);
-    k_cell_keys<<<g, kBlock, 0, c.stream()>>>(P.xb, P.yb, n, G.gx0, G.gy0, w, G.ncx, G.ncy,
-                                              keys, vals);
+    k_cell_keys<<<g, kBlock, 0, c.stream()>>>(P.xb, P.yb, n, P.seg, ns, G.sgx0, G.sgy0, w,
+                                              G.sncx, G.sncy, G.skbase, keys, vals);
     VDB_HIP(hipGetLastError());
     uint32_t *sk, *sv;
     VDB_TRY(radix_sort_pairs(c, keys, vals, tk, tv, n, bits_for(nkeys), &sk, &sv));
@@ -341,8 +525,9 @@ int build_grid(Context &c, const Points &P, double w_min, int R, const char *tag
     size_t W = 2 * (size_t)R + 1;
     VDB_TRY(c.get(slot(tag, "row_lo").c_str(), ncell * W, &G.row_lo));
     VDB_TRY(c.get(slot(tag, "row_hi").c_str(), ncell * W, &G.row_hi));
+    VDB_TRY(c.get(slot(tag, "cell_ds").c_str(), ncell, &G.cell_ds));
     k_row_ranges<<<ceil_div(ncell * W, kBlock), kBlock, 0, c.stream()>>>(
-        G.cell_key, ncell, G.ncx, G.ncy, R, G.row_lo, G.row_hi);
+        G.cell_key, ncell, G.sncx, G.sncy, G.skbase, ns, R, G.row_lo, G.row_hi, G.cell_ds);
     VDB_HIP(hipGetLastError());
 
     void *h = nullptr;
