@@ -25,7 +25,8 @@ EXPORTED = ("libVDBSCAN", "vdb_last_error", "vdb_version", "vdb_set_stream", "vd
 
 class VdbOptions(ctypes.Structure):
     _fields_ = [("path", ctypes.c_int), ("precision", ctypes.c_int),
-                ("table_budget_bytes", ctypes.c_uint64), ("verbose", ctypes.c_int)]
+                ("table_budget_bytes", ctypes.c_uint64), ("verbose", ctypes.c_int),
+                ("cells_per_eps", ctypes.c_int)]
 
 
 class VdbStats(ctypes.Structure):
@@ -112,8 +113,10 @@ def _u32(a):
 
 
 def cluster(x, y, eps, minpts, path=PATH_AUTO, precision=PREC_FP64, verbose=False,
-            table_budget_bytes=0, return_stats=False):
-    """Labels (V, n) uint32 for host arrays; retArr layout of libVDBSCAN."""
+            table_budget_bytes=0, return_stats=False, cells_per_eps=0):
+    """Labels (V, n) uint32 for host arrays; retArr layout of libVDBSCAN.
+    cells_per_eps (1..4) sets the per-variant grid's cells per eps; 0 lets
+    the library choose."""
     L = load()
     x, y = _f64(x), _f64(y)
     eps = _f64(np.atleast_1d(eps))
@@ -122,7 +125,7 @@ def cluster(x, y, eps, minpts, path=PATH_AUTO, precision=PREC_FP64, verbose=Fals
         raise ValueError("length mismatch")
     n, v = len(x), len(eps)
     out = np.zeros(n * v, dtype=np.uint32)
-    opt = VdbOptions(path, precision, table_budget_bytes, 1 if verbose else 0)
+    opt = VdbOptions(path, precision, table_budget_bytes, 1 if verbose else 0, cells_per_eps)
     st = VdbStats()
     _check(L.vdb_cluster(_ptr(x), _ptr(y), n, _ptr(eps), _ptr(mp), v, _ptr(out), 0,
                          ctypes.byref(opt), ctypes.byref(st)))
@@ -131,12 +134,12 @@ def cluster(x, y, eps, minpts, path=PATH_AUTO, precision=PREC_FP64, verbose=Fals
 
 
 def cluster_device(x_ptr, y_ptr, n, eps, minpts, labels_ptr, path=PATH_AUTO,
-                   precision=PREC_FP64, stats=True):
+                   precision=PREC_FP64, stats=True, cells_per_eps=0):
     """Device-resident inputs/outputs (raw device pointers, e.g. tensor.data_ptr())."""
     L = load()
     eps = _f64(np.atleast_1d(eps))
     mp = _u32(np.atleast_1d(minpts))
-    opt = VdbOptions(path, precision, 0, 0)
+    opt = VdbOptions(path, precision, 0, 0, cells_per_eps)
     st = VdbStats()
     _check(L.vdb_cluster(ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), n, _ptr(eps),
                          _ptr(mp), len(eps), ctypes.c_void_p(labels_ptr), DEVICE_PTRS,

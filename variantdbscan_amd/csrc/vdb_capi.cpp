@@ -16,13 +16,6 @@ using namespace vdb;
 
 namespace {
 
-// Grid side for a window of radius 1: any pair accepted at eps has
-// |dx| <= eps*(1+2^-50) (s <= T(eps) bounds the rounded difference), and the
-// computed cell coordinate differs from the exact one by < 2^-31 for grids
-// below 2^20 cells per axis, so cells of side eps*(1+2^-24) keep every
-// accepted pair within one cell per axis.
-double plain_cell_side(double eps) { return eps * (1.0 + std::ldexp(1.0, -24)); }
-
 struct Timer {
     hipEvent_t a = nullptr, b = nullptr;
     hipStream_t s;
@@ -81,7 +74,7 @@ size_t default_budget(Context &c, const vdb_options *opt)
 // per-variant paths: one grid per distinct eps, counts shared by its variants
 int run_per_variant(Context &c, Points &P, const double *eps, const uint32_t *minpts,
                     const std::vector<uint32_t> &which, uint32_t *labels_dev, int mode,
-                    int precision, size_t budget, vdb_stats *st, bool verbose)
+                    int precision, int cells, size_t budget, vdb_stats *st, bool verbose)
 {
     // group by eps (NaN eps compare unequal to themselves: key them by bits)
     std::map<uint64_t, std::vector<uint32_t>> by_eps;
@@ -100,17 +93,19 @@ int run_per_variant(Context &c, Points &P, const double *eps, const uint32_t *mi
         double e = eps[kv.second[0]];
         Grid G;
         Items I;
-        bool enl = false;
         Timer t0(c.stream());
-        VDB_TRY(build_grid(c, P, plain_cell_side(e), 1, "g1", G, &enl));
+        int k = 0;
+        VDB_TRY(tune_cells_per_eps(c, P, e, precision, cells, &k));
+        VDB_TRY(build_eps_grid(c, P, e, k, "g1", G));
         VDB_TRY(build_items(c, G, "g1", I));
         if (st) st->ms_prepare += t0.ms();
         Timer t1(c.stream());
         VDB_TRY(count_neighbors(c, G, I, e, precision, counts, (uint64_t *)rc));
         if (st) st->ms_count += t1.ms();
         if (verbose)
-            printf("\n[vdbscan] eps=%g grid %ux%u, %u non-empty cells, largest %u points",
-                   e, G.ncx, G.ncy, G.ncell, G.max_cell);
+            printf("\n[vdbscan] eps=%g grid %ux%u (%d cells per eps), %u non-empty cells, "
+                   "largest %u points",
+                   e, G.ncx, G.ncy, G.R, G.ncell, G.max_cell);
         for (uint32_t v : kv.second) {
             VDB_TRY(cluster_variant(c, P, G, I, e, minpts[v], counts, mode, budget,
                                     labels_dev + (size_t)v * P.n, st ? &pt : nullptr));
@@ -218,7 +213,8 @@ int cluster_points(Context &c, Points &P, const char *tag, const double *eps,
         }
     } else {
         VDB_TRY(run_per_variant(c, P, eps, minpts, rest, out, path == VDB_PATH_TABLE ? 1 : 0,
-                                precision, default_budget(c, opt), st, verbose));
+                                precision, cells_per_eps(opt), default_budget(c, opt), st,
+                                verbose));
     }
     *used_out = used;
     return kOk;
@@ -404,8 +400,9 @@ int vdb_neighbor_counts(const double *x, const double *y, uint32_t n, double eps
     VDB_TRY(build_rank_order(c, P, "pts"));
     Grid G;
     Items I;
-    bool enl;
-    VDB_TRY(build_grid(c, P, plain_cell_side(eps), 1, "g1", G, &enl));
+    int k = 0;
+    VDB_TRY(tune_cells_per_eps(c, P, eps, precision, 0, &k));
+    VDB_TRY(build_eps_grid(c, P, eps, k, "g1", G));
     VDB_TRY(build_items(c, G, "g1", I));
     uint32_t *cpos, *ccal;
     unsigned long long *rc;
@@ -457,8 +454,9 @@ int vdb_neighbor_table(const double *x, const double *y, uint32_t n, double eps,
     VDB_TRY(build_rank_order(c, P, "pts"));
     Grid G;
     Items I;
-    bool enl;
-    VDB_TRY(build_grid(c, P, plain_cell_side(eps), 1, "g1", G, &enl));
+    int k = 0;
+    VDB_TRY(tune_cells_per_eps(c, P, eps, precision, 0, &k));
+    VDB_TRY(build_eps_grid(c, P, eps, k, "g1", G));
     VDB_TRY(build_items(c, G, "g1", I));
     if (st) st->ms_prepare = tprep.ms();
     bool dev = flags & VDB_DEVICE_PTRS;
@@ -523,8 +521,9 @@ int vdb_neighbor_table_digest(const double *x, const double *y, uint32_t n, doub
     VDB_TRY(build_rank_order(c, P, "pts"));
     Grid G;
     Items I;
-    bool enl;
-    VDB_TRY(build_grid(c, P, plain_cell_side(eps), 1, "g1", G, &enl));
+    int k = 0;
+    VDB_TRY(tune_cells_per_eps(c, P, eps, precision, 0, &k));
+    VDB_TRY(build_eps_grid(c, P, eps, k, "g1", G));
     VDB_TRY(build_items(c, G, "g1", I));
     return table_digest(c, P, G, I, eps, precision, digest, pairs, rechecked);
 }

@@ -11,6 +11,7 @@
 #pragma once
 
 #include "vdb_grid.h"
+#include "vdbscan.h"
 
 namespace vdb {
 
@@ -31,6 +32,25 @@ struct Items {
 constexpr int kOrigins = 8;
 
 int build_items(Context &c, const Grid &G, const char *tag, Items &I);
+
+// Grid of the per-variant kernels at eps: cells of side eps*(1+2^-24)/k and a
+// window of k cells each way (any pair accepted at eps is at most
+// eps*(1+2^-50) apart per axis, and computed cell coordinates are within
+// 2^-31 of exact below 2^20 cells per axis, so the pair spans at most k
+// cells), with its window rows classified for T(eps).
+int build_eps_grid(Context &c, const Points &P, double eps, int k, const char *tag, Grid &G);
+
+// Cells per eps for the per-variant kernels: the option's value when set
+// (1..kMaxCellsPerEps), else 0 (let tune_cells_per_eps choose).
+constexpr int kMaxCellsPerEps = 4;
+int cells_per_eps(const vdb_options *opt);
+
+// Autotuner: the first time a workload (point count, eps, extent, precision)
+// is seen, times grid build + count pass at every resolution 1..4 and keeps
+// the fastest; later calls reuse the choice. Small inputs skip the timing.
+// Uses the "g1" grid slots. requested > 0 is returned unchanged.
+int tune_cells_per_eps(Context &c, const Points &P, double eps, int precision, int requested,
+                       int *k_out);
 
 // counts[p] = |N_eps(p)| for every position p of G (self included).
 // precision 0 = FP64 exact, 1 = FP32 with an FP64 recheck inside the error band.

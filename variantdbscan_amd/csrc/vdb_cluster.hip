@@ -12,6 +12,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -22,17 +23,56 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr uint32_t kNone = 0xffffffffu;
+constexpr int kDefaultCellsPerEps = 2;
 
 struct GridView {
     const double *xs, *ys;
-    const uint32_t *cell_start, *row_lo, *row_hi;
+    const uint32_t *cell_start, *row_lo, *row_hi, *row_f0, *row_f1;
     int W;
     uint32_t n;
 };
 
 GridView view(const Grid &G, uint32_t n)
 {
-    return GridView{G.xs, G.ys, G.cell_start, G.row_lo, G.row_hi, 2 * G.R + 1, n};
+    return GridView{G.xs,     G.ys,     G.cell_start,   G.row_lo, G.row_hi,
+                    G.row_f0, G.row_f1, 2 * G.R + 1,    n};
+}
+
+// Row r of cell c's window as cell-sorted positions: candidates to test in
+// [a, f0) and [f1, e), candidates accepted without a test in [f0, f1).
+struct RowSeg {
+    uint32_t a, f0, f1, e;
+};
+
+__device__ __forceinline__ RowSeg row_seg(const GridView &g, uint32_t c, int r)
+{
+    size_t t = (size_t)c * g.W + r;
+    uint32_t lo = g.row_lo[t], hi = g.row_hi[t];
+    RowSeg s;
+    s.a = g.cell_start[lo];
+    s.e = lo < hi ? g.cell_start[hi] : s.a;
+    s.f0 = s.f1 = s.e;
+    if (g.row_f0 && lo < hi) {
+        s.f0 = g.cell_start[g.row_f0[t]];
+        s.f1 = g.cell_start[g.row_f1[t]];
+    }
+    return s;
+}
+
+// 64-candidate blocks of positions [a, e): lane l holds a + 64*i + l; lanes
+// past e see NaN coordinates (never accepted).
+template <class F>
+__device__ __forceinline__ void run_blocks(const GridView &g, uint32_t a, uint32_t e, F &&f)
+{
+    for (uint32_t q0 = a; q0 < e; q0 += kWave) {
+        uint32_t q = q0 + lane_id();
+        bool v = q < e;
+        uint32_t qc = v ? q : e - 1;
+        double qx = g.xs[qc], qy = g.ys[qc];
+        qx = v ? qx : __longlong_as_double(0x7ff8000000000000ll);
+        qy = v ? qy : __longlong_as_double(0x7ff8000000000000ll);
+        f(q, v, qx, qy);
+    }
 }
 
 // Visit the window of cell c in 64-candidate blocks. With `half`, only rows
@@ -63,43 +103,36 @@ __device__ __forceinline__ void window_blocks(const GridView &g, uint32_t c, boo
     }
 }
 
-// Like window_blocks, but hands out chunks of up to kChunk 64-candidate blocks
-// held in registers (lane l holds candidates q0 + i*64 + l). Used where the
-// consumer wants to finish one origin over a whole chunk before the next.
+// Chunks of up to kChunk 64-candidate blocks of positions [a, e) held in
+// registers (lane l holds candidates q0 + i*64 + l), for consumers that finish
+// one origin over a whole chunk before the next.
 constexpr int kChunk = 8;
 template <class F>
-__device__ __forceinline__ void window_chunks(const GridView &g, uint32_t c, F &&f)
+__device__ __forceinline__ void run_chunks(const GridView &g, uint32_t a, uint32_t e, F &&f)
 {
-    int R = g.W / 2;
     const double qnan = __longlong_as_double(0x7ff8000000000000ll);
-    for (int r = 0; r < g.W; r++) {
-        (void)R;
-        uint32_t lo = g.row_lo[(size_t)c * g.W + r], hi = g.row_hi[(size_t)c * g.W + r];
-        if (lo >= hi) continue;
-        uint32_t a = g.cell_start[lo], e = g.cell_start[hi];
-        for (uint32_t q0 = a; q0 < e; q0 += kWave * kChunk) {
-            int nblk = (int)((e - q0 + kWave - 1) / kWave);
-            nblk = nblk < kChunk ? nblk : kChunk;
-            double qx[kChunk], qy[kChunk];
+    for (uint32_t q0 = a; q0 < e; q0 += kWave * kChunk) {
+        int nblk = (int)((e - q0 + kWave - 1) / kWave);
+        nblk = nblk < kChunk ? nblk : kChunk;
+        double qx[kChunk], qy[kChunk];
 #pragma unroll
-            for (int i = 0; i < kChunk; i++) {
-                uint32_t q = q0 + i * kWave + lane_id();
-                bool v = i < nblk && q < e;
-                uint32_t qc = v ? q : e - 1;
-                double x = g.xs[qc], y = g.ys[qc];
-                qx[i] = v ? x : qnan;
-                qy[i] = v ? y : qnan;
-            }
-            f(q0, nblk, qx, qy);
+        for (int i = 0; i < kChunk; i++) {
+            uint32_t q = q0 + i * kWave + lane_id();
+            bool v = i < nblk && q < e;
+            uint32_t qc = v ? q : e - 1;
+            double x = g.xs[qc], y = g.ys[qc];
+            qx[i] = v ? x : qnan;
+            qy[i] = v ? y : qnan;
         }
+        f(q0, nblk, qx, qy);
     }
 }
 
 // FP32 screening with an exact FP64 decision inside the band. Coordinates
-// are taken relative to the item's first origin r: every origin and candidate
-// of a window lies within 3w of r per axis, so converting a difference to
+// are taken relative to the item's first origin r: origins lie within w of r
+// and candidates within (R+1)w per axis, so converting a difference to
 // float32 and evaluating dx*dx+dy*dy in float32 stays within band of the
-// binary64 value s (band = w*w*2^-14, see FP32_BAND in band_for()).
+// binary64 value s (band_for()).
 struct Band {
     float lo, hi; // accept if s32 <= lo, reject if s32 > hi, else recheck
 };
@@ -179,8 +212,8 @@ __global__ void __launch_bounds__(kBlock)
         fy[b] = (float)__dsub_rn(oy[b], ry);
         cnt[b] = 0;
     }
-    uint32_t nre = 0;
-    window_blocks(g, c, false, 0, [&](uint32_t q, bool v, double qx, double qy) {
+    uint32_t nre = 0, full = 0;
+    auto test = [&](uint32_t q, bool v, double qx, double qy) {
         if (PREC == 0) {
 #pragma unroll
             for (int b = 0; b < kOrigins; b++)
@@ -199,11 +232,17 @@ __global__ void __launch_bounds__(kBlock)
                 cnt[b] += popc64(ballot(acc));
             }
         }
-    });
+    };
+    for (int r = 0; r < g.W; r++) {
+        RowSeg sg = row_seg(g, c, r);
+        run_blocks(g, sg.a, sg.f0, test);
+        full += sg.f1 - sg.f0;
+        run_blocks(g, sg.f1, sg.e, test);
+    }
     uint32_t mine = 0;
 #pragma unroll
     for (int b = 0; b < kOrigins; b++)
-        if (lane_id() == b) mine = cnt[b];
+        if (lane_id() == b) mine = cnt[b] + full;
     if (lane_id() < (int)nb) counts[o0 + lane_id()] = mine;
     if (PREC == 1 && nre && lane_id() == 0) atomicAdd(recheck, (unsigned long long)nre);
 }
@@ -261,6 +300,43 @@ __global__ void __launch_bounds__(kBlock)
     }
     bool over = false;
     uint64_t dig = 0;
+    // Row staging (ids only: no D2, no digest). Accepted ids are appended to
+    // a 128-slot LDS ring per origin and leave in 64-entry segments aligned in
+    // the table, so nearly every global store writes 256 aligned bytes with
+    // all lanes active; appending in 88-byte pieces straight to the table
+    // costs a third of the write bandwidth. Row b holds cur[b] appended ids,
+    // of which fl[b] are written; bnd[b] ends the aligned segment holding id
+    // fl[b], and cur[b] < bnd[b] between appends. Id j sits in slot j mod 128.
+    constexpr bool STAGE = !D2 && !DIGEST;
+    constexpr int kRing = 2 * kWave;
+    __shared__ uint32_t ring_all[STAGE ? kWavesPerBlock * kOrigins * kRing : 1];
+    uint32_t *ring = ring_all + (STAGE ? wave_id_in_block() * kOrigins * kRing : 0);
+    uint32_t cur[kOrigins], fl[kOrigins], bnd[kOrigins], rlen[kOrigins];
+#pragma unroll
+    for (int b = 0; b < kOrigins; b++) {
+        cur[b] = 0;
+        fl[b] = 0;
+        bnd[b] = STAGE ? kWave - (uint32_t)(run[b] & (kWave - 1)) : 0;
+        rlen[b] = STAGE ? (uint32_t)(lim[b] - run[b]) : 0;
+    }
+    // write staged ids [fl, upto) of row b: one store into the aligned segment
+    auto flush = [&](int b, uint32_t upto) {
+        // the ring entries were written by other lanes of this wave; LDS
+        // operations of a wave complete in order
+        __builtin_amdgcn_wave_barrier();
+        uint64_t g0 = (run[b] + fl[b]) & ~(uint64_t)(kWave - 1);
+        int64_t t = (int64_t)(g0 + lane_id()) - (int64_t)run[b];
+        bool w = t >= (int64_t)fl[b] && t < (int64_t)upto && t < (int64_t)rlen[b];
+        if (w) ids[g0 + lane_id()] = ring[b * kRing + ((uint32_t)t & (kRing - 1))];
+        fl[b] = upto;
+        bnd[b] = upto + kWave;
+    };
+    // append the lanes of m (at most 64 ids) to row b
+    auto append = [&](int b, bool acc, uint64_t m, uint32_t id) {
+        if (acc) ring[b * kRing + ((cur[b] + (uint32_t)mask_rank(m)) & (kRing - 1))] = id;
+        cur[b] += (uint32_t)popc64(m);
+        if (cur[b] >= bnd[b]) flush(b, bnd[b]);
+    };
     // origin-major over a register chunk: each origin's accepted ids of the
     // chunk go out as one contiguous run before the next origin's
     auto chunk = [&](uint32_t q0, int nblk, const double *qx, const double *qy,
@@ -291,6 +367,10 @@ __global__ void __launch_bounds__(kBlock)
                 }
                 uint64_t m = ballot(acc);
                 if (m) {
+                    if (STAGE) {
+                        append(b, acc, m, qid[i]);
+                        continue;
+                    }
                     uint64_t dst = run[b] + (uint64_t)mask_rank(m);
                     // the count pass sized every row; never write past it
                     bool w = acc && dst < lim[b];
@@ -304,16 +384,51 @@ __global__ void __launch_bounds__(kBlock)
             }
         }
     };
+    // candidates accepted without a test: every origin's row gets the run of
+    // ids as it stands (no distances, so only without D2)
+    auto full = [&](uint32_t a, uint32_t e) {
+        for (uint32_t q0 = a; q0 < e; q0 += kWave * kChunk) {
+            int nblk = (int)((e - q0 + kWave - 1) / kWave);
+            nblk = nblk < kChunk ? nblk : kChunk;
+            uint32_t qid[kChunk];
+#pragma unroll
+            for (int i = 0; i < kChunk; i++) {
+                uint32_t q = q0 + i * kWave + lane_id();
+                qid[i] = (idmap && i < nblk) ? idmap[q < e ? q : a] : q;
+            }
+#pragma unroll
+            for (int b = 0; b < kOrigins; b++) {
+                if (b >= (int)nb) break;
+#pragma unroll
+                for (int i = 0; i < kChunk; i++) {
+                    if (i >= nblk) break;
+                    uint32_t left = e - (q0 + i * kWave);
+                    uint32_t cnt = left < (uint32_t)kWave ? left : (uint32_t)kWave;
+                    bool v = (uint32_t)lane_id() < cnt;
+                    if (DIGEST) {
+                        if (v) dig += pair_mix(rowid[b], qid[i]);
+                        continue;
+                    }
+                    if (STAGE) {
+                        append(b, v, cnt == (uint32_t)kWave ? ~0ull : (1ull << cnt) - 1, qid[i]);
+                        continue;
+                    }
+                    uint64_t dst = run[b] + (uint64_t)lane_id();
+                    bool w = v && dst < lim[b];
+                    over |= v && !w;
+                    if (w) ids[dst] = qid[i];
+                    run[b] += cnt;
+                }
+            }
+        }
+    };
     if (shared) {
         // the block's 4 items share one cell: stage each chunk of the window
         // in LDS once for all 32 origins
         __shared__ double sx[kChunk * kWave], sy[kChunk * kWave];
         __shared__ uint32_t sid[kChunk * kWave];
         const double qnan2 = __longlong_as_double(0x7ff8000000000000ll);
-        for (int r = 0; r < g.W; r++) {
-            uint32_t lo = g.row_lo[(size_t)c * g.W + r], hi = g.row_hi[(size_t)c * g.W + r];
-            if (lo >= hi) continue;
-            uint32_t a = g.cell_start[lo], e = g.cell_start[hi];
+        auto staged = [&](uint32_t a, uint32_t e) {
             for (uint32_t q0 = a; q0 < e; q0 += kWave * kChunk) {
                 int nblk = (int)((e - q0 + kWave - 1) / kWave);
                 nblk = nblk < kChunk ? nblk : kChunk;
@@ -338,9 +453,16 @@ __global__ void __launch_bounds__(kBlock)
                 __syncthreads();
                 chunk(q0, nblk, qx, qy, qid);
             }
+        };
+        for (int r = 0; r < g.W; r++) {
+            RowSeg sg = row_seg(g, c, r);
+            if (D2) sg.f0 = sg.f1 = sg.e;
+            staged(sg.a, sg.f0);
+            full(sg.f0, sg.f1);
+            staged(sg.f1, sg.e);
         }
     } else {
-        window_chunks(g, c, [&](uint32_t q0, int nblk, const double *qx, const double *qy) {
+        auto tested = [&](uint32_t q0, int nblk, const double *qx, const double *qy) {
             // ids of the chunk's candidates, loaded before any store: a load
             // between stores would make every store wait for all earlier ones
             uint32_t qid[kChunk];
@@ -350,7 +472,23 @@ __global__ void __launch_bounds__(kBlock)
                 qid[i] = (idmap && i < nblk) ? idmap[q < g.n ? q : 0] : q;
             }
             chunk(q0, nblk, qx, qy, qid);
-        });
+        };
+        for (int r = 0; r < g.W; r++) {
+            RowSeg sg = row_seg(g, c, r);
+            if (D2) sg.f0 = sg.f1 = sg.e;
+            run_chunks(g, sg.a, sg.f0, tested);
+            full(sg.f0, sg.f1);
+            run_chunks(g, sg.f1, sg.e, tested);
+        }
+    }
+    if (STAGE) {
+#pragma unroll
+        for (int b = 0; b < kOrigins; b++) {
+            if (b >= (int)nb) break;
+            if (cur[b] > fl[b]) flush(b, cur[b]);
+            // the count pass sized every row
+            over |= cur[b] != rlen[b];
+        }
     }
     if (DIGEST) {
         for (int o = kWave / 2; o > 0; o >>= 1) dig += __shfl_xor(dig, o);
@@ -610,11 +748,13 @@ __global__ void k_perm_counts(const uint32_t *cnt, const uint32_t *pos_rank,
 
 Band band_for(const Grid &G, double T)
 {
-    // |s32 - s| <= w^2 * 2^-14 for coordinates within 3w of the reference
-    // origin (float32 conversion, subtraction, product and sum each add at
-    // most a few ulps of (3w)^2); the band used is that bound, widened to
-    // the float grid around T.
-    double band = G.w * G.w * std::ldexp(1.0, -14);
+    // Origins lie within w and candidates within (R+1)w of the reference
+    // origin per axis, so |dx|, |dy| <= (R+2)w =: D. The float32 conversions
+    // and subtraction leave dx within 2^-23 D, and squares and sum add
+    // |s32 - s| <= D^2 * 2^-20.4 in all; the band used is D^2 * 2^-14 / 9
+    // (w^2 * 2^-14 for R = 1), widened to the float grid around T.
+    double D = (G.R + 2) * G.w;
+    double band = D * D * std::ldexp(1.0, -14) / 9.0;
     Band b;
     double lo = T - band, hi = T + band;
     b.lo = lo > 0 ? std::nextafter((float)lo, 0.0f) : -1.0f;
@@ -670,6 +810,74 @@ int build_items(Context &c, const Grid &G, const char *tag, Items &I)
     k_item_fill<<<ceil_div(G.ncell, kBlock), kBlock, 0, c.stream()>>>(G.cell_start, excl,
                                                                       G.ncell, I.cell, I.o0);
     VDB_HIP(hipGetLastError());
+    return kOk;
+}
+
+int build_eps_grid(Context &c, const Points &P, double eps, int k, const char *tag, Grid &G)
+{
+    k = k < 1 ? 1 : (k > kMaxCellsPerEps ? kMaxCellsPerEps : k);
+    bool enlarged = false;
+    double w = eps * (1.0 + std::ldexp(1.0, -24)) / k;
+    VDB_TRY(build_grid(c, P, w, k, tag, G, &enlarged));
+    return classify_rows(c, G, threshold_for_eps(eps), tag);
+}
+
+int cells_per_eps(const vdb_options *opt)
+{
+    if (opt && opt->cells_per_eps >= 1 && opt->cells_per_eps <= kMaxCellsPerEps)
+        return opt->cells_per_eps;
+    return 0;
+}
+
+int tune_cells_per_eps(Context &c, const Points &P, double eps, int precision, int requested,
+                       int *k_out)
+{
+    if (requested >= 1 && requested <= kMaxCellsPerEps) {
+        *k_out = requested;
+        return kOk;
+    }
+    // below this many points the whole per-variant pass takes well under a
+    // millisecond and timings are noise
+    constexpr uint32_t kTuneMinPoints = 1u << 17;
+    if (P.n < kTuneMinPoints || threshold_for_eps(eps) < 0) {
+        *k_out = kDefaultCellsPerEps;
+        return kOk;
+    }
+    auto bits = [](double v) {
+        uint64_t u;
+        memcpy(&u, &v, sizeof u);
+        return (unsigned long long)u;
+    };
+    char key[160];
+    snprintf(key, sizeof key, "cells:%u:%d:%llx:%llx:%llx:%llx:%llx", P.n, precision, bits(eps),
+             bits(P.ext[0]), bits(P.ext[1]), bits(P.ext[2]), bits(P.ext[3]));
+    auto hit = c.tuned.find(key);
+    if (hit != c.tuned.end()) {
+        *k_out = hit->second;
+        return kOk;
+    }
+    uint32_t *counts;
+    unsigned long long *rc;
+    VDB_TRY(c.get("tune.counts", P.n, &counts));
+    VDB_TRY(c.get("tune.recheck", 1, &rc));
+    int best = kDefaultCellsPerEps;
+    float best_ms = 0;
+    for (int k = 1; k <= kMaxCellsPerEps; k++) {
+        Grid G;
+        Items I;
+        EvTimer tm(c.stream());
+        tm.start();
+        VDB_TRY(build_eps_grid(c, P, eps, k, "g1", G));
+        VDB_TRY(build_items(c, G, "g1", I));
+        VDB_TRY(count_neighbors(c, G, I, eps, precision, counts, (uint64_t *)rc));
+        float ms = tm.stop();
+        if (k == 1 || ms < best_ms) {
+            best = k;
+            best_ms = ms;
+        }
+    }
+    c.tuned[key] = best;
+    *k_out = best;
     return kOk;
 }
 

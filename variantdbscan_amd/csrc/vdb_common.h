@@ -79,6 +79,26 @@ __device__ __forceinline__ double box_dist2_max(double px, double py, double bx0
     return __dadd_rn(__dmul_rn(gx, gx), __dmul_rn(gy, gy));
 }
 
+// Exact lower / upper bounds of dist2_exact(p, q) over every p in box a and q
+// in box b (boxes {x0, x1, y0, y1} of actual point coordinates), by the same
+// monotonicity argument.
+__device__ __forceinline__ double boxes_dist2_min(const double *a, const double *b)
+{
+    double gx = 0.0, gy = 0.0;
+    if (b[0] > a[1]) gx = __dsub_rn(b[0], a[1]);
+    else if (a[0] > b[1]) gx = __dsub_rn(a[0], b[1]);
+    if (b[2] > a[3]) gy = __dsub_rn(b[2], a[3]);
+    else if (a[2] > b[3]) gy = __dsub_rn(a[2], b[3]);
+    return __dadd_rn(__dmul_rn(gx, gx), __dmul_rn(gy, gy));
+}
+
+__device__ __forceinline__ double boxes_dist2_max(const double *a, const double *b)
+{
+    double gx = fmax(__dsub_rn(b[1], a[0]), __dsub_rn(a[1], b[0]));
+    double gy = fmax(__dsub_rn(b[3], a[2]), __dsub_rn(a[3], b[2]));
+    return __dadd_rn(__dmul_rn(gx, gx), __dmul_rn(gy, gy));
+}
+
 // --------------------------------------------------------------- wave ops
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 

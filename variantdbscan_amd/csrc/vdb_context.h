@@ -49,6 +49,9 @@ public:
 
     std::mutex mu; // serialises library calls on this device
 
+    // autotuner choices, keyed by workload (see tune_cells_per_eps)
+    std::unordered_map<std::string, int> tuned;
+
 private:
     int device_;
     hipStream_t own_stream_ = nullptr;

@@ -55,6 +55,12 @@ struct Grid {
     double *cell_box = nullptr;     // [ncell*4] x0,x1,y0,y1 of the cell's points
     uint32_t *row_lo = nullptr;     // [ncell*(2R+1)] first cell of window row
     uint32_t *row_hi = nullptr;     // [ncell*(2R+1)] one past the last
+    // after classify_rows (per-variant grids): inside every window row,
+    // cells [row_f0, row_f1) are within eps of the whole centre cell (every
+    // pair accepted without a test); row_lo/row_hi are then trimmed to cells
+    // that can hold a neighbour at all. Null until classified.
+    uint32_t *row_f0 = nullptr;
+    uint32_t *row_f1 = nullptr;
     uint32_t max_cell = 0;          // largest cell population
     uint32_t npts = 0;              // points in the grid
     // per dataset: grid origin, dimensions and first key (keys of dataset d
@@ -82,5 +88,8 @@ __device__ __forceinline__ uint32_t seg_of(const uint32_t *seg, uint32_t nsets, 
 // tag prefixes the pool slot names so several grids can coexist.
 int build_grid(Context &c, const Points &P, double w_min, int R, const char *tag,
                Grid &G, bool *enlarged);
+
+// Classify the window rows of G for the threshold T (see Grid::row_f0).
+int classify_rows(Context &c, Grid &G, double T, const char *tag);
 
 } // namespace vdb

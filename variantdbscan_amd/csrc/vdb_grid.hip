@@ -399,6 +399,44 @@ int build_rank_order(Context &c, Points &P, const char *tag)
     return kOk;
 }
 
+// Thread per (cell, window row). Rows are runs of consecutive cells whose
+// point boxes are ordered along the row, so trimming from both ends removes
+// every cell whose box lies beyond eps of the centre cell's box; the first
+// run of cells wholly within eps becomes the full range.
+__global__ void k_classify_rows(const double *box, uint32_t ncell, int W, double T,
+                                uint32_t *row_lo, uint32_t *row_hi, uint32_t *row_f0,
+                                uint32_t *row_f1)
+{
+    uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= ncell * (uint32_t)W) return;
+    uint32_t c = t / W;
+    const double *A = box + 4 * (size_t)c;
+    uint32_t lo = row_lo[t], hi = row_hi[t];
+    while (lo < hi && boxes_dist2_min(A, box + 4 * (size_t)lo) > T) lo++;
+    while (hi > lo && boxes_dist2_min(A, box + 4 * (size_t)(hi - 1)) > T) hi--;
+    uint32_t f0 = lo;
+    while (f0 < hi && boxes_dist2_max(A, box + 4 * (size_t)f0) > T) f0++;
+    uint32_t f1 = f0;
+    while (f1 < hi && boxes_dist2_max(A, box + 4 * (size_t)f1) <= T) f1++;
+    if (f0 == f1) f0 = f1 = hi;
+    row_lo[t] = lo;
+    row_hi[t] = hi;
+    row_f0[t] = f0;
+    row_f1[t] = f1;
+}
+
+int classify_rows(Context &c, Grid &G, double T, const char *tag)
+{
+    size_t W = 2 * (size_t)G.R + 1;
+    VDB_TRY(c.get(slot(tag, "row_f0").c_str(), G.ncell * W, &G.row_f0));
+    VDB_TRY(c.get(slot(tag, "row_f1").c_str(), G.ncell * W, &G.row_f1));
+    if (G.ncell == 0) return kOk;
+    k_classify_rows<<<ceil_div(G.ncell * W, kBlock), kBlock, 0, c.stream()>>>(
+        G.cell_box, G.ncell, (int)W, T, G.row_lo, G.row_hi, G.row_f0, G.row_f1);
+    VDB_HIP(hipGetLastError());
+    return kOk;
+}
+
 int build_grid(Context &c, const Points &P, double w_min, int R, const char *tag, Grid &G,
                bool *enlarged)
 {
