@@ -6,19 +6,26 @@ batched engine on 2,000,000 points x 64 variants (eps 0.2..0.9 step 0.1 x
 minpts {4,6,8,12,16,24,32,48}); points and labels resident in HBM; one step =
 one call clustering all 64 variants.
 
-Also measured in the same run (each in its own timed region):
-  * `roofline`: the CSR neighbour-table build (count, scan, fill) at eps 0.5 on
-    the same points, the table kernels' achieved algorithmic GB/s against
-    8 TB/s; the dominant table kernel is the fill.
-  * `plain_fp64`: the plain per-variant FP64 path (grid, count/scan/fill
-    table, union-find over it, labels), one variant per eps level, scaled to
-    variants/s; `ratio_vs_plain` = value / that.
+Also measured in the same run (each in its own timed region, none of them
+`value`):
+  * `roofline`: the CSR neighbour-table fill kernel at eps 0.5 on the same
+    points (ids only, FP64), algorithmic bytes per launch over its HIP-event
+    duration against 8 TB/s; `table_build` adds the count and scan kernels.
+  * `plain_fp64`: the per-variant FP64 engines, one variant per eps level
+    (minpts 8): the fused engine (no table, the library's per-variant
+    default) and the CSR table engine; `ratio_vs_plain` = value / fused
+    variants/s, `ratio_vs_plain_table` = value / table variants/s.
+  * `skew_8m`: 8,000,000 points with a dense 1x1 deg clump, eps 0.3, minpts
+    10, table engine in FP64 and in FP32 with the binary64 recheck (rechecked
+    pair count and fraction).
+  * `multi_256`: 256 uneven datasets (17.8M points) x 16 variants in one
+    vdb_cluster_multi call, datasets x variants per second.
   * `cpu_baseline`: the reference library (compiled from its sources into
-    oracle/_ref) on a 100,000-point random subsample of the same points, all
-    64 variants in one call, on the host's cores.
+    oracle/_ref) on a 100,000-point random subsample of the 2M points, all 64
+    variants in one call, on the host's cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-       [--no-table] [--no-plain] [--only batch|table]
+       [--no-table] [--no-plain] [--no-extra] [--only batch|table]
 """
 import argparse
 import json
@@ -59,6 +66,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-table", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the skew_8m and multi_256 side measurements")
     ap.add_argument("--only", choices=["batch", "table"], default=None)
     ap.add_argument("--table-steps", type=int, default=2)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -168,7 +177,8 @@ def main():
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                            "traffic_source": "profiles/hbm_counters_k_fill.md",
-                           "kernel": "k_fill<0> (CSR neighbour-table fill, FP64)",
+                           "kernel": "k_fill<0, false, false> (CSR neighbour-table fill, FP64, "
+                                     "rows staged in LDS)",
                            "workload": "2M points, eps 0.5, %d pairs, %d tile(s)" %
                            (P, t["tiles"]),
                            "algorithmic_bytes": fb, "ms_per_launch": fill_ms}
@@ -181,33 +191,93 @@ def main():
 
     # ------------------------------------------------ plain per-variant path
     if args.only is None and not args.no_plain:
-        # one variant per eps level (minpts 8, the cost is set by eps)
+        # one variant per eps level (minpts 8, the cost is set by eps), each
+        # clustered on its own by the per-variant FP64 engines: fused (no
+        # table; the library's per-variant default) and CSR table
         lab1 = torch.empty(n, dtype=torch.int32, device=dev)
-        tot_ms = 0.0
-        per = []
-        # largest eps first: its table sizes the pooled buffer the others reuse
-        for e in sorted(E, reverse=True):
-            torch.cuda.synchronize()
-            ea = torch.cuda.Event(enable_timing=True)
-            eb = torch.cuda.Event(enable_timing=True)
-            ea.record(stream)
-            vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, [e], [8], lab1.data_ptr(),
-                              path=vd.PATH_TABLE)
-            eb.record(stream)
-            torch.cuda.synchronize()
-            ms = ea.elapsed_time(eb)
-            per.append(ms)
-            tot_ms += ms
-            log("plain fp64 table path eps %.1f: %.1f ms" % (e, ms))
-        L.vdb_release()
-        torch.cuda.empty_cache()
-        plain_vps = len(E) / (tot_ms / 1e3)
-        per = per[::-1]
-        out["plain_fp64"] = {"variants_per_s": plain_vps, "ms_per_eps_level": per,
+        res = {}
+        for path, nm in ((vd.PATH_FUSED, "fused"), (vd.PATH_TABLE, "table")):
+            per = {}
+            # largest eps first: its table sizes the pooled buffer the others reuse
+            for e in sorted(E, reverse=True):
+                # first call of a workload runs the grid autotuner: untimed
+                vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, [e], [8], lab1.data_ptr(),
+                                  path=path)
+                torch.cuda.synchronize()
+                ea = torch.cuda.Event(enable_timing=True)
+                eb = torch.cuda.Event(enable_timing=True)
+                ea.record(stream)
+                vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, [e], [8], lab1.data_ptr(),
+                                  path=path)
+                eb.record(stream)
+                torch.cuda.synchronize()
+                per[e] = ea.elapsed_time(eb)
+                log("plain fp64 %s path eps %.1f: %.1f ms" % (nm, e, per[e]))
+            ms = [per[e] for e in E]
+            res[nm] = {"variants_per_s": len(E) / (sum(ms) / 1e3), "ms_per_eps_level": ms}
+            L.vdb_release()
+            torch.cuda.empty_cache()
+        out["plain_fp64"] = {"variants_per_s": res["fused"]["variants_per_s"],
+                             "engine": "fused per-variant (grid, count, union-find over the "
+                                       "grid window, labels)",
+                             "ms_per_eps_level": res["fused"]["ms_per_eps_level"],
+                             "table_engine": res["table"],
                              "sample": "minpts 8 at each of the 8 eps levels"}
         if "value" in out:
-            out["ratio_vs_plain"] = out["value"] / plain_vps
+            out["ratio_vs_plain"] = out["value"] / res["fused"]["variants_per_s"]
+            out["ratio_vs_plain_table"] = out["value"] / res["table"]["variants_per_s"]
         del lab1
+
+    # ------------------------------------------ other workloads (not `value`)
+    if args.only is None and not args.no_extra:
+        del dx, dy
+        torch.cuda.empty_cache()
+        # 8M points with a dense clump, eps 0.3, minpts 10: FP32 screening
+        # with binary64 recheck against plain FP64
+        sx, sy = datasets.skew_8m()
+        dsx = torch.from_numpy(sx).to(dev)
+        dsy = torch.from_numpy(sy).to(dev)
+        lab = torch.empty(len(sx), dtype=torch.int32, device=dev)
+        rec = {}
+        for prec, nm in ((vd.PREC_FP64, "fp64"), (vd.PREC_FP32_RECHECK, "fp32_recheck")):
+            for rep in range(2):
+                st = vd.cluster_device(dsx.data_ptr(), dsy.data_ptr(), len(sx), [0.3], [10],
+                                       lab.data_ptr(), path=vd.PATH_TABLE, precision=prec)
+            rec[nm] = {"ms_total": st["ms_total"], "ms_count": st["ms_count"],
+                       "ms_fill": st["ms_fill"], "pairs": st["pairs"],
+                       "rechecked": st["rechecked"]}
+            log("skew_8m table path %s: %.1f ms, rechecked %d" % (nm, st["ms_total"],
+                                                                 st["rechecked"]))
+        st = vd.cluster_device(dsx.data_ptr(), dsy.data_ptr(), len(sx), [0.3], [10],
+                               lab.data_ptr(), path=vd.PATH_AUTO)
+        rec["auto_ms_total"] = st["ms_total"]
+        rec["recheck_fraction"] = rec["fp32_recheck"]["rechecked"] / max(1, rec["fp32_recheck"]["pairs"])
+        out["skew_8m"] = rec
+        del dsx, dsy, lab
+        L.vdb_release()
+        torch.cuda.empty_cache()
+        # 256 uneven datasets x 16 variants in one call
+        sets = datasets.multi_256()
+        E5, M5 = datasets.MULTI_EPS, datasets.MULTI_MINPTS
+        eps5 = np.array([e for e in E5 for m in M5])
+        mp5 = np.array([m for e in E5 for m in M5], dtype=np.uint32)
+        offs = np.zeros(len(sets) + 1, dtype=np.uint64)
+        offs[1:] = np.cumsum([len(s[0]) for s in sets])
+        mx = torch.from_numpy(np.concatenate([s[0] for s in sets])).to(dev)
+        my = torch.from_numpy(np.concatenate([s[1] for s in sets])).to(dev)
+        ml = torch.empty(int(offs[-1]) * len(eps5), dtype=torch.int32, device=dev)
+        for rep in range(3):
+            st = vd.cluster_multi_device(mx.data_ptr(), my.data_ptr(), offs, eps5, mp5,
+                                         ml.data_ptr())
+        out["multi_256"] = {"datasets": len(sets), "points": int(offs[-1]),
+                            "variants": len(eps5), "ms_total": st["ms_total"],
+                            "datasets_x_variants_per_s": len(sets) * len(eps5) /
+                            (st["ms_total"] / 1e3)}
+        log("multi_256: %.1f ms -> %.0f datasets x variants / s" %
+            (st["ms_total"], out["multi_256"]["datasets_x_variants_per_s"]))
+        del mx, my, ml
+        L.vdb_release()
+        torch.cuda.empty_cache()
 
     # ----------------------------------------------------------- CPU leg
     if args.only is None and not args.no_cpu:
@@ -266,7 +336,8 @@ def main():
         "roofline": out.get("roofline"),
         "cpu_baseline": out.get("cpu_baseline"),
     }
-    for k in ("ratio_vs_plain", "plain_fp64", "table_build", "batch_phases_ms",
+    for k in ("ratio_vs_plain", "ratio_vs_plain_table", "plain_fp64", "table_build",
+              "skew_8m", "multi_256", "batch_phases_ms",
               "labels_match_per_variant_engine", "wall_s"):
         if k in out:
             line[k] = out[k]

@@ -970,9 +970,12 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
         VDB_TRY(c.get("cv.ovf", 1, &ovf));
         VDB_HIP(hipMemsetAsync(ovf, 0, sizeof(uint32_t), c.stream()));
         uint64_t want = P_total < cap ? P_total : cap;
-        // the slot keeps the largest table seen so far; reuse it whole
+        // the slot keeps the largest table seen so far; reuse it whole, and
+        // when the table is tiled anyway keep a slot of at least half the
+        // budget rather than reallocating hundreds of GB for a few percent
         uint64_t have = c.slot_bytes("cv.ids") / sizeof(uint32_t);
         if (have > want && have <= cap) want = have;
+        if (have < want && P_total > have && have >= cap / 2) want = have;
         VDB_TRY(c.get("cv.ids", want ? want : 1, &ids));
         // tiles are runs of items (items are ordered by cell, origins ascending)
         uint32_t ntiles = 0;
