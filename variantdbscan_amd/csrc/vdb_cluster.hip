@@ -267,7 +267,8 @@ __global__ void __launch_bounds__(kBlock)
            const uint32_t *__restrict__ item_o0, uint32_t it0, uint32_t it1, double T,
            Band band, const uint64_t *__restrict__ offsets, uint64_t off_base,
            const uint32_t *__restrict__ rowmap, const uint32_t *__restrict__ idmap,
-           uint32_t *__restrict__ ids, float *__restrict__ d2, uint32_t *overflow)
+           uint32_t *__restrict__ ids, float *__restrict__ d2, uint64_t cap,
+           uint32_t *overflow)
 {
     // block-uniform: do the block's 4 items exist and share one cell?
     uint32_t ib = it0 + blockIdx.x * kWavesPerBlock;
@@ -317,7 +318,11 @@ __global__ void __launch_bounds__(kBlock)
         cur[b] = 0;
         fl[b] = 0;
         bnd[b] = STAGE ? kWave - (uint32_t)(run[b] & (kWave - 1)) : 0;
-        rlen[b] = STAGE ? (uint32_t)(lim[b] - run[b]) : 0;
+        // writes stop at the row end and at the end of the ids buffer (a row
+        // cut short there raises the overflow flag below)
+        uint64_t room = cap > run[b] ? cap - run[b] : 0;
+        uint64_t len = lim[b] - run[b];
+        rlen[b] = STAGE ? (uint32_t)(len < room ? len : room) : 0;
     }
     // write staged ids [fl, upto) of row b: one store into the aligned segment
     auto flush = [&](int b, uint32_t upto) {
@@ -373,7 +378,7 @@ __global__ void __launch_bounds__(kBlock)
                     }
                     uint64_t dst = run[b] + (uint64_t)mask_rank(m);
                     // the count pass sized every row; never write past it
-                    bool w = acc && dst < lim[b];
+                    bool w = acc && dst < lim[b] && dst < cap;
                     over |= acc && !w;
                     if (w) {
                         ids[dst] = qid[i];
@@ -414,7 +419,7 @@ __global__ void __launch_bounds__(kBlock)
                         continue;
                     }
                     uint64_t dst = run[b] + (uint64_t)lane_id();
-                    bool w = v && dst < lim[b];
+                    bool w = v && dst < lim[b] && dst < cap;
                     over |= v && !w;
                     if (w) ids[dst] = qid[i];
                     run[b] += cnt;
@@ -663,7 +668,10 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // --------------------------------------------- union-find over a CSR table
-// one wave per row; rows [p0, p1) of position space, ids are positions
+// One wave per row; rows [p0, p1) of position space, ids are positions.
+// Each unordered core pair is united from its larger end only. A row lists
+// its window rows in ascending cell key and each run in position order, so
+// its ids ascend: the scan stops at the first block that reaches p.
 __global__ void __launch_bounds__(kBlock)
     k_uf_table(const uint64_t *__restrict__ offsets, uint64_t off_base,
                const uint32_t *__restrict__ ids, uint32_t p0, uint32_t p1,
@@ -684,6 +692,7 @@ __global__ void __launch_bounds__(kBlock)
             if (need) uf_unite(parent, rank, q, p);
             rp = uf_find(parent, p);
         }
+        if (ballot(j < e && q >= p)) break;
     }
 }
 
@@ -977,6 +986,8 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
         if (have > want && have <= cap) want = have;
         if (have < want && P_total > have && have >= cap / 2) want = have;
         VDB_TRY(c.get("cv.ids", want ? want : 1, &ids));
+        // tiles never exceed the buffer actually held
+        cap = want;
         // tiles are runs of items (items are ordered by cell, origins ascending)
         uint32_t ntiles = 0;
         uint32_t i0 = 0;
@@ -996,7 +1007,7 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
             if (pt) tm.start();
             k_fill<0, false><<<ceil_div(i1 - i0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
                 view(G, G.npts), I.cell, I.o0, i0, i1, T, Band{}, offsets, base, nullptr, nullptr, ids,
-                nullptr, ovf);
+                nullptr, cap, ovf);
             VDB_HIP(hipGetLastError());
             if (pt) pt->fill += tm.stop(), tm.start();
             uint32_t p0 = hio0[i0];
@@ -1027,7 +1038,7 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
                 // the table no longer holds this tile: rebuild it
                 k_fill<0, false><<<ceil_div(i1 - i0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
                     view(G, G.npts), I.cell, I.o0, i0, i1, T, Band{}, offsets, base, nullptr, nullptr,
-                    ids, nullptr, ovf);
+                    ids, nullptr, cap, ovf);
                 VDB_HIP(hipGetLastError());
             }
             k_border_table<<<ceil_div(pend - p0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
@@ -1097,11 +1108,11 @@ int table_digest(Context &c, const Points &P, const Grid &G, const Items &I, dou
         if (precision == 0)
             k_fill<0, false, true><<<blocks, kBlock, 0, c.stream()>>>(
                 view(G, G.npts), I.cell, I.o0, 0, I.n, T, bd, nullptr, 0, caller_of_pos, caller_of_pos,
-                (uint32_t *)acc, nullptr, nullptr);
+                (uint32_t *)acc, nullptr, 0, nullptr);
         else
             k_fill<1, false, true><<<blocks, kBlock, 0, c.stream()>>>(
                 view(G, G.npts), I.cell, I.o0, 0, I.n, T, bd, nullptr, 0, caller_of_pos, caller_of_pos,
-                (uint32_t *)acc, nullptr, nullptr);
+                (uint32_t *)acc, nullptr, 0, nullptr);
         VDB_HIP(hipGetLastError());
     }
     unsigned long long h[2];
@@ -1161,19 +1172,19 @@ int neighbor_table(Context &c, const Points &P, const Grid &G, const Items &I, d
     if (precision == 0 && !d2)
         k_fill<0, false><<<blocks, kBlock, 0, c.stream()>>>(
             view(G, G.npts), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
-            d2, ovf);
+            d2, capacity, ovf);
     else if (precision == 0)
         k_fill<0, true><<<blocks, kBlock, 0, c.stream()>>>(
             view(G, G.npts), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
-            d2, ovf);
+            d2, capacity, ovf);
     else if (!d2)
         k_fill<1, false><<<blocks, kBlock, 0, c.stream()>>>(
             view(G, G.npts), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
-            d2, ovf);
+            d2, capacity, ovf);
     else
         k_fill<1, true><<<blocks, kBlock, 0, c.stream()>>>(
             view(G, G.npts), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
-            d2, ovf);
+            d2, capacity, ovf);
     VDB_HIP(hipGetLastError());
     if (pt) pt->fill += tm.stop();
     uint32_t hov = 0;
