@@ -138,7 +138,7 @@ __device__ __forceinline__ bool before(double s1, uint32_t i1, double s2, uint32
 
 // Insert the candidates of one 64-lane block that pass `hit` into the sorted
 // lane-resident list (entry e in lane e < L), keeping the L smallest (s, id).
-__device__ __forceinline__ void knn_block(double s, uint32_t q0, bool hit, int L, double Tmax,
+__device__ __forceinline__ void knn_block(double s, uint32_t q, bool hit, int L, double Tmax,
                                           double &es, uint32_t &eid, int &cnt, double &kth,
                                           uint32_t &kid, double &thr)
 {
@@ -148,7 +148,7 @@ __device__ __forceinline__ void knn_block(double s, uint32_t q0, bool hit, int L
         int hb = __builtin_ctzll(hm);
         hm &= hm - 1;
         double cs = __shfl(s, hb);
-        uint32_t cq = q0 + hb;
+        uint32_t cq = __shfl(q, hb);
         if (cnt == L && !before(cs, cq, kth, kid)) continue;
         bool lt = lane < L && before(es, eid, cs, cq);
         int pos = popc64(ballot(lt));
@@ -170,16 +170,49 @@ __device__ __forceinline__ void knn_block(double s, uint32_t q0, bool hit, int L
     }
 }
 
-// rows holding at most this many points are swept whole, without per-cell pruning
-constexpr uint32_t kKnnRowSweep = 128;
+// spans of a row holding at most this many points are swept whole, without
+// per-cell pruning
+constexpr uint32_t kKnnRowSweep = 512;
+// windows holding at most this many points are swept as one flat list
+constexpr uint32_t kKnnFlat = 256;
+
+// sort the lane-resident entries (lane e holds entry e) by (s, id), ascending
+__device__ __forceinline__ void wave_sort(double &es, uint32_t &eid)
+{
+    int lane = lane_id();
+#pragma unroll
+    for (int k = 2; k <= kWave; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            double ps = __shfl_xor(es, j);
+            uint32_t pid = __shfl_xor(eid, j);
+            bool low = (lane & j) == 0, asc = (lane & k) == 0;
+            // the lower lane of an ascending pair keeps the smaller entry
+            bool take = (low == asc) ? before(ps, pid, es, eid) : before(es, eid, ps, pid);
+            if (take) {
+                es = ps;
+                eid = pid;
+            }
+        }
+    }
+}
 
 __global__ void __launch_bounds__(kBlock)
     k_knn(FG g, const uint32_t *__restrict__ list_pos, uint32_t nlist, int L, double Tmax,
           double T0, double *__restrict__ lst_s, uint32_t *__restrict__ lst_id,
           uint32_t *__restrict__ lst_n)
 {
+    // Two modes. While at most 64 candidates within eps_max have been seen,
+    // they are appended unsorted (compacted through LDS) and sorted once at
+    // the end: a sparse point costs one sort instead of one insertion per
+    // candidate. A block that would overflow 64 entries sorts, keeps the L
+    // smallest and switches to sorted insertion with a shrinking threshold.
+    __shared__ double abuf_s[kWPB][kWave];
+    __shared__ uint32_t abuf_id[kWPB][kWave];
     uint32_t i = blockIdx.x * kWPB + wave_id_in_block();
     if (i >= nlist) return;
+    double *bs = abuf_s[wave_id_in_block()];
+    uint32_t *bid = abuf_id[wave_id_in_block()];
     int lane = lane_id();
     uint32_t p = list_pos[i];
     double px = g.xs[p], py = g.ys[p];
@@ -191,26 +224,95 @@ __global__ void __launch_bounds__(kBlock)
     uint32_t kid = kNone;
     double thr = Tmax;     // candidates must have s <= thr
     bool done = false;
+    bool appending = true;
+    // sort the appended entries, keep the L smallest, enter insertion mode
+    auto settle = [&]() {
+        wave_sort(es, eid);
+        if (lane >= L) {
+            es = INFINITY;
+            eid = kNone;
+        }
+        cnt = cnt < L ? cnt : L;
+        if (cnt == L) {
+            kth = __shfl(es, L - 1);
+            kid = __shfl(eid, L - 1);
+            thr = kth < Tmax ? kth : Tmax;
+        }
+        appending = false;
+    };
+    // one block of candidates: lane's candidate q at squared distance s
+    // (INF: no candidate)
+    auto block = [&](double s, uint32_t q) {
+        if (appending) {
+            bool hit = s <= Tmax;
+            uint64_t hm = ballot(hit);
+            int h = popc64(hm);
+            if (cnt + h <= kWave) {
+                if (hit) {
+                    int at = cnt + mask_rank(hm);
+                    bs[at] = s;
+                    bid[at] = q;
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (lane >= cnt && lane < cnt + h) {
+                    es = bs[lane];
+                    eid = bid[lane];
+                }
+                __builtin_amdgcn_wave_barrier();
+                cnt += h;
+                // L entries within eps_0: core in every variant, list unused
+                if (cnt >= L && popc64(ballot(es <= T0)) >= L) {
+                    settle();
+                    done = true;
+                }
+                return;
+            }
+            settle();
+        }
+        bool hit = s <= thr && (cnt < L ? true : before(s, q, kth, kid));
+        knn_block(s, q, hit, L, Tmax, es, eid, cnt, kth, kid, thr);
+        // L neighbours within eps_0: core in every variant, list unused
+        if (cnt == L && kth <= T0) done = true;
+    };
     int R = g.W / 2;
+    // All window rows' point ranges in one round of loads (lane r: row r).
+    // A window holding few points is swept as one flat list of candidates,
+    // 64 per step, instead of row by row.
+    uint32_t ra = 0, rn = 0;
+    if (lane < g.W) {
+        uint32_t lo = g.row_lo[(size_t)c * g.W + lane], hi = g.row_hi[(size_t)c * g.W + lane];
+        if (lo < hi) {
+            ra = g.cell_start[lo];
+            rn = g.cell_start[hi] - ra;
+        }
+    }
+    uint32_t pre = rn; // inclusive prefix over rows
+    for (int o = 1; o < kWave; o <<= 1) {
+        uint32_t t = __shfl_up(pre, o);
+        if (lane >= o) pre += t;
+    }
+    uint32_t total = __shfl(pre, kWave - 1);
+    pre -= rn; // exclusive
+    if (total <= kKnnFlat) {
+        for (uint32_t f0 = 0; f0 < total && !done; f0 += kWave) {
+            uint32_t f = f0 + lane;
+            // row of flat index f: the last row whose prefix is <= f
+            int row = 0;
+            for (int r = 1; r < g.W; r++) row += __shfl(pre, r) <= f ? 1 : 0;
+            uint32_t q = __shfl(ra, row) + (f - __shfl(pre, row));
+            double s = INFINITY;
+            if (f < total) s = dist2_exact(px, py, g.xs[q], g.ys[q]);
+            block(s, f < total ? q : kNone);
+        }
+        done = true;
+    }
     for (int t = 0; t < g.W && !done; t++) {
         int d = (t & 1) ? -((t + 1) >> 1) : (t >> 1); // 0, -1, +1, -2, +2, ...
         int r = d + R;
         uint32_t lo = g.row_lo[(size_t)c * g.W + r], hi = g.row_hi[(size_t)c * g.W + r];
         if (lo >= hi) continue;
-        uint32_t ra = g.cell_start[lo], re = g.cell_start[hi];
-        if (re - ra <= kKnnRowSweep) {
-            // sparse row: every point of the row's cells, contiguous in memory
-            for (uint32_t q0 = ra; q0 < re; q0 += kWave) {
-                uint32_t q = q0 + lane;
-                double s = INFINITY;
-                if (q < re) s = dist2_exact(px, py, g.xs[q], g.ys[q]);
-                bool hit = s <= thr && (cnt < L ? true : before(s, q, kth, kid));
-                knn_block(s, q0, hit, L, Tmax, es, eid, cnt, kth, kid, thr);
-            }
-            if (cnt == L && kth <= T0) done = true;
-            continue;
-        }
         for (uint32_t cb = lo; cb < hi && !done; cb += kWave) {
+            // the row's cells that can hold a candidate (exact box bound)
             uint32_t C = cb + lane;
             double S = INFINITY;
             if (C < hi) {
@@ -218,25 +320,37 @@ __global__ void __launch_bounds__(kBlock)
                 S = box_dist2_min(px, py, b[0], b[1], b[2], b[3]);
             }
             uint64_t m = ballot(S <= thr);
+            if (!m) continue;
+            // their points are one contiguous span (cells of a row are
+            // consecutive): sweep it whole when short, else cell by cell
+            uint32_t cf = cb + __builtin_ctzll(m), cl = cb + 63 - __builtin_clzll(m);
+            uint32_t a = g.cell_start[cf], e = g.cell_start[cl + 1];
+            if (e - a <= kKnnRowSweep) {
+                for (uint32_t q0 = a; q0 < e && !done; q0 += kWave) {
+                    uint32_t q = q0 + lane;
+                    double s = INFINITY;
+                    if (q < e) s = dist2_exact(px, py, g.xs[q], g.ys[q]);
+                    block(s, q);
+                }
+                continue;
+            }
             while (m && !done) {
                 int bit = __builtin_ctzll(m);
                 m &= m - 1;
                 double SC = __shfl(S, bit);
                 if (SC > thr) continue;
                 uint32_t Cc = cb + bit;
-                uint32_t a = g.cell_start[Cc], e = g.cell_start[Cc + 1];
-                for (uint32_t q0 = a; q0 < e; q0 += kWave) {
+                uint32_t ca = g.cell_start[Cc], ce = g.cell_start[Cc + 1];
+                for (uint32_t q0 = ca; q0 < ce && !done; q0 += kWave) {
                     uint32_t q = q0 + lane;
                     double s = INFINITY;
-                    if (q < e) s = dist2_exact(px, py, g.xs[q], g.ys[q]);
-                    bool hit = s <= thr && (cnt < L ? true : before(s, q, kth, kid));
-                    knn_block(s, q0, hit, L, Tmax, es, eid, cnt, kth, kid, thr);
+                    if (q < ce) s = dist2_exact(px, py, g.xs[q], g.ys[q]);
+                    block(s, q);
                 }
-                // L neighbours within eps_0: core in every variant, list unused
-                if (cnt == L && kth <= T0) done = true;
             }
         }
     }
+    if (appending) settle();
     if (lane < L) {
         lst_s[(size_t)i * L + lane] = es;
         lst_id[(size_t)i * L + lane] = eid;
@@ -885,6 +999,11 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     Ev tprep(c.stream());
     double d = lev[0] / std::sqrt(2.0) * (1.0 - std::ldexp(1.0, -20));
     int R = (int)std::ceil(lev[K - 1] * (1.0 + std::ldexp(1.0, -30)) / d);
+    // a window row per lane in k_knn (eps ratio 16 gives R = 23)
+    if (2 * R + 1 > kWave) {
+        set_error("eps range too wide for one fine grid");
+        return kErrUnsupported;
+    }
     Grid G;
     bool enlarged = false;
     VDB_TRY(build_grid(c, P, d, R, "gb", G, &enlarged));
