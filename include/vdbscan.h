@@ -138,6 +138,35 @@ int vdb_neighbor_table_digest(const double *x, const double *y, uint32_t n, doub
                               uint32_t flags, int precision, uint64_t *digest,
                               uint64_t *pairs, uint64_t *rechecked);
 
+/* ---------------------------------------------------------------------- */
+/* Dataset and variant files of the reference's callers (host only: no GPU
+ * is touched). Output arrays are malloc'd by the library; release them with
+ * vdb_free. Fields parse like atof/atoi (C strtod/strtol), so values equal
+ * the reference's; lines without a comma are skipped. */
+
+/* x,y CSV, one point per line (replaces the import loop of the reference's
+ * harness, src/c_test_prog.cpp:86-104). */
+int vdb_read_xy_csv(const char *path, double **x, double **y, uint32_t *n);
+
+/* lon,lat[,TEC] CSV of the GPU prototype (replaces importDataset,
+ * src/gpu/import_dataset.cpp:63-134): x = latitude moved to [0, 180]
+ * (90 -> 0, -90 -> 180) with the prototype's arithmetic, y = longitude. */
+int vdb_read_lonlat_csv(const char *path, double **x, double **y, uint32_t *n);
+
+/* eps,minpts CSV, one variant per line (replaces importDBScanInstances,
+ * src/gpu/import_dataset.cpp:232-262, without its sort: file order is kept). */
+int vdb_read_variants_csv(const char *path, double **eps, uint32_t **minpts,
+                          uint32_t *nvar);
+
+/* Labels as CSV, one line per point: x,y,label_0,...,label_{nvar-1} (the
+ * layout of the reference's src/test_output.csv); labels in retArr layout.
+ * path "-" writes to stdout. No counterpart symbol in the reference. */
+int vdb_write_labels_csv(const char *path, const double *x, const double *y, uint32_t n,
+                         const uint32_t *labels, uint32_t nvar);
+
+/* Releases an array returned by the readers above. */
+void vdb_free(void *p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,3 +77,20 @@ def test_datasets_are_seeded():
     assert x.min() >= -90 and x.max() <= 90 and y.min() >= -180 and y.max() <= 180
     s = datasets.multi_sizes()
     assert len(s) == 256 and s.min() >= 1000 and s.max() <= 500_000
+
+
+def test_reference_package_names_resolve():
+    # the reference's package and module names (python_package/setup.py,
+    # src/vtest.py) give the drop-in call
+    import importlib
+    import os
+    import sys
+    import variantdbscan_amd
+    compat = os.path.join(os.path.dirname(variantdbscan_amd.__file__), "compat")
+    sys.path.insert(0, compat)
+    try:
+        for mod in ("vdbscan", "vdbscan.variantdbscan", "variantdbscan"):
+            m = importlib.import_module(mod)
+            assert m.VariantDBSCAN is variantdbscan_amd.VariantDBSCAN, mod
+    finally:
+        sys.path.remove(compat)

@@ -20,7 +20,8 @@ PREC_FP64, PREC_FP32_RECHECK = 0, 1
 
 EXPORTED = ("libVDBSCAN", "vdb_last_error", "vdb_version", "vdb_set_stream", "vdb_release",
             "vdb_cluster", "vdb_rank_order", "vdb_neighbor_counts", "vdb_neighbor_table",
-            "vdb_neighbor_table_digest", "vdb_cluster_multi")
+            "vdb_neighbor_table_digest", "vdb_cluster_multi", "vdb_read_xy_csv",
+            "vdb_read_lonlat_csv", "vdb_read_variants_csv", "vdb_write_labels_csv", "vdb_free")
 
 
 class VdbOptions(ctypes.Structure):
@@ -90,6 +91,19 @@ def load():
                                     ctypes.c_uint32, ctypes.POINTER(VdbOptions),
                                     ctypes.POINTER(VdbStats)]
     L.vdb_cluster_multi.restype = ctypes.c_int
+    dpp = ctypes.POINTER(ctypes.POINTER(ctypes.c_double))
+    upp = ctypes.POINTER(ctypes.POINTER(ctypes.c_uint32))
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    for nm in ("vdb_read_xy_csv", "vdb_read_lonlat_csv"):
+        getattr(L, nm).argtypes = [ctypes.c_char_p, dpp, dpp, u32p]
+        getattr(L, nm).restype = ctypes.c_int
+    L.vdb_read_variants_csv.argtypes = [ctypes.c_char_p, dpp, upp, u32p]
+    L.vdb_read_variants_csv.restype = ctypes.c_int
+    L.vdb_write_labels_csv.argtypes = [ctypes.c_char_p, vp, vp, ctypes.c_uint32, vp,
+                                       ctypes.c_uint32]
+    L.vdb_write_labels_csv.restype = ctypes.c_int
+    L.vdb_free.argtypes = [vp]
+    L.vdb_free.restype = None
     _lib = L
     return L
 
@@ -255,3 +269,48 @@ def cluster_multi_device(x_ptr, y_ptr, offsets, eps, minpts, labels_ptr, path=PA
                                ctypes.c_void_p(labels_ptr), DEVICE_PTRS, ctypes.byref(opt),
                                ctypes.byref(st)))
     return st.as_dict()
+
+
+# ------------------------------------------------------------ files (host only)
+def _take(L, p, n, dtype):
+    """Copy a library-allocated array of n values into numpy and release it."""
+    try:
+        return np.ctypeslib.as_array(p, shape=(n,)).astype(dtype, copy=True) if n else \
+            np.zeros(0, dtype=dtype)
+    finally:
+        L.vdb_free(ctypes.cast(p, ctypes.c_void_p))
+
+
+def read_points_csv(path, lonlat=False):
+    """(x, y) float64 from an x,y CSV (the reference harness's format) or, with
+    lonlat=True, a lon,lat[,TEC] CSV (the GPU prototype's: x = latitude moved
+    to [0, 180], y = longitude)."""
+    L = load()
+    px = ctypes.POINTER(ctypes.c_double)()
+    py = ctypes.POINTER(ctypes.c_double)()
+    n = ctypes.c_uint32(0)
+    fn = L.vdb_read_lonlat_csv if lonlat else L.vdb_read_xy_csv
+    _check(fn(os.fsencode(path), ctypes.byref(px), ctypes.byref(py), ctypes.byref(n)))
+    return _take(L, px, n.value, np.float64), _take(L, py, n.value, np.float64)
+
+
+def read_variants_csv(path):
+    """(eps float64, minpts uint32) from an eps,minpts CSV, file order."""
+    L = load()
+    pe = ctypes.POINTER(ctypes.c_double)()
+    pm = ctypes.POINTER(ctypes.c_uint32)()
+    n = ctypes.c_uint32(0)
+    _check(L.vdb_read_variants_csv(os.fsencode(path), ctypes.byref(pe), ctypes.byref(pm),
+                                   ctypes.byref(n)))
+    return _take(L, pe, n.value, np.float64), _take(L, pm, n.value, np.uint32)
+
+
+def write_labels_csv(path, x, y, labels):
+    """Write x,y,label_0,..,label_{V-1} per point; labels (V, n) uint32."""
+    L = load()
+    x, y = _f64(x), _f64(y)
+    lab = np.ascontiguousarray(np.atleast_2d(labels), dtype=np.uint32)
+    if lab.shape[1] != len(x) or len(x) != len(y):
+        raise ValueError("length mismatch")
+    _check(L.vdb_write_labels_csv(os.fsencode(path), _ptr(x), _ptr(y), len(x), _ptr(lab),
+                                  lab.shape[0]))
