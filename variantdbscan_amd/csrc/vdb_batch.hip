@@ -1201,6 +1201,14 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             k_test<<<kPersistentBlocks, kBlock, 0, c.stream()>>>(a, tests2, cnts + 3, parent,
                                                                  lout, cnts + out_idx);
             VDB_HIP(hipGetLastError());
+            if (verbose) {
+                // worklist sizes of this level (a sync per level: verbose only)
+                uint32_t hc[4];
+                VDB_HIP(hipMemcpyAsync(hc, cnts, sizeof hc, hipMemcpyDeviceToHost, c.stream()));
+                VDB_HIP(hipStreamSynchronize(c.stream()));
+                printf("\n[vdbscan] batch j=%d k=%d: %u fresh, %u near + %u far tests, %u carried on",
+                       j, k, nfresh, hc[2], hc[3], hc[out_idx]);
+            }
             std::swap(lin, lout);
             in_idx = out_idx;
             if (want.find({k, j}) == want.end()) continue;
