@@ -682,17 +682,34 @@ __global__ void __launch_bounds__(kBlock)
     if (p >= p1 || !core[p]) return;
     uint64_t a = offsets[p] - off_base, e = offsets[p + 1] - off_base;
     uint32_t rp = uf_find(parent, p);
-    for (uint64_t j0 = a; j0 < e; j0 += kWave) {
-        uint64_t j = j0 + lane_id();
-        uint32_t q = j < e ? ids[j] : kNone;
-        bool qc = q < p && core[q];
-        uint32_t pq = qc ? ld_par(parent + q) : 0u;
-        bool need = qc && pq != rp;
-        if (ballot(need)) {
-            if (need) uf_unite(parent, rank, q, p);
-            rp = uf_find(parent, p);
+    // eight blocks per step: their id, core and parent loads overlap
+    constexpr int U = 8;
+    for (uint64_t j0 = a; j0 < e; j0 += U * kWave) {
+        uint32_t q[U], pq[U];
+        bool qc[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            uint64_t j = j0 + u * kWave + lane_id();
+            q[u] = j < e ? ids[j] : kNone;
         }
-        if (ballot(j < e && q >= p)) break;
+#pragma unroll
+        for (int u = 0; u < U; u++) qc[u] = q[u] < p && core[q[u]];
+        // A cached parent may be stale: a stale "same root" is still right
+        // (a node once under rp stays in rp's component), a stale "different"
+        // only costs a union attempt, which rereads at agent scope.
+#pragma unroll
+        for (int u = 0; u < U; u++) pq[u] = qc[u] ? parent[q[u]] : 0u;
+        bool past = false;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            bool need = qc[u] && pq[u] != rp;
+            if (ballot(need)) {
+                if (need) uf_unite(parent, rank, q[u], p);
+                rp = uf_find(parent, p);
+            }
+            past |= q[u] != kNone && q[u] >= p;
+        }
+        if (ballot(past)) break;
     }
 }
 
