@@ -582,21 +582,42 @@ __global__ void __launch_bounds__(kBlock)
     }
     if (!any) return;
     uint32_t qmax = o0 + nb;
-    window_blocks(g, c, true, qmax, [&](uint32_t q, bool v, double qx, double qy) {
+    // the half window (rows below the centre, the centre row before the
+    // item's last origin), row segments accepted without a test included
+    auto blockfn = [&](uint32_t q, bool v, double qx, double qy, bool full) {
         bool qc = v && core[q];
         if (!ballot(qc)) return;
         uint32_t pq = qc ? ld_par(parent + q) : 0u;
 #pragma unroll
         for (int b = 0; b < kOrigins; b++) {
             if (rp[b] == kNone) continue;
-            bool e = qc && dist2_exact(ox[b], oy[b], qx, qy) <= T && pq != rp[b];
+            bool e = qc && (full || dist2_exact(ox[b], oy[b], qx, qy) <= T) && pq != rp[b];
             uint64_t m = ballot(e);
             if (m) {
                 if (e) uf_unite(parent, rank, q, o0 + b);
                 rp[b] = uf_find(parent, o0 + b);
             }
         }
-    });
+    };
+    int R = g.W / 2;
+    for (int r = 0; r <= R; r++) {
+        RowSeg sg = row_seg(g, c, r);
+        if (r == R) {
+            sg.e = sg.e < qmax ? sg.e : qmax;
+            sg.f1 = sg.f1 < sg.e ? sg.f1 : sg.e;
+            sg.f0 = sg.f0 < sg.f1 ? sg.f0 : sg.f1;
+        }
+        run_blocks(g, sg.a, sg.f0, [&](uint32_t q, bool v, double qx, double qy) {
+            blockfn(q, v, qx, qy, false);
+        });
+        for (uint32_t q0 = sg.f0; q0 < sg.f1; q0 += kWave) {
+            uint32_t q = q0 + lane_id();
+            blockfn(q, q < sg.f1, 0.0, 0.0, true);
+        }
+        run_blocks(g, sg.f1, sg.e, [&](uint32_t q, bool v, double qx, double qy) {
+            blockfn(q, v, qx, qy, false);
+        });
+    }
 }
 
 // Root of x without path-halving stores. Used once every union is done:
