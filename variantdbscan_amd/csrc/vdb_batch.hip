@@ -795,6 +795,7 @@ struct LabelSlots {
     uint32_t *out;
     uint32_t nrows;
     const uint32_t *seg;                   // nsets+1 dataset starts
+    uint32_t *tmp;                         // kLabelSlots rows of n labels, position order
 };
 
 // label of a cell's core points at a snapshot: 1 + cluster id of its root,
@@ -859,8 +860,7 @@ __global__ void __launch_bounds__(kBlock)
         }
     }
     uint32_t ds = cell_ds[C];
-    uint32_t s0 = S.seg[ds], nd = S.seg[ds + 1] - s0;
-    uint32_t *base = S.out + (size_t)s0 * S.nrows + (i - s0);
+    uint32_t s0 = S.seg[ds];
 #pragma unroll
     for (int t = 0; t < kLabelSlots; t++) {
         if (t >= S.n) break;
@@ -871,8 +871,36 @@ __global__ void __launch_bounds__(kBlock)
             uint32_t b = best[t];
             lbl = (b != kNone && b < r) ? S.cid[t][b] - S.cid[t][s0] + 1 : 0u;
         }
-        base[(size_t)S.row[t] * nd] = lbl;
+        // position order: coalesced; k_rows_from_pos moves them to caller order
+        S.tmp[(size_t)t * n + p] = lbl;
     }
+    (void)i;
+}
+
+// Label rows from position order to caller order: thread per caller index,
+// so every row is written contiguously; the reads gather from one row of the
+// staging buffer at a time.
+__global__ void k_rows_from_pos(LabelSlots S, const uint32_t *pos_of_caller, uint32_t nsets,
+                                uint32_t n)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t p = pos_of_caller[i];
+    uint32_t ds = seg_of(S.seg, nsets, i);
+    uint32_t s0 = S.seg[ds], nd = S.seg[ds + 1] - s0;
+    uint32_t *base = S.out + (size_t)s0 * S.nrows + (i - s0);
+#pragma unroll
+    for (int t = 0; t < kLabelSlots; t++) {
+        if (t >= S.n) break;
+        base[(size_t)S.row[t] * nd] = S.tmp[(size_t)t * n + p];
+    }
+}
+
+__global__ void k_pos_of_caller(const uint32_t *caller_of_pos, uint32_t n, uint32_t *pos_of_caller)
+{
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    pos_of_caller[caller_of_pos[p]] = p;
 }
 
 __global__ void k_caller_of_pos(const uint32_t *pos_rank, const uint32_t *mapping, uint32_t n,
@@ -1123,6 +1151,11 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_TRY(c.get("b.cop", n, &caller_of_pos));
     k_caller_of_pos<<<gn, kBlock, 0, c.stream()>>>(G.pos_rank, P.mapping, n, caller_of_pos);
     VDB_HIP(hipGetLastError());
+    uint32_t *pos_of_caller, *labtmp;
+    VDB_TRY(c.get("b.poc", n, &pos_of_caller));
+    VDB_TRY(c.get("b.labtmp", (size_t)kLabelSlots * n, &labtmp));
+    k_pos_of_caller<<<gn, kBlock, 0, c.stream()>>>(caller_of_pos, n, pos_of_caller);
+    VDB_HIP(hipGetLastError());
     if (st) st->ms_prepare += tpairs.ms();
 
     // ---- variants of each (k, j)
@@ -1261,9 +1294,12 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             S.out = out;
             S.nrows = nrows;
             S.seg = P.seg;
+            S.tmp = labtmp;
             k_labels_j<<<gn, kBlock, 0, c.stream()>>>(S, caller_of_pos, G.cell_of_pos, G.cell_ds,
                                                       G.pos_rank, kc, J, j, lst_idx, lst_s,
                                                       lst_id, lst_n, (int)L, n);
+            VDB_HIP(hipGetLastError());
+            k_rows_from_pos<<<gn, kBlock, 0, c.stream()>>>(S, pos_of_caller, P.nsets, n);
             VDB_HIP(hipGetLastError());
         }
         ms_labels += tl.ms();
