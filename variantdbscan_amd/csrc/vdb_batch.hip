@@ -44,7 +44,7 @@ constexpr int kMaxMinpts = 64;
 // pairs whose cells hold at most this many point pairs are tested by one thread
 constexpr uint32_t kSmallPairWork = 16;
 // grid of the worklist kernels, whose sizes live on the device
-constexpr unsigned kPersistentBlocks = 1024;
+constexpr unsigned kPersistentBlocks = 512;
 
 struct FG {
     const double *xs, *ys, *box;
