@@ -43,8 +43,13 @@ constexpr int kMaxLevels = 64;
 constexpr int kMaxMinpts = 64;
 // pairs whose cells hold at most this many point pairs are tested by one thread
 constexpr uint32_t kSmallPairWork = 16;
-// grid of the worklist kernels, whose sizes live on the device
-constexpr unsigned kPersistentBlocks = 512;
+// grid of the worklist kernels, whose sizes live on the device: one block
+// per kPairsPerBlock pairs becoming eligible at the level, within
+// [kMinPersistent, kMaxPersistent] (more waves than the work needs only
+// contend on the union-find)
+constexpr unsigned kPairsPerBlock = 8192;
+constexpr unsigned kMinPersistent = 512;
+constexpr unsigned kMaxPersistent = 1024;
 
 struct FG {
     const double *xs, *ys, *box;
@@ -1224,14 +1229,16 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             uint32_t nfresh = npairs ? bst[k + 1] - bst[k] : 0;
             const PairRec *fresh = rec + (npairs ? bst[k] : 0);
             // worklist sizes stay on the device: bounded grids stride over them
-            k_sweep<<<kPersistentBlocks, kBlock, 0, c.stream()>>>(
+            unsigned pb = std::min(kMaxPersistent,
+                                   std::max(kMinPersistent, ceil_div(nfresh, kPairsPerBlock)));
+            k_sweep<<<pb, kBlock, 0, c.stream()>>>(
                 a, lin, cnts + in_idx, fresh, nfresh, parent, lout, cnts + out_idx, tests,
                 cnts + 2, tests2, cnts + 3);
             VDB_HIP(hipGetLastError());
-            k_test<<<kPersistentBlocks, kBlock, 0, c.stream()>>>(a, tests, cnts + 2, parent, lout,
+            k_test<<<pb, kBlock, 0, c.stream()>>>(a, tests, cnts + 2, parent, lout,
                                                                  cnts + out_idx);
             VDB_HIP(hipGetLastError());
-            k_test<<<kPersistentBlocks, kBlock, 0, c.stream()>>>(a, tests2, cnts + 3, parent,
+            k_test<<<pb, kBlock, 0, c.stream()>>>(a, tests2, cnts + 3, parent,
                                                                  lout, cnts + out_idx);
             VDB_HIP(hipGetLastError());
             if (verbose) {
