@@ -392,6 +392,41 @@ __global__ void __launch_bounds__(kBlock)
     // candidates accepted without a test: every origin's row gets the run of
     // ids as it stands (no distances, so only without D2)
     auto full = [&](uint32_t a, uint32_t e) {
+        if (STAGE && !idmap) {
+            // the ids are the positions a..e-1 themselves: top up the open
+            // segment of each row through the ring, store the whole aligned
+            // segments that follow straight from registers (no LDS round
+            // trip), and leave the rest in the ring
+#pragma unroll
+            for (int b = 0; b < kOrigins; b++) {
+                if (b >= (int)nb || a >= e) break;
+                uint32_t q = a, left = e - a;
+                uint32_t h = bnd[b] - cur[b]; // > 0 between appends
+                h = left < h ? left : h;
+                append(b, (uint32_t)lane_id() < h, h == (uint32_t)kWave ? ~0ull : (1ull << h) - 1,
+                       q + (uint32_t)lane_id());
+                q += h;
+                left -= h;
+                if (left >= (uint32_t)kWave) {
+                    // the top-up flushed the open segment: cur = fl and
+                    // run + fl is a multiple of 64
+                    uint32_t nseg = left / kWave;
+                    uint32_t *dst = ids + (run[b] + fl[b]);
+                    for (uint32_t s = 0; s < nseg; s++) {
+                        uint32_t t = fl[b] + s * kWave + (uint32_t)lane_id();
+                        if (t < rlen[b]) dst[s * kWave + lane_id()] = q + s * kWave + lane_id();
+                    }
+                    fl[b] += nseg * kWave;
+                    cur[b] = fl[b];
+                    bnd[b] = fl[b] + kWave;
+                    q += nseg * kWave;
+                    left -= nseg * kWave;
+                }
+                if (left)
+                    append(b, (uint32_t)lane_id() < left, (1ull << left) - 1, q + (uint32_t)lane_id());
+            }
+            return;
+        }
         for (uint32_t q0 = a; q0 < e; q0 += kWave * kChunk) {
             int nblk = (int)((e - q0 + kWave - 1) / kWave);
             nblk = nblk < kChunk ? nblk : kChunk;
