@@ -106,7 +106,7 @@ __device__ __forceinline__ void window_blocks(const GridView &g, uint32_t c, boo
 // Chunks of up to kChunk 64-candidate blocks of positions [a, e) held in
 // registers (lane l holds candidates q0 + i*64 + l), for consumers that finish
 // one origin over a whole chunk before the next.
-constexpr int kChunk = 8;
+constexpr int kChunk = 2;
 template <class F>
 __device__ __forceinline__ void run_chunks(const GridView &g, uint32_t a, uint32_t e, F &&f)
 {
@@ -285,11 +285,25 @@ __global__ void __launch_bounds__(kBlock)
     uint64_t run[kOrigins], lim[kOrigins];
     uint32_t rowid[kOrigins];
     double rx = sgpr_d(g.xs[o0]), ry = sgpr_d(g.ys[o0]);
+    // FP64 tests read the origins for every candidate: they live in vector
+    // registers (broadcast from one lane-parallel load), since the scalar
+    // file cannot hold them beside the rows' write state without spilling
+    double xl = 0.0, yl = 0.0;
+    if (PREC == 0) {
+        uint32_t ol = o0 + ((uint32_t)lane_id() < nb ? (uint32_t)lane_id() : 0u);
+        xl = g.xs[ol];
+        yl = g.ys[ol];
+    }
 #pragma unroll
     for (int b = 0; b < kOrigins; b++) {
         uint32_t ob = o0 + (b < (int)nb ? b : 0);
-        ox[b] = b < (int)nb ? sgpr_d(g.xs[ob]) : qnan;
-        oy[b] = sgpr_d(g.ys[ob]);
+        if (PREC == 0) {
+            ox[b] = b < (int)nb ? __shfl(xl, b) : qnan;
+            oy[b] = __shfl(yl, b);
+        } else {
+            ox[b] = b < (int)nb ? sgpr_d(g.xs[ob]) : qnan;
+            oy[b] = sgpr_d(g.ys[ob]);
+        }
         fx[b] = (float)__dsub_rn(ox[b], rx);
         fy[b] = (float)__dsub_rn(oy[b], ry);
         uint32_t row = rowmap ? rowmap[ob] : ob;
@@ -329,10 +343,15 @@ __global__ void __launch_bounds__(kBlock)
         // the ring entries were written by other lanes of this wave; LDS
         // operations of a wave complete in order
         __builtin_amdgcn_wave_barrier();
-        uint64_t g0 = (run[b] + fl[b]) & ~(uint64_t)(kWave - 1);
-        int64_t t = (int64_t)(g0 + lane_id()) - (int64_t)run[b];
-        bool w = t >= (int64_t)fl[b] && t < (int64_t)upto && t < (int64_t)rlen[b];
-        if (w) ids[g0 + lane_id()] = ring[b * kRing + ((uint32_t)t & (kRing - 1))];
+        // entry fl sits in lane lo of its aligned segment; lanes [lo, hi)
+        // take entries fl .. min(upto, rlen) - 1 (hi <= 64: upto <= bnd)
+        uint64_t at = run[b] + fl[b];
+        uint32_t lo = (uint32_t)at & (kWave - 1);
+        uint32_t end = upto < rlen[b] ? upto : rlen[b];
+        uint32_t hi = end > fl[b] ? lo + (end - fl[b]) : lo;
+        uint32_t l = (uint32_t)lane_id();
+        if (l >= lo && l < hi)
+            ids[(at - lo) + l] = ring[b * kRing + ((fl[b] - lo + l) & (kRing - 1))];
         fl[b] = upto;
         bnd[b] = upto + kWave;
     };
