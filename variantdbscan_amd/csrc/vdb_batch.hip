@@ -85,6 +85,26 @@ __device__ uint32_t cf_find(uint32_t *parent, uint32_t x)
     }
 }
 
+// same component? Parent pointers stay inside a component, so equal parents
+// settle it after one pair of loads issued together (the common case once a
+// snapshot has flattened the forest); otherwise the roots decide.
+__device__ __forceinline__ bool cf_same(uint32_t *parent, uint32_t a, uint32_t b)
+{
+    uint32_t pa = ld_u32(parent + a), pb = ld_u32(parent + b);
+    return pa == pb || cf_find(parent, pa) == cf_find(parent, pb);
+}
+
+// cf_same that also returns the roots when the components differ
+__device__ __forceinline__ bool cf_roots(uint32_t *parent, uint32_t a, uint32_t b,
+                                         uint32_t &ra, uint32_t &rb)
+{
+    uint32_t pa = ld_u32(parent + a), pb = ld_u32(parent + b);
+    if (pa == pb) return true;
+    ra = cf_find(parent, pa);
+    rb = cf_find(parent, pb);
+    return ra == rb;
+}
+
 __device__ void cf_unite(uint32_t *parent, uint32_t a, uint32_t b)
 {
     while (true) {
@@ -98,6 +118,12 @@ __device__ void cf_unite(uint32_t *parent, uint32_t a, uint32_t b)
         }
         if (atomicCAS(parent + b, b, a) == b) return;
     }
+}
+
+// link root x under m < x, or unite the two if x stopped being a root
+__device__ __forceinline__ void cf_hook(uint32_t *parent, uint32_t x, uint32_t m)
+{
+    if (atomicCAS(parent + x, x, m) != x) cf_unite(parent, x, m);
 }
 
 // ------------------------------------------------------------- cliques
@@ -363,7 +389,10 @@ __global__ void __launch_bounds__(kBlock)
     if (lane == 0) lst_n[i] = (uint32_t)cnt;
 }
 
-// kc[p*J+j] = first level k with count_k(p) >= m_j (K = never)
+// kc[p*J+j] = first level k with count_k(p) >= m_j (K = never). A listed
+// point's list holds exact binary64 s in ascending order, so count_k >= m
+// exactly when entry m-1 exists and has s <= T_k; an unlisted point has L
+// (>= every m_j) entries within eps_0.
 __global__ void k_core_levels(const uint32_t *lst_idx, const double *lst_s,
                               const uint32_t *lst_n, int L, const double *Tk, int K,
                               const uint32_t *mj, int J, uint32_t n, uint8_t *kc)
@@ -371,27 +400,21 @@ __global__ void k_core_levels(const uint32_t *lst_idx, const double *lst_s,
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= n) return;
     uint32_t i = lst_idx[p];
-    int ck[kMaxLevels];
-    for (int k = 0; k < K; k++) ck[k] = L;
-    if (i != kNone) {
-        int m = (int)lst_n[i];
-        const double *s = lst_s + (size_t)i * L;
-        // count_k = entries with s <= T_k (lists hold exact binary64 s, sorted)
-        int e = 0;
-        for (int k = 0; k < K; k++) {
-            while (e < m && s[e] <= Tk[k]) e++;
-            ck[k] = e;
-        }
-    }
+    uint32_t m = i != kNone ? lst_n[i] : 0;
+    const double *s = lst_s + (size_t)(i != kNone ? i : 0) * L;
     for (int j = 0; j < J; j++) {
         uint32_t need = mj[j];
         int lev = K;
         if (need <= 0x7fffffffu) {
-            for (int k = 0; k < K; k++)
-                if ((uint32_t)ck[k] >= need) {
-                    lev = k;
-                    break;
-                }
+            if (i == kNone) {
+                lev = need <= (uint32_t)L ? 0 : K;
+            } else if (need == 0) {
+                lev = 0;
+            } else if (need <= m) {
+                double sv = s[need - 1];
+                lev = 0;
+                while (lev < K && sv > Tk[lev]) lev++;
+            }
         }
         kc[(size_t)p * J + j] = (uint8_t)lev;
     }
@@ -643,16 +666,17 @@ __global__ void __launch_bounds__(kBlock)
         bool valid = i < total;
         PairRec pr{0, 0, 0};
         if (valid) pr = i < nc ? carried[i] : fresh[i - nc];
-        bool kp = false, tnear = false, tfar = false;
+        bool kp = false, tnear = false, tfar = false, uni = false;
+        uint32_t ra = 0, rb = 0;
         if (valid) {
             uint32_t A = pr.a, B = pr.b;
             int kfull = (int)((pr.k >> 8) & 0xff);
             bool exact = (pr.k >> 16) & 1u;
-            if (cf_find(parent, A) == cf_find(parent, B)) {
+            if (cf_roots(parent, A, B, ra, rb)) {
                 // connected now, hence at every later level
             } else if (exact || kfull <= a.k) {
                 // exact pairs arrive at the level they connect
-                cf_unite(parent, A, B);
+                uni = true;
             } else {
                 uint32_t na = a.cell_start[A + 1] - a.cell_start[A];
                 uint32_t nb = a.cell_start[B + 1] - a.cell_start[B];
@@ -671,6 +695,28 @@ __global__ void __launch_bounds__(kBlock)
                 }
             }
         }
+        // Lanes sharing the A-side root (consecutive pairs of one cell) hook
+        // every root of their group straight under the group's smallest
+        // root, instead of racing to relink the shared root one pair at a
+        // time. A hook that loses its race falls back to a full union.
+        uint32_t m_l = kNone;
+        bool lead = false;
+        uint64_t todo = ballot(uni);
+        while (todo) {
+            int leader = __builtin_ctzll(todo);
+            uint32_t r = __shfl(ra, leader);
+            bool mine = uni && ra == r;
+            todo &= ~ballot(mine);
+            uint32_t m = mine ? (rb < ra ? rb : ra) : kNone;
+            for (int o = kWave / 2; o > 0; o >>= 1) {
+                uint32_t t = __shfl_xor(m, o);
+                m = t < m ? t : m;
+            }
+            if (mine) m_l = m;
+            if (lane_id() == leader) lead = true;
+        }
+        if (uni && rb != m_l) cf_hook(parent, rb, m_l);
+        if (lead && ra != m_l) cf_hook(parent, ra, m_l);
         wave_append_rec(kp, pr, keep, nkeep);
         wave_append_rec(tnear, pr, near, nnear);
         wave_append_rec(tfar, pr, far, nfar);
@@ -689,7 +735,7 @@ __device__ void test_pair(const LevelArgs &a, const PairRec &pr, uint32_t *paren
         A = B;
         B = t;
     }
-    bool found = cf_find(parent, A) == cf_find(parent, B);
+    bool found = cf_same(parent, A, B);
     if (!found) {
         uint32_t a0 = a.cell_start[A], a1 = a.cell_start[A + 1];
         uint32_t b0 = a.cell_start[B], b1 = a.cell_start[B + 1];
