@@ -423,6 +423,8 @@ __global__ void k_core_levels(const uint32_t *lst_idx, const double *lst_s,
 // per cell, per j, per k: smallest rank of a point with kc_j <= k (cells list
 // their points in ascending rank, so it is the first such point), and the
 // first level at which the cell holds a core point. Lane k tracks level k.
+// Both are stored level-major (cmr[(j*K + k)*ncell + c], cellkc[j*ncell + c])
+// so that the per-level passes over all cells read them contiguously.
 __global__ void __launch_bounds__(kBlock)
     k_cell_core(const uint32_t *cell_start, uint32_t ncell, const uint8_t *kc, int J, int K,
                 const uint32_t *pos_rank, uint32_t *cmr, uint8_t *cellkc)
@@ -445,9 +447,9 @@ __global__ void __launch_bounds__(kBlock)
             all = __shfl(first, 0) != kNone;
         }
         if (lane < K)
-            cmr[((size_t)c * J + j) * K + lane] = first == kNone ? kNone : pos_rank[first];
+            cmr[((size_t)j * K + lane) * ncell + c] = first == kNone ? kNone : pos_rank[first];
         uint64_t have = ballot(lane < K && first != kNone);
-        if (lane == 0) cellkc[(size_t)c * J + j] = (uint8_t)(have ? __builtin_ctzll(have) : K);
+        if (lane == 0) cellkc[(size_t)j * ncell + c] = (uint8_t)(have ? __builtin_ctzll(have) : K);
     }
 }
 
@@ -567,7 +569,7 @@ __device__ bool small_pair_connected(const LevelArgs &a, uint32_t A, uint32_t B)
 //   min over (p, q) of max(level(s(p,q)), kc_j(p), kc_j(q)),
 // and the value carries bit 31 so the sweep unites them without a test.
 __global__ void k_pair_elig(const uint32_t *pa, const uint32_t *pb, const uint16_t *pk,
-                            const uint8_t *cellkc, const uint32_t *cell_start, const double *xs,
+                            const uint8_t *ckj, const uint32_t *cell_start, const double *xs,
                             const double *ys, const uint8_t *kc, const double *Tk, int J, int j,
                             int K, uint32_t npairs, uint32_t *key, uint32_t *val)
 {
@@ -575,7 +577,7 @@ __global__ void k_pair_elig(const uint32_t *pa, const uint32_t *pb, const uint16
     if (i >= npairs) return;
     uint32_t A = pa[i], B = pb[i];
     int e = pk[i] & 0xff;
-    int ca = cellkc[(size_t)A * J + j], cb = cellkc[(size_t)B * J + j];
+    int ca = ckj[A], cb = ckj[B];
     e = e > ca ? e : ca;
     e = e > cb ? e : cb;
     uint32_t flag = 0;
@@ -788,13 +790,13 @@ __global__ void k_cell_roots(uint32_t *parent, uint32_t ncell)
 // compmin[root] = smallest core rank of the component. Lanes of a wave that
 // share a root (the common case inside a large cluster) reduce first, so a
 // giant component takes one atomic per wave rather than one per cell.
-__global__ void k_comp_min(const uint32_t *parent, const uint32_t *cmr, const uint8_t *cellkc,
-                           uint32_t ncell, int J, int j, int K, int k, uint32_t *compmin)
+__global__ void k_comp_min(const uint32_t *parent, const uint32_t *cmrjk, const uint8_t *ckj,
+                           uint32_t ncell, int k, uint32_t *compmin)
 {
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-    bool valid = c < ncell && cellkc[(size_t)c * J + j] <= k;
+    bool valid = c < ncell && ckj[c] <= k;
     uint32_t root = valid ? parent[c] : kNone;
-    uint32_t v = valid ? cmr[((size_t)c * J + j) * K + k] : kNone;
+    uint32_t v = valid ? cmrjk[c] : kNone;
     uint64_t todo = ballot(valid);
     while (todo) {
         int leader = __builtin_ctzll(todo);
@@ -811,23 +813,21 @@ __global__ void k_comp_min(const uint32_t *parent, const uint32_t *cmr, const ui
 }
 
 __global__ void k_root_flags_cells(const uint32_t *parent, const uint32_t *compmin,
-                                   const uint8_t *cellkc, uint32_t ncell, int J, int j, int k,
-                                   uint32_t *flag)
+                                   const uint8_t *ckj, uint32_t ncell, int k, uint32_t *flag)
 {
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= ncell) return;
-    if (cellkc[(size_t)c * J + j] > k || parent[c] != c) return;
+    if (ckj[c] > k || parent[c] != c) return;
     flag[compmin[c]] = 1u;
 }
 
 // root rank of every cell at level k (NONE if the cell holds no core point)
 __global__ void k_snapshot_roots(const uint32_t *parent, const uint32_t *compmin,
-                                 const uint8_t *cellkc, uint32_t ncell, int J, int j, int k,
-                                 uint32_t *rootrank)
+                                 const uint8_t *ckj, uint32_t ncell, int k, uint32_t *rootrank)
 {
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= ncell) return;
-    rootrank[c] = cellkc[(size_t)c * J + j] <= k ? compmin[parent[c]] : kNone;
+    rootrank[c] = ckj[c] <= k ? compmin[parent[c]] : kNone;
 }
 
 // Labels of up to kLabelSlots variants sharing minpts j, from their level
@@ -894,19 +894,37 @@ __global__ void __launch_bounds__(kBlock)
             uint32_t m = lst_n[li];
             const double *sv = lst_s + (size_t)li * L;
             const uint32_t *iv = lst_id + (size_t)li * L;
-            for (uint32_t e = 0; e < m; e++) {
-                double se = sv[e];
-                if (se > Tmax) break;
-                uint32_t q = iv[e];
-                int kq = kc[(size_t)q * J + j];
-                uint32_t cq = cell_of_pos[q];
+            // four entries per step, their loads issued together (the list
+            // is sorted, so the walk ends at the first entry beyond Tmax)
+            constexpr int kU = 4;
+            for (uint32_t e0 = 0; e0 < m; e0 += kU) {
+                double se[kU];
+                uint32_t q[kU];
 #pragma unroll
-                for (int t = 0; t < kLabelSlots; t++) {
-                    if (t < S.n && kcp > S.k[t] && se <= S.T[t] && kq <= S.k[t]) {
-                        uint32_t rr = S.rootrank[t][cq];
-                        best[t] = rr < best[t] ? rr : best[t];
+                for (int u = 0; u < kU; u++) {
+                    bool ok = e0 + u < m;
+                    se[u] = ok ? sv[e0 + u] : INFINITY;
+                    q[u] = ok ? iv[e0 + u] : 0u;
+                }
+                int kq[kU];
+                uint32_t cq[kU];
+#pragma unroll
+                for (int u = 0; u < kU; u++) {
+                    bool ok = se[u] <= Tmax;
+                    kq[u] = ok ? (int)kc[(size_t)q[u] * J + j] : kMaxLevels;
+                    cq[u] = ok ? cell_of_pos[q[u]] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < kU; u++) {
+#pragma unroll
+                    for (int t = 0; t < kLabelSlots; t++) {
+                        if (t < S.n && kcp > S.k[t] && se[u] <= S.T[t] && kq[u] <= S.k[t]) {
+                            uint32_t rr = S.rootrank[t][cq[u]];
+                            best[t] = rr < best[t] ? rr : best[t];
+                        }
                     }
                 }
+                if (!(se[kU - 1] <= Tmax)) break;
             }
         }
     }
@@ -1239,7 +1257,8 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         std::vector<uint32_t> bst(K + 2, 0);
         if (npairs) {
             k_pair_elig<<<ceil_div(npairs, kBlock), kBlock, 0, c.stream()>>>(
-                pa, pb, pk, cellkc, G.cell_start, G.xs, G.ys, kc, dTk, J, j, K, npairs, ekey,
+                pa, pb, pk, cellkc + (size_t)j * G.ncell, G.cell_start, G.xs, G.ys, kc, dTk, J, j,
+                K, npairs, ekey,
                 eval);
             VDB_HIP(hipGetLastError());
             uint32_t *sk, *sv;
@@ -1308,17 +1327,18 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             VDB_HIP(hipGetLastError());
             k_fill_u32<<<gc, kBlock, 0, c.stream()>>>(compmin, kNone, G.ncell);
             VDB_HIP(hipGetLastError());
-            k_comp_min<<<gc, kBlock, 0, c.stream()>>>(parent, cmr, cellkc, G.ncell, J, j, K, k,
-                                                      compmin);
+            const uint8_t *ckj = cellkc + (size_t)j * G.ncell;
+            k_comp_min<<<gc, kBlock, 0, c.stream()>>>(
+                parent, cmr + ((size_t)j * K + k) * G.ncell, ckj, G.ncell, k, compmin);
             VDB_HIP(hipGetLastError());
             k_fill_u32<<<gn, kBlock, 0, c.stream()>>>(rflag, 0u, n);
             VDB_HIP(hipGetLastError());
-            k_root_flags_cells<<<gc, kBlock, 0, c.stream()>>>(parent, compmin, cellkc, G.ncell,
-                                                              J, j, k, rflag);
+            k_root_flags_cells<<<gc, kBlock, 0, c.stream()>>>(parent, compmin, ckj, G.ncell, k,
+                                                              rflag);
             VDB_HIP(hipGetLastError());
             VDB_TRY(scan_exclusive_u32(c, rflag, scid, n, nullptr));
-            k_snapshot_roots<<<gc, kBlock, 0, c.stream()>>>(parent, compmin, cellkc, G.ncell, J,
-                                                            j, k, sroot);
+            k_snapshot_roots<<<gc, kBlock, 0, c.stream()>>>(parent, compmin, ckj, G.ncell, k,
+                                                            sroot);
             VDB_HIP(hipGetLastError());
             k_cell_labels<<<gc, kBlock, 0, c.stream()>>>(sroot, scid, G.cell_ds, P.seg,
                                                          G.ncell,
