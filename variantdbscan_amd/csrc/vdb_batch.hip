@@ -509,10 +509,11 @@ __global__ void k_iota(uint32_t *a, uint32_t n)
     if (i < n) a[i] = i;
 }
 
-__global__ void k_init_parent(uint32_t *parent, uint32_t n)
+__global__ void k_init_parent(uint32_t *parent, uint32_t n, uint32_t *cnt, uint32_t ncnt)
 {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) parent[i] = i;
+    if (i < ncnt) cnt[i] = 0;
 }
 
 // Append v to list[] for every lane with take set: one atomic per wave.
@@ -638,18 +639,6 @@ __device__ __forceinline__ void wave_append_rec(bool take, const PairRec &v, Pai
     if (take) list[base + mask_rank(m)] = v;
 }
 
-// counters: cnt[0], cnt[1] sizes of the two carried lists, cnt[2] near
-// tests, cnt[3] far tests
-__global__ void k_level_reset(uint32_t *cnt, int out_idx, int clear_in, int in_idx)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        cnt[out_idx] = 0;
-        cnt[2] = 0;
-        cnt[3] = 0;
-        if (clear_in) cnt[in_idx] = 0;
-    }
-}
-
 // Grid-stride over the carried pairs (count on the device) followed by the
 // pairs that become eligible at this level. Each eligible pair is settled
 // (already connected; fully inside eps; small and tested here), kept for the
@@ -765,20 +754,24 @@ __device__ void test_pair(const LevelArgs &a, const PairRec &pr, uint32_t *paren
     }
 }
 
+// the near tests, then the far ones, in one grid-stride pass (waves take
+// them in order, so most far tests start after the near ones)
 __global__ void __launch_bounds__(kBlock)
-    k_test(LevelArgs a, const PairRec *tests, const uint32_t *ntests, uint32_t *parent,
-           PairRec *list_out, uint32_t *nout)
+    k_test(LevelArgs a, const PairRec *near, const uint32_t *nnear, const PairRec *far,
+           const uint32_t *nfar, uint32_t *parent, PairRec *list_out, uint32_t *nout)
 {
-    uint32_t nt = *ntests;
+    uint32_t nn = *nnear, nt = nn + *nfar;
     for (uint32_t w = blockIdx.x * kWPB + wave_id_in_block(); w < nt; w += gridDim.x * kWPB)
-        test_pair(a, tests[w], parent, list_out, nout);
+        test_pair(a, w < nn ? near[w] : far[w - nn], parent, list_out, nout);
 }
 
 // ---------------------------------------------------------- snapshots
-__global__ void k_cell_roots(uint32_t *parent, uint32_t ncell)
+// flattens the forest (every cell points at its root) and clears compmin
+__global__ void k_cell_roots(uint32_t *parent, uint32_t ncell, uint32_t *compmin)
 {
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= ncell) return;
+    compmin[c] = kNone;
     uint32_t x = c, p = ld_u32(parent + x);
     while (p != x) { // read-only walk; only true roots are stored
         x = p;
@@ -789,7 +782,9 @@ __global__ void k_cell_roots(uint32_t *parent, uint32_t ncell)
 
 // compmin[root] = smallest core rank of the component. Lanes of a wave that
 // share a root (the common case inside a large cluster) reduce first, so a
-// giant component takes one atomic per wave rather than one per cell.
+// giant component takes one atomic per wave rather than one per cell; after
+// the two most common roots of the wave the remaining lanes, spread over
+// small components, update theirs one atomic each.
 __global__ void k_comp_min(const uint32_t *parent, const uint32_t *cmrjk, const uint8_t *ckj,
                            uint32_t ncell, int k, uint32_t *compmin)
 {
@@ -798,7 +793,7 @@ __global__ void k_comp_min(const uint32_t *parent, const uint32_t *cmrjk, const 
     uint32_t root = valid ? parent[c] : kNone;
     uint32_t v = valid ? cmrjk[c] : kNone;
     uint64_t todo = ballot(valid);
-    while (todo) {
+    for (int it = 0; it < 2 && todo; it++) {
         int leader = __builtin_ctzll(todo);
         uint32_t r = __shfl(root, leader);
         bool same = valid && root == r;
@@ -810,6 +805,7 @@ __global__ void k_comp_min(const uint32_t *parent, const uint32_t *cmrjk, const 
         }
         if (lane_id() == leader && m < ld_u32(compmin + r)) atomicMin(compmin + r, m);
     }
+    if (((todo >> lane_id()) & 1ull) && v < ld_u32(compmin + root)) atomicMin(compmin + root, v);
 }
 
 __global__ void k_root_flags_cells(const uint32_t *parent, const uint32_t *compmin,
@@ -823,11 +819,17 @@ __global__ void k_root_flags_cells(const uint32_t *parent, const uint32_t *compm
 
 // root rank of every cell at level k (NONE if the cell holds no core point)
 __global__ void k_snapshot_roots(const uint32_t *parent, const uint32_t *compmin,
-                                 const uint8_t *ckj, uint32_t ncell, int k, uint32_t *rootrank)
+                                 const uint8_t *ckj, uint32_t ncell, int k, uint32_t *rootrank,
+                                 uint32_t *flag)
 {
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= ncell) return;
-    rootrank[c] = ckj[c] <= k ? compmin[parent[c]] : kNone;
+    bool core = ckj[c] <= k;
+    uint32_t r = parent[c];
+    rootrank[c] = core ? compmin[r] : kNone;
+    // the cluster-id scan has read the flags k_root_flags_cells set: clear
+    // them for the next snapshot
+    if (core && r == c) flag[compmin[c]] = 0u;
 }
 
 // Labels of up to kLabelSlots variants sharing minpts j, from their level
@@ -1206,7 +1208,11 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_TRY(c.get("b.lb", psz, &lb));
     VDB_TRY(c.get("b.tests", psz, &tests));
     VDB_TRY(c.get("b.tests2", psz, &tests2));
-    VDB_TRY(c.get("b.cnts", 4, &cnts));
+    // per level: carried-in list size, near and far test counts (level k
+    // reads carried[k], writes carried[k+1])
+    const uint32_t ncnt = 3 * ((uint32_t)K + 1);
+    VDB_TRY(c.get("b.cnts", ncnt, &cnts));
+    uint32_t *c_keep = cnts, *c_near = cnts + K + 1, *c_far = cnts + 2 * (K + 1);
     uint32_t *ekey, *eval, *stk, *stv, *bstart;
     VDB_TRY(c.get("b.ekey", psz, &ekey));
     VDB_TRY(c.get("b.eval", psz, &eval));
@@ -1239,6 +1245,9 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_TRY(c.get("b.parent", G.ncell, &parent));
     VDB_TRY(c.get("b.compmin", G.ncell, &compmin));
     VDB_TRY(c.get("b.rflag", n, &rflag));
+    // root flags start clear; every snapshot clears the ones it set
+    k_fill_u32<<<gn, kBlock, 0, c.stream()>>>(rflag, 0u, n);
+    VDB_HIP(hipGetLastError());
     // per-level snapshots of one j: cell root ranks and cluster ids by rank
     uint32_t *snap_root, *snap_cid, *snap_lab;
     VDB_TRY(c.get("b.snap_root", (size_t)K * G.ncell, &snap_root));
@@ -1251,7 +1260,8 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         for (auto &w : want)
             if (w.first.second == j) kmax = std::max(kmax, w.first.first);
         if (kmax < 0) continue;
-        k_init_parent<<<gc, kBlock, 0, c.stream()>>>(parent, G.ncell);
+        k_init_parent<<<std::max(gc, ceil_div(ncnt, kBlock)), kBlock, 0, c.stream()>>>(
+            parent, G.ncell, cnts, ncnt);
         VDB_HIP(hipGetLastError());
         // pair records bucketed by the level at which they become eligible
         std::vector<uint32_t> bst(K + 2, 0);
@@ -1281,41 +1291,36 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
                                                                                nelig, rec);
             VDB_HIP(hipGetLastError());
         }
-        int in_idx = 0;
         PairRec *lin = la, *lout = lb;
         std::vector<int> snap_k;
         for (int k = 0; k <= kmax; k++) {
             LevelArgs a{G.xs,  G.ys,  G.cell_box, G.cell_start, G.cell_key, G.cell_ds,
                         G.skbase, G.sncy, kc,     cellkc,       pa,         pb,
                         pk,    J,     j,          k,            Th[k]};
-            int out_idx = 1 - in_idx;
-            k_level_reset<<<1, 64, 0, c.stream()>>>(cnts, out_idx, k == 0, in_idx);
-            VDB_HIP(hipGetLastError());
             uint32_t nfresh = npairs ? bst[k + 1] - bst[k] : 0;
             const PairRec *fresh = rec + (npairs ? bst[k] : 0);
             // worklist sizes stay on the device: bounded grids stride over them
             unsigned pb = std::min(kMaxPersistent,
                                    std::max(kMinPersistent, ceil_div(nfresh, kPairsPerBlock)));
-            k_sweep<<<pb, kBlock, 0, c.stream()>>>(
-                a, lin, cnts + in_idx, fresh, nfresh, parent, lout, cnts + out_idx, tests,
-                cnts + 2, tests2, cnts + 3);
+            k_sweep<<<pb, kBlock, 0, c.stream()>>>(a, lin, c_keep + k, fresh, nfresh, parent, lout,
+                                                   c_keep + k + 1, tests, c_near + k, tests2,
+                                                   c_far + k);
             VDB_HIP(hipGetLastError());
-            k_test<<<pb, kBlock, 0, c.stream()>>>(a, tests, cnts + 2, parent, lout,
-                                                                 cnts + out_idx);
-            VDB_HIP(hipGetLastError());
-            k_test<<<pb, kBlock, 0, c.stream()>>>(a, tests2, cnts + 3, parent,
-                                                                 lout, cnts + out_idx);
+            k_test<<<pb, kBlock, 0, c.stream()>>>(a, tests, c_near + k, tests2, c_far + k, parent,
+                                                  lout, c_keep + k + 1);
             VDB_HIP(hipGetLastError());
             if (verbose) {
                 // worklist sizes of this level (a sync per level: verbose only)
-                uint32_t hc[4];
-                VDB_HIP(hipMemcpyAsync(hc, cnts, sizeof hc, hipMemcpyDeviceToHost, c.stream()));
+                uint32_t hc[3];
+                VDB_HIP(hipMemcpyAsync(hc, c_near + k, 4, hipMemcpyDeviceToHost, c.stream()));
+                VDB_HIP(hipMemcpyAsync(hc + 1, c_far + k, 4, hipMemcpyDeviceToHost, c.stream()));
+                VDB_HIP(hipMemcpyAsync(hc + 2, c_keep + k + 1, 4, hipMemcpyDeviceToHost,
+                                       c.stream()));
                 VDB_HIP(hipStreamSynchronize(c.stream()));
                 printf("\n[vdbscan] batch j=%d k=%d: %u fresh, %u near + %u far tests, %u carried on",
-                       j, k, nfresh, hc[2], hc[3], hc[out_idx]);
+                       j, k, nfresh, hc[0], hc[1], hc[2]);
             }
             std::swap(lin, lout);
-            in_idx = out_idx;
             if (want.find({k, j}) == want.end()) continue;
             // snapshot of level k: component roots, their smallest core rank,
             // cluster ids in ascending root rank
@@ -1323,22 +1328,18 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             snap_k.push_back(k);
             uint32_t *sroot = snap_root + (size_t)slot * G.ncell;
             uint32_t *scid = snap_cid + (size_t)slot * n;
-            k_cell_roots<<<gc, kBlock, 0, c.stream()>>>(parent, G.ncell);
-            VDB_HIP(hipGetLastError());
-            k_fill_u32<<<gc, kBlock, 0, c.stream()>>>(compmin, kNone, G.ncell);
+            k_cell_roots<<<gc, kBlock, 0, c.stream()>>>(parent, G.ncell, compmin);
             VDB_HIP(hipGetLastError());
             const uint8_t *ckj = cellkc + (size_t)j * G.ncell;
             k_comp_min<<<gc, kBlock, 0, c.stream()>>>(
                 parent, cmr + ((size_t)j * K + k) * G.ncell, ckj, G.ncell, k, compmin);
-            VDB_HIP(hipGetLastError());
-            k_fill_u32<<<gn, kBlock, 0, c.stream()>>>(rflag, 0u, n);
             VDB_HIP(hipGetLastError());
             k_root_flags_cells<<<gc, kBlock, 0, c.stream()>>>(parent, compmin, ckj, G.ncell, k,
                                                               rflag);
             VDB_HIP(hipGetLastError());
             VDB_TRY(scan_exclusive_u32(c, rflag, scid, n, nullptr));
             k_snapshot_roots<<<gc, kBlock, 0, c.stream()>>>(parent, compmin, ckj, G.ncell, k,
-                                                            sroot);
+                                                            sroot, rflag);
             VDB_HIP(hipGetLastError());
             k_cell_labels<<<gc, kBlock, 0, c.stream()>>>(sroot, scid, G.cell_ds, P.seg,
                                                          G.ncell,
