@@ -562,48 +562,76 @@ __device__ bool small_pair_connected(const LevelArgs &a, uint32_t A, uint32_t B)
     return false;
 }
 
-// Eligibility level of every pair for minpts j: the first level at which
-// both cells hold a core point and some pair of their boxes is within eps.
-// K marks pairs that never become eligible.
+// Eligibility level of every pair for the minpts values j0 .. j0+nj-1
+// (nj <= kEligGroup): the first level at which both cells hold a core point
+// and some pair of their boxes is within eps. K marks pairs that never
+// become eligible. Output g of the group goes to key/val + g*npairs.
 // For pairs of small cells (at most kSmallPairWork point pairs) the level is
 // exact instead: the first level at which some core-core pair is within eps,
 //   min over (p, q) of max(level(s(p,q)), kc_j(p), kc_j(q)),
-// and the value carries bit 31 so the sweep unites them without a test.
+// and the value carries bit 31 so the sweep unites them without a test. The
+// point distances are computed once for the whole group.
+constexpr int kEligGroup = 8;
 __global__ void k_pair_elig(const uint32_t *pa, const uint32_t *pb, const uint16_t *pk,
-                            const uint8_t *ckj, const uint32_t *cell_start, const double *xs,
-                            const double *ys, const uint8_t *kc, const double *Tk, int J, int j,
-                            int K, uint32_t npairs, uint32_t *key, uint32_t *val)
+                            const uint8_t *cellkc, uint32_t ncell, const uint32_t *cell_start,
+                            const double *xs, const double *ys, const uint8_t *kc,
+                            const double *Tk, int J, int j0, int nj, int K, uint32_t npairs,
+                            uint32_t *key, uint32_t *val)
 {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= npairs) return;
     uint32_t A = pa[i], B = pb[i];
-    int e = pk[i] & 0xff;
-    int ca = ckj[A], cb = ckj[B];
-    e = e > ca ? e : ca;
-    e = e > cb ? e : cb;
-    uint32_t flag = 0;
+    int e0 = pk[i] & 0xff;
     uint32_t a0 = cell_start[A], a1 = cell_start[A + 1];
     uint32_t b0 = cell_start[B], b1 = cell_start[B + 1];
-    if (e < K && (uint64_t)(a1 - a0) * (b1 - b0) <= kSmallPairWork) {
-        int best = K;
+    int e[kEligGroup];
+    uint32_t exact = 0; // bit g: level of j0+g computed exactly
+    bool small = (uint64_t)(a1 - a0) * (b1 - b0) <= kSmallPairWork;
+#pragma unroll
+    for (int g = 0; g < kEligGroup; g++) {
+        e[g] = K;
+        if (g < nj) {
+            const uint8_t *ckj = cellkc + (size_t)(j0 + g) * ncell;
+            int v = e0, ca = ckj[A], cb = ckj[B];
+            v = v > ca ? v : ca;
+            v = v > cb ? v : cb;
+            e[g] = v;
+            if (small && v < K) exact |= 1u << g;
+        }
+    }
+    if (exact) {
+        int best[kEligGroup];
+#pragma unroll
+        for (int g = 0; g < kEligGroup; g++) best[g] = K;
         for (uint32_t p = a0; p < a1; p++) {
-            int kp = kc[(size_t)p * J + j];
-            if (kp >= best) continue;
             double px = xs[p], py = ys[p];
+            const uint8_t *kp = kc + (size_t)p * J + j0;
             for (uint32_t q = b0; q < b1; q++) {
-                int lv = kc[(size_t)q * J + j];
-                lv = lv > kp ? lv : kp;
-                if (lv >= best) continue;
                 double sq = dist2_exact(px, py, xs[q], ys[q]);
-                while (lv < best && sq > Tk[lv]) lv++;
-                if (lv < best) best = lv;
+                int ls = 0; // first level whose eps admits the pair
+                while (ls < K && sq > Tk[ls]) ls++;
+                if (ls >= K) continue;
+                const uint8_t *kq = kc + (size_t)q * J + j0;
+#pragma unroll
+                for (int g = 0; g < kEligGroup; g++) {
+                    if (!((exact >> g) & 1u)) continue;
+                    int lv = ls, x = kp[g], y = kq[g];
+                    lv = lv > x ? lv : x;
+                    lv = lv > y ? lv : y;
+                    best[g] = lv < best[g] ? lv : best[g];
+                }
             }
         }
-        e = best;
-        flag = 0x80000000u;
+#pragma unroll
+        for (int g = 0; g < kEligGroup; g++)
+            if ((exact >> g) & 1u) e[g] = best[g];
     }
-    key[i] = (uint32_t)(e < K ? e : K);
-    val[i] = i | flag;
+#pragma unroll
+    for (int g = 0; g < kEligGroup; g++) {
+        if (g >= nj) break;
+        key[(size_t)g * npairs + i] = (uint32_t)(e[g] < K ? e[g] : K);
+        val[(size_t)g * npairs + i] = i | (((exact >> g) & 1u) ? 0x80000000u : 0u);
+    }
 }
 
 // bstart[k] = first index of sorted keys with key >= k, for k = 0..K+1
@@ -617,11 +645,13 @@ __global__ void k_bucket_starts(const uint32_t *key, uint32_t n, int K, uint32_t
     for (uint32_t k = prev; k <= cur && k <= (uint32_t)K + 1; k++) bstart[k] = i;
 }
 
+// records of the pairs that ever become eligible (the first *nelig of the
+// sorted order; the grid covers all pairs)
 __global__ void k_pack_pairs(const uint32_t *order, const uint32_t *pa, const uint32_t *pb,
-                             const uint16_t *pk, uint32_t n, PairRec *rec)
+                             const uint16_t *pk, const uint32_t *nelig, PairRec *rec)
 {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
+    if (i >= *nelig) return;
     uint32_t v = order[i] & 0x7fffffffu;
     uint32_t exact = order[i] >> 31;
     rec[i] = PairRec{pa[v], pb[v], (uint32_t)pk[v] | (exact << 16)};
@@ -639,6 +669,18 @@ __device__ __forceinline__ void wave_append_rec(bool take, const PairRec &v, Pai
     if (take) list[base + mask_rank(m)] = v;
 }
 
+// Blocks of a worklist grid that take part at level k: one per
+// kPairsPerBlock pairs becoming eligible at the level, within
+// [kMinPersistent, launched grid] (more waves than the work needs only
+// contend on the union-find). The bucket starts stay on the device.
+__device__ __forceinline__ uint32_t level_blocks(const uint32_t *bstart, int k)
+{
+    uint32_t nfresh = bstart[k + 1] - bstart[k];
+    uint32_t want = (nfresh + kPairsPerBlock - 1) / kPairsPerBlock;
+    want = want > kMinPersistent ? want : kMinPersistent;
+    return want < gridDim.x ? want : gridDim.x;
+}
+
 // Grid-stride over the carried pairs (count on the device) followed by the
 // pairs that become eligible at this level. Each eligible pair is settled
 // (already connected; fully inside eps; small and tested here), kept for the
@@ -646,13 +688,17 @@ __device__ __forceinline__ void wave_append_rec(bool take, const PairRec &v, Pai
 // `near`, the rest to `far`; the near tests run first so most far pairs are
 // already connected when theirs run.
 __global__ void __launch_bounds__(kBlock)
-    k_sweep(LevelArgs a, const PairRec *carried, const uint32_t *ncarried,
-            const PairRec *fresh, uint32_t nfresh, uint32_t *parent, PairRec *keep,
-            uint32_t *nkeep, PairRec *near, uint32_t *nnear, PairRec *far, uint32_t *nfar)
+    k_sweep(LevelArgs a, const PairRec *carried, const uint32_t *ncarried, const PairRec *rec,
+            const uint32_t *bstart, uint32_t *parent, PairRec *keep, uint32_t *nkeep,
+            PairRec *near, uint32_t *nnear, PairRec *far, uint32_t *nfar)
 {
+    uint32_t nb = level_blocks(bstart, a.k);
+    if (blockIdx.x >= nb) return;
+    const PairRec *fresh = rec + bstart[a.k];
+    uint32_t nfresh = bstart[a.k + 1] - bstart[a.k];
     uint32_t nc = *ncarried;
     uint32_t total = nc + nfresh;
-    for (uint32_t base = blockIdx.x * kBlock; base < total; base += gridDim.x * kBlock) {
+    for (uint32_t base = blockIdx.x * kBlock; base < total; base += nb * kBlock) {
         uint32_t i = base + threadIdx.x;
         bool valid = i < total;
         PairRec pr{0, 0, 0};
@@ -757,11 +803,14 @@ __device__ void test_pair(const LevelArgs &a, const PairRec &pr, uint32_t *paren
 // the near tests, then the far ones, in one grid-stride pass (waves take
 // them in order, so most far tests start after the near ones)
 __global__ void __launch_bounds__(kBlock)
-    k_test(LevelArgs a, const PairRec *near, const uint32_t *nnear, const PairRec *far,
-           const uint32_t *nfar, uint32_t *parent, PairRec *list_out, uint32_t *nout)
+    k_test(LevelArgs a, const uint32_t *bstart, const PairRec *near, const uint32_t *nnear,
+           const PairRec *far, const uint32_t *nfar, uint32_t *parent, PairRec *list_out,
+           uint32_t *nout)
 {
+    uint32_t nb = level_blocks(bstart, a.k);
+    if (blockIdx.x >= nb) return;
     uint32_t nn = *nnear, nt = nn + *nfar;
-    for (uint32_t w = blockIdx.x * kWPB + wave_id_in_block(); w < nt; w += gridDim.x * kWPB)
+    for (uint32_t w = blockIdx.x * kWPB + wave_id_in_block(); w < nt; w += nb * kWPB)
         test_pair(a, w < nn ? near[w] : far[w - nn], parent, list_out, nout);
 }
 
@@ -1214,8 +1263,10 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_TRY(c.get("b.cnts", ncnt, &cnts));
     uint32_t *c_keep = cnts, *c_near = cnts + K + 1, *c_far = cnts + 2 * (K + 1);
     uint32_t *ekey, *eval, *stk, *stv, *bstart;
-    VDB_TRY(c.get("b.ekey", psz, &ekey));
-    VDB_TRY(c.get("b.eval", psz, &eval));
+    // eligibility keys of up to kEligGroup minpts values at once
+    const int jgroup = std::min(J, kEligGroup);
+    VDB_TRY(c.get("b.ekey", psz * jgroup, &ekey));
+    VDB_TRY(c.get("b.eval", psz * jgroup, &eval));
     VDB_TRY(c.get("b.stk", psz, &stk));
     VDB_TRY(c.get("b.stv", psz, &stv));
     VDB_TRY(c.get("b.bstart", (size_t)K + 2, &bstart));
@@ -1255,6 +1306,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_TRY(c.get("b.snap_cid", (size_t)K * n, &snap_cid));
     Ev tuf(c.stream());
     double ms_labels = 0;
+    int jg0 = -1, jgn = 0; // minpts values whose keys ekey/eval hold
     for (int j = 0; j < J; j++) {
         int kmax = -1;
         for (auto &w : want)
@@ -1266,29 +1318,32 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         // pair records bucketed by the level at which they become eligible
         std::vector<uint32_t> bst(K + 2, 0);
         if (npairs) {
-            k_pair_elig<<<ceil_div(npairs, kBlock), kBlock, 0, c.stream()>>>(
-                pa, pb, pk, cellkc + (size_t)j * G.ncell, G.cell_start, G.xs, G.ys, kc, dTk, J, j,
-                K, npairs, ekey,
-                eval);
-            VDB_HIP(hipGetLastError());
+            if (j < jg0 || j >= jg0 + jgn) {
+                jg0 = j;
+                jgn = std::min(kEligGroup, J - j);
+                k_pair_elig<<<ceil_div(npairs, kBlock), kBlock, 0, c.stream()>>>(
+                    pa, pb, pk, cellkc, G.ncell, G.cell_start, G.xs, G.ys, kc, dTk, J, jg0, jgn,
+                    K, npairs, ekey, eval);
+                VDB_HIP(hipGetLastError());
+            }
             uint32_t *sk, *sv;
-            VDB_TRY(radix_sort_pairs(c, ekey, eval, stk, stv, npairs, bits_for(K + 1), &sk, &sv));
+            size_t g = (size_t)(j - jg0) * npairs;
+            VDB_TRY(radix_sort_pairs(c, ekey + g, eval + g, stk, stv, npairs, bits_for(K + 1),
+                                     &sk, &sv));
             k_bucket_starts<<<ceil_div(npairs + 1, kBlock), kBlock, 0, c.stream()>>>(sk, npairs, K,
                                                                                    bstart);
             VDB_HIP(hipGetLastError());
-            uint32_t nelig = 0;
-            {
-                void *hs;
-                VDB_TRY(c.host_scratch(sizeof(uint32_t) * (K + 2), &hs));
-                VDB_HIP(hipMemcpyAsync(hs, bstart, sizeof(uint32_t) * (K + 2),
+            k_pack_pairs<<<ceil_div(npairs, kBlock), kBlock, 0, c.stream()>>>(sv, pa, pb, pk,
+                                                                            bstart + K, rec);
+            VDB_HIP(hipGetLastError());
+            if (verbose) {
+                // bucket sizes (a sync per minpts value: verbose only)
+                VDB_HIP(hipMemcpyAsync(bst.data(), bstart, sizeof(uint32_t) * (K + 2),
                                        hipMemcpyDeviceToHost, c.stream()));
                 VDB_HIP(hipStreamSynchronize(c.stream()));
-                memcpy(bst.data(), hs, sizeof(uint32_t) * (K + 2));
-                nelig = bst[K];
             }
-            if (nelig)
-                k_pack_pairs<<<ceil_div(nelig, kBlock), kBlock, 0, c.stream()>>>(sv, pa, pb, pk,
-                                                                               nelig, rec);
+        } else {
+            k_fill_u32<<<1, kBlock, 0, c.stream()>>>(bstart, 0u, (uint32_t)K + 2);
             VDB_HIP(hipGetLastError());
         }
         PairRec *lin = la, *lout = lb;
@@ -1297,17 +1352,15 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             LevelArgs a{G.xs,  G.ys,  G.cell_box, G.cell_start, G.cell_key, G.cell_ds,
                         G.skbase, G.sncy, kc,     cellkc,       pa,         pb,
                         pk,    J,     j,          k,            Th[k]};
-            uint32_t nfresh = npairs ? bst[k + 1] - bst[k] : 0;
-            const PairRec *fresh = rec + (npairs ? bst[k] : 0);
             // worklist sizes stay on the device: bounded grids stride over them
-            unsigned pb = std::min(kMaxPersistent,
-                                   std::max(kMinPersistent, ceil_div(nfresh, kPairsPerBlock)));
-            k_sweep<<<pb, kBlock, 0, c.stream()>>>(a, lin, c_keep + k, fresh, nfresh, parent, lout,
-                                                   c_keep + k + 1, tests, c_near + k, tests2,
-                                                   c_far + k);
+            uint32_t nfresh = bst[k + 1] - bst[k]; // verbose only
+            k_sweep<<<kMaxPersistent, kBlock, 0, c.stream()>>>(
+                a, lin, c_keep + k, rec, bstart, parent, lout, c_keep + k + 1, tests, c_near + k,
+                tests2, c_far + k);
             VDB_HIP(hipGetLastError());
-            k_test<<<pb, kBlock, 0, c.stream()>>>(a, tests, c_near + k, tests2, c_far + k, parent,
-                                                  lout, c_keep + k + 1);
+            k_test<<<kMaxPersistent, kBlock, 0, c.stream()>>>(a, bstart, tests, c_near + k, tests2,
+                                                              c_far + k, parent, lout,
+                                                              c_keep + k + 1);
             VDB_HIP(hipGetLastError());
             if (verbose) {
                 // worklist sizes of this level (a sync per level: verbose only)
