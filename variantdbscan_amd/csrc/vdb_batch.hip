@@ -207,12 +207,16 @@ constexpr uint32_t kKnnRowSweep = 512;
 // windows holding at most this many points are swept as one flat list
 constexpr uint32_t kKnnFlat = 256;
 
-// sort the lane-resident entries (lane e holds entry e) by (s, id), ascending
-__device__ __forceinline__ void wave_sort(double &es, uint32_t &eid)
+// sort the lane-resident entries (lane e holds entry e) by (s, id), ascending.
+// Only lanes below cnt hold entries (the rest hold (INF, NONE)), so the
+// bitonic network stops at the first power of two >= cnt: its stages sort
+// every aligned group of that size, the first one ascending.
+__device__ __forceinline__ void wave_sort(double &es, uint32_t &eid, int cnt)
 {
     int lane = lane_id();
 #pragma unroll
     for (int k = 2; k <= kWave; k <<= 1) {
+        if (k >= 2 * cnt) break;
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
             double ps = __shfl_xor(es, j);
@@ -258,7 +262,7 @@ __global__ void __launch_bounds__(kBlock)
     bool appending = true;
     // sort the appended entries, keep the L smallest, enter insertion mode
     auto settle = [&]() {
-        wave_sort(es, eid);
+        wave_sort(es, eid, cnt);
         if (lane >= L) {
             es = INFINITY;
             eid = kNone;
@@ -329,7 +333,8 @@ __global__ void __launch_bounds__(kBlock)
             uint32_t f = f0 + lane;
             // row of flat index f: the last row whose prefix is <= f
             int row = 0;
-            for (int r = 1; r < g.W; r++) row += __shfl(pre, r) <= f ? 1 : 0;
+            for (int r = 1; r < g.W; r++)
+                row += (uint32_t)__builtin_amdgcn_readlane((int)pre, r) <= f ? 1 : 0;
             uint32_t q = __shfl(ra, row) + (f - __shfl(pre, row));
             double s = INFINITY;
             if (f < total) s = dist2_exact(px, py, g.xs[q], g.ys[q]);
