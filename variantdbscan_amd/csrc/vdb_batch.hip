@@ -891,6 +891,7 @@ __global__ void k_snapshot_roots(const uint32_t *parent, const uint32_t *compmin
 // owns the block out + seg[d]*nrows, holding nrows rows of its n_d labels;
 // variant t writes row row[t] of it.
 constexpr int kLabelSlots = 8;
+static_assert(kLabelSlots == 8, "staged labels move as two 16-byte vectors per point");
 struct LabelSlots {
     int n;
     int k[kLabelSlots];
@@ -902,7 +903,7 @@ struct LabelSlots {
     uint32_t *out;
     uint32_t nrows;
     const uint32_t *seg;                   // nsets+1 dataset starts
-    uint32_t *tmp;                         // kLabelSlots rows of n labels, position order
+    uint32_t *tmp; // per position, its kLabelSlots labels side by side (32 bytes)
 };
 
 // label of a cell's core points at a snapshot: 1 + cluster id of its root,
@@ -986,25 +987,29 @@ __global__ void __launch_bounds__(kBlock)
     }
     uint32_t ds = cell_ds[C];
     uint32_t s0 = S.seg[ds];
+    uint32_t lbl[kLabelSlots];
 #pragma unroll
     for (int t = 0; t < kLabelSlots; t++) {
-        if (t >= S.n) break;
-        uint32_t lbl;
+        lbl[t] = 0u;
+        if (t >= S.n) continue;
         if (kcp <= S.k[t]) {
-            lbl = S.celllab[t][C];
+            lbl[t] = S.celllab[t][C];
         } else {
             uint32_t b = best[t];
-            lbl = (b != kNone && b < r) ? S.cid[t][b] - S.cid[t][s0] + 1 : 0u;
+            lbl[t] = (b != kNone && b < r) ? S.cid[t][b] - S.cid[t][s0] + 1 : 0u;
         }
-        // position order: coalesced; k_rows_from_pos moves them to caller order
-        S.tmp[(size_t)t * n + p] = lbl;
     }
+    // position order, the point's labels side by side: k_rows_from_pos moves
+    // them to caller order with one 32-byte read per point
+    uint4 *dst = (uint4 *)(S.tmp + (size_t)p * kLabelSlots);
+    dst[0] = make_uint4(lbl[0], lbl[1], lbl[2], lbl[3]);
+    dst[1] = make_uint4(lbl[4], lbl[5], lbl[6], lbl[7]);
     (void)i;
 }
 
 // Label rows from position order to caller order: thread per caller index,
-// so every row is written contiguously; the reads gather from one row of the
-// staging buffer at a time.
+// so every row is written contiguously; each point's labels are gathered in
+// one 32-byte read of the staging buffer.
 __global__ void k_rows_from_pos(LabelSlots S, const uint32_t *pos_of_caller, uint32_t nsets,
                                 uint32_t n)
 {
@@ -1014,10 +1019,13 @@ __global__ void k_rows_from_pos(LabelSlots S, const uint32_t *pos_of_caller, uin
     uint32_t ds = seg_of(S.seg, nsets, i);
     uint32_t s0 = S.seg[ds], nd = S.seg[ds + 1] - s0;
     uint32_t *base = S.out + (size_t)s0 * S.nrows + (i - s0);
+    const uint4 *src = (const uint4 *)(S.tmp + (size_t)p * kLabelSlots);
+    uint4 a = src[0], b = src[1];
+    uint32_t lbl[kLabelSlots] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
     for (int t = 0; t < kLabelSlots; t++) {
         if (t >= S.n) break;
-        base[(size_t)S.row[t] * nd] = S.tmp[(size_t)t * n + p];
+        base[(size_t)S.row[t] * nd] = lbl[t];
     }
 }
 
