@@ -514,11 +514,14 @@ __global__ void k_iota(uint32_t *a, uint32_t n)
     if (i < n) a[i] = i;
 }
 
-__global__ void k_init_parent(uint32_t *parent, uint32_t n, uint32_t *cnt, uint32_t ncnt)
+// parent[i] = i, and two arrays cleared (counters, snapshot root bits)
+__global__ void k_init_parent(uint32_t *parent, uint32_t n, uint32_t *z1, uint32_t n1,
+                              uint32_t *z2, uint32_t n2)
 {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) parent[i] = i;
-    if (i < ncnt) cnt[i] = 0;
+    if (i < n1) z1[i] = 0;
+    if (i < n2) z2[i] = 0;
 }
 
 // Append v to list[] for every lane with take set: one atomic per wave.
@@ -863,27 +866,22 @@ __global__ void k_comp_min(const uint32_t *parent, const uint32_t *cmrjk, const 
 }
 
 __global__ void k_root_flags_cells(const uint32_t *parent, const uint32_t *compmin,
-                                   const uint8_t *ckj, uint32_t ncell, int k, uint32_t *flag)
+                                   const uint8_t *ckj, uint32_t ncell, int k, uint32_t *bits)
 {
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= ncell) return;
     if (ckj[c] > k || parent[c] != c) return;
-    flag[compmin[c]] = 1u;
+    uint32_t r = compmin[c];
+    atomicOr(bits + (r >> 5), 1u << (r & 31));
 }
 
 // root rank of every cell at level k (NONE if the cell holds no core point)
 __global__ void k_snapshot_roots(const uint32_t *parent, const uint32_t *compmin,
-                                 const uint8_t *ckj, uint32_t ncell, int k, uint32_t *rootrank,
-                                 uint32_t *flag)
+                                 const uint8_t *ckj, uint32_t ncell, int k, uint32_t *rootrank)
 {
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= ncell) return;
-    bool core = ckj[c] <= k;
-    uint32_t r = parent[c];
-    rootrank[c] = core ? compmin[r] : kNone;
-    // the cluster-id scan has read the flags k_root_flags_cells set: clear
-    // them for the next snapshot
-    if (core && r == c) flag[compmin[c]] = 0u;
+    rootrank[c] = ckj[c] <= k ? compmin[parent[c]] : kNone;
 }
 
 // Labels of up to kLabelSlots variants sharing minpts j, from their level
@@ -897,7 +895,8 @@ struct LabelSlots {
     int k[kLabelSlots];
     double T[kLabelSlots];
     const uint32_t *rootrank[kLabelSlots]; // per cell
-    const uint32_t *cid[kLabelSlots];      // per rank: root ranks below it
+    const uint32_t *cbits[kLabelSlots];    // root ranks as a bitset over ranks
+    const uint32_t *cpre[kLabelSlots];     // per 32-rank word: roots before it
     const uint32_t *celllab[kLabelSlots];  // per cell: label of its core points
     uint32_t row[kLabelSlots];
     uint32_t *out;
@@ -906,16 +905,27 @@ struct LabelSlots {
     uint32_t *tmp; // per position, its kLabelSlots labels side by side (32 bytes)
 };
 
+// number of root ranks below rank r of a snapshot (bit r of bits is set for
+// every root rank; pre[w] counts the set bits of words 0..w-1)
+__device__ __forceinline__ uint32_t roots_below(const uint32_t *bits, const uint32_t *pre,
+                                                uint32_t r)
+{
+    uint32_t w = r >> 5;
+    return pre[w] + (uint32_t)__popc(bits[w] & ((1u << (r & 31)) - 1u));
+}
+
 // label of a cell's core points at a snapshot: 1 + cluster id of its root,
 // counted from the first root of the cell's dataset
-__global__ void k_cell_labels(const uint32_t *rootrank, const uint32_t *cid,
-                              const uint32_t *cell_ds, const uint32_t *seg, uint32_t ncell,
-                              uint32_t *celllab)
+__global__ void k_cell_labels(const uint32_t *rootrank, const uint32_t *bits,
+                              const uint32_t *pre, const uint32_t *cell_ds, const uint32_t *seg,
+                              uint32_t ncell, uint32_t *celllab)
 {
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= ncell) return;
     uint32_t rr = rootrank[c];
-    celllab[c] = rr == kNone ? 0u : cid[rr] - cid[seg[cell_ds[c]]] + 1;
+    celllab[c] = rr == kNone ? 0u
+                             : roots_below(bits, pre, rr) -
+                                   roots_below(bits, pre, seg[cell_ds[c]]) + 1;
 }
 
 // Thread per cell-order position: the point's data are read in sequence, the
@@ -996,7 +1006,9 @@ __global__ void __launch_bounds__(kBlock)
             lbl[t] = S.celllab[t][C];
         } else {
             uint32_t b = best[t];
-            lbl[t] = (b != kNone && b < r) ? S.cid[t][b] - S.cid[t][s0] + 1 : 0u;
+            lbl[t] = (b != kNone && b < r) ? roots_below(S.cbits[t], S.cpre[t], b) -
+                                                 roots_below(S.cbits[t], S.cpre[t], s0) + 1
+                                           : 0u;
         }
     }
     // position order, the point's labels side by side: k_rows_from_pos moves
@@ -1305,18 +1317,17 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         int j = (int)(std::lower_bound(mps.begin(), mps.end(), minpts[v]) - mps.begin());
         want[{k, j}].push_back(v);
     }
-    uint32_t *parent, *compmin, *rflag;
+    uint32_t *parent, *compmin;
     VDB_TRY(c.get("b.parent", G.ncell, &parent));
     VDB_TRY(c.get("b.compmin", G.ncell, &compmin));
-    VDB_TRY(c.get("b.rflag", n, &rflag));
-    // root flags start clear; every snapshot clears the ones it set
-    k_fill_u32<<<gn, kBlock, 0, c.stream()>>>(rflag, 0u, n);
-    VDB_HIP(hipGetLastError());
-    // per-level snapshots of one j: cell root ranks and cluster ids by rank
-    uint32_t *snap_root, *snap_cid, *snap_lab;
+    // per-level snapshots of one j: cell root ranks, the root ranks as a
+    // bitset over ranks with its per-word prefix counts (cluster ids)
+    const uint32_t nw = (n + 31) / 32;
+    uint32_t *snap_root, *snap_lab, *snap_bits, *snap_pre;
     VDB_TRY(c.get("b.snap_root", (size_t)K * G.ncell, &snap_root));
     VDB_TRY(c.get("b.snap_lab", (size_t)K * G.ncell, &snap_lab));
-    VDB_TRY(c.get("b.snap_cid", (size_t)K * n, &snap_cid));
+    VDB_TRY(c.get("b.snap_bits", (size_t)K * nw, &snap_bits));
+    VDB_TRY(c.get("b.snap_pre", (size_t)K * nw, &snap_pre));
     Ev tuf(c.stream());
     double ms_labels = 0;
     int jg0 = -1, jgn = 0; // minpts values whose keys ekey/eval hold
@@ -1325,8 +1336,9 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         for (auto &w : want)
             if (w.first.second == j) kmax = std::max(kmax, w.first.first);
         if (kmax < 0) continue;
-        k_init_parent<<<std::max(gc, ceil_div(ncnt, kBlock)), kBlock, 0, c.stream()>>>(
-            parent, G.ncell, cnts, ncnt);
+        uint32_t nbits = (uint32_t)K * nw;
+        k_init_parent<<<std::max({gc, ceil_div(ncnt, kBlock), ceil_div(nbits, kBlock)}), kBlock, 0,
+                        c.stream()>>>(parent, G.ncell, cnts, ncnt, snap_bits, nbits);
         VDB_HIP(hipGetLastError());
         // pair records bucketed by the level at which they become eligible
         std::vector<uint32_t> bst(K + 2, 0);
@@ -1393,7 +1405,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             int slot = (int)snap_k.size();
             snap_k.push_back(k);
             uint32_t *sroot = snap_root + (size_t)slot * G.ncell;
-            uint32_t *scid = snap_cid + (size_t)slot * n;
+            uint32_t *sbits = snap_bits + (size_t)slot * nw, *spre = snap_pre + (size_t)slot * nw;
             k_cell_roots<<<gc, kBlock, 0, c.stream()>>>(parent, G.ncell, compmin);
             VDB_HIP(hipGetLastError());
             const uint8_t *ckj = cellkc + (size_t)j * G.ncell;
@@ -1401,13 +1413,13 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
                 parent, cmr + ((size_t)j * K + k) * G.ncell, ckj, G.ncell, k, compmin);
             VDB_HIP(hipGetLastError());
             k_root_flags_cells<<<gc, kBlock, 0, c.stream()>>>(parent, compmin, ckj, G.ncell, k,
-                                                              rflag);
+                                                              sbits);
             VDB_HIP(hipGetLastError());
-            VDB_TRY(scan_exclusive_u32(c, rflag, scid, n, nullptr));
+            VDB_TRY(scan_popc_u32(c, sbits, spre, nw));
             k_snapshot_roots<<<gc, kBlock, 0, c.stream()>>>(parent, compmin, ckj, G.ncell, k,
-                                                            sroot, rflag);
+                                                            sroot);
             VDB_HIP(hipGetLastError());
-            k_cell_labels<<<gc, kBlock, 0, c.stream()>>>(sroot, scid, G.cell_ds, P.seg,
+            k_cell_labels<<<gc, kBlock, 0, c.stream()>>>(sroot, sbits, spre, G.cell_ds, P.seg,
                                                          G.ncell,
                                                          snap_lab + (size_t)slot * G.ncell);
             VDB_HIP(hipGetLastError());
@@ -1427,7 +1439,8 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
                 S.k[t] = k;
                 S.T[t] = Th[k];
                 S.rootrank[t] = snap_root + (size_t)sl * G.ncell;
-                S.cid[t] = snap_cid + (size_t)sl * n;
+                S.cbits[t] = snap_bits + (size_t)sl * nw;
+                S.cpre[t] = snap_pre + (size_t)sl * nw;
                 S.celllab[t] = snap_lab + (size_t)sl * G.ncell;
                 S.row[t] = rows[rows_j[r0 + t].second];
             }

@@ -9,6 +9,8 @@ namespace vdb {
 // out[i] = sum(in[0..i)) ; *total_dev (device pointer, may be null) = sum(in).
 int scan_exclusive_u32(Context &c, const uint32_t *in, uint32_t *out, size_t n,
                        uint32_t *total_dev);
+// out[i] = set bits in words[0..i).
+int scan_popc_u32(Context &c, const uint32_t *words, uint32_t *out, size_t n);
 int scan_exclusive_u32_u64(Context &c, const uint32_t *in, uint64_t *out, size_t n,
                            uint64_t *total_dev);
 

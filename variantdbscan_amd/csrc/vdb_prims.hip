@@ -46,8 +46,15 @@ __device__ __forceinline__ T block_excl_scan(T v, T *sm, T *block_total)
     return r;
 }
 
-template <typename In, typename Out>
-__global__ void __launch_bounds__(kScanBlock) k_tile_sums(const In *in, size_t n, Out *sums)
+// Scan inputs: a plain array, or the population counts of an array of
+// 32-bit words (the scan then numbers the set bits).
+struct PopcSrc {
+    const uint32_t *w;
+    __device__ uint32_t operator[](size_t i) const { return (uint32_t)__popc(w[i]); }
+};
+
+template <typename Src, typename Out>
+__global__ void __launch_bounds__(kScanBlock) k_tile_sums(Src in, size_t n, Out *sums)
 {
     __shared__ Out sm[kScanBlock / kWave + 1];
     size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
@@ -61,9 +68,9 @@ __global__ void __launch_bounds__(kScanBlock) k_tile_sums(const In *in, size_t n
 }
 
 // tile_off == nullptr: single tile, writes total
-template <typename In, typename Out>
+template <typename Src, typename Out>
 __global__ void __launch_bounds__(kScanBlock)
-    k_tile_scan(const In *in, Out *out, size_t n, const Out *tile_off, Out *total)
+    k_tile_scan(Src in, Out *out, size_t n, const Out *tile_off, Out *total)
 {
     __shared__ Out sm[kScanBlock / kWave + 1];
     size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
@@ -85,15 +92,15 @@ __global__ void __launch_bounds__(kScanBlock)
     if (!tile_off && total && threadIdx.x == 0) *total = tot;
 }
 
-template <typename In, typename Out>
-int scan_impl(Context &c, const In *in, Out *out, size_t n, Out *total_dev, int level)
+template <typename Src, typename Out>
+int scan_impl(Context &c, Src in, Out *out, size_t n, Out *total_dev, int level)
 {
     if (n == 0) {
         if (total_dev) VDB_HIP(hipMemsetAsync(total_dev, 0, sizeof(Out), c.stream()));
         return kOk;
     }
     if (n <= (size_t)kScanTile) {
-        k_tile_scan<In, Out><<<1, kScanBlock, 0, c.stream()>>>(in, out, n, nullptr, total_dev);
+        k_tile_scan<Src, Out><<<1, kScanBlock, 0, c.stream()>>>(in, out, n, nullptr, total_dev);
         VDB_HIP(hipGetLastError());
         return kOk;
     }
@@ -104,10 +111,10 @@ int scan_impl(Context &c, const In *in, Out *out, size_t n, Out *total_dev, int 
     Out *sums = nullptr, *offs = nullptr;
     VDB_TRY(c.get(na, tiles, &sums));
     VDB_TRY(c.get(nb, tiles, &offs));
-    k_tile_sums<In, Out><<<(unsigned)tiles, kScanBlock, 0, c.stream()>>>(in, n, sums);
+    k_tile_sums<Src, Out><<<(unsigned)tiles, kScanBlock, 0, c.stream()>>>(in, n, sums);
     VDB_HIP(hipGetLastError());
-    VDB_TRY((scan_impl<Out, Out>(c, sums, offs, tiles, total_dev, level + 1)));
-    k_tile_scan<In, Out><<<(unsigned)tiles, kScanBlock, 0, c.stream()>>>(in, out, n, offs, nullptr);
+    VDB_TRY((scan_impl<const Out *, Out>(c, sums, offs, tiles, total_dev, level + 1)));
+    k_tile_scan<Src, Out><<<(unsigned)tiles, kScanBlock, 0, c.stream()>>>(in, out, n, offs, nullptr);
     VDB_HIP(hipGetLastError());
     return kOk;
 }
@@ -256,13 +263,18 @@ int bits_for(uint64_t m)
 int scan_exclusive_u32(Context &c, const uint32_t *in, uint32_t *out, size_t n,
                        uint32_t *total_dev)
 {
-    return scan_impl<uint32_t, uint32_t>(c, in, out, n, total_dev, 0);
+    return scan_impl<const uint32_t *, uint32_t>(c, in, out, n, total_dev, 0);
+}
+
+int scan_popc_u32(Context &c, const uint32_t *words, uint32_t *out, size_t n)
+{
+    return scan_impl<PopcSrc, uint32_t>(c, PopcSrc{words}, out, n, nullptr, 0);
 }
 
 int scan_exclusive_u32_u64(Context &c, const uint32_t *in, uint64_t *out, size_t n,
                            uint64_t *total_dev)
 {
-    return scan_impl<uint32_t, uint64_t>(c, in, out, n, total_dev, 0);
+    return scan_impl<const uint32_t *, uint64_t>(c, in, out, n, total_dev, 0);
 }
 
 int compact_count(Context &c, const uint32_t *flags, uint32_t *out, size_t n,
