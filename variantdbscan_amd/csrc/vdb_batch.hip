@@ -216,7 +216,7 @@ __device__ __forceinline__ void wave_sort(double &es, uint32_t &eid, int cnt)
     int lane = lane_id();
 #pragma unroll
     for (int k = 2; k <= kWave; k <<= 1) {
-        if (k >= 2 * cnt) break;
+        if (k >= 2 * cnt) continue;
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
             double ps = __shfl_xor(es, j);
@@ -514,12 +514,16 @@ __global__ void k_iota(uint32_t *a, uint32_t n)
     if (i < n) a[i] = i;
 }
 
-// parent[i] = i, and two arrays cleared (counters, snapshot root bits)
-__global__ void k_init_parent(uint32_t *parent, uint32_t n, uint32_t *z1, uint32_t n1,
-                              uint32_t *z2, uint32_t n2)
+// parent[i] = i, compmin[i] = NONE, and two arrays cleared (counters,
+// snapshot root bits)
+__global__ void k_init_parent(uint32_t *parent, uint32_t *compmin, uint32_t n, uint32_t *z1,
+                              uint32_t n1, uint32_t *z2, uint32_t n2)
 {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) parent[i] = i;
+    if (i < n) {
+        parent[i] = i;
+        compmin[i] = kNone;
+    }
     if (i < n1) z1[i] = 0;
     if (i < n2) z2[i] = 0;
 }
@@ -823,31 +827,28 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---------------------------------------------------------- snapshots
-// flattens the forest (every cell points at its root) and clears compmin
-__global__ void k_cell_roots(uint32_t *parent, uint32_t ncell, uint32_t *compmin)
-{
-    uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= ncell) return;
-    compmin[c] = kNone;
-    uint32_t x = c, p = ld_u32(parent + x);
-    while (p != x) { // read-only walk; only true roots are stored
-        x = p;
-        p = ld_u32(parent + x);
-    }
-    st_u32(parent + c, x);
-}
-
-// compmin[root] = smallest core rank of the component. Lanes of a wave that
+// Flattens the forest (every cell points at its root) and sets
+// compmin[root] = smallest core rank of the component (compmin holds NONE on
+// entry: k_init_parent and k_cell_labels clear it). Lanes of a wave that
 // share a root (the common case inside a large cluster) reduce first, so a
 // giant component takes one atomic per wave rather than one per cell; after
 // the two most common roots of the wave the remaining lanes, spread over
 // small components, update theirs one atomic each.
-__global__ void k_comp_min(const uint32_t *parent, const uint32_t *cmrjk, const uint8_t *ckj,
-                           uint32_t ncell, int k, uint32_t *compmin)
+__global__ void k_roots_min(uint32_t *parent, const uint32_t *cmrjk, const uint8_t *ckj,
+                            uint32_t ncell, int k, uint32_t *compmin)
 {
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t root = kNone;
+    if (c < ncell) {
+        uint32_t x = c, p = ld_u32(parent + x);
+        while (p != x) { // read-only walk; only true roots are stored
+            x = p;
+            p = ld_u32(parent + x);
+        }
+        st_u32(parent + c, x);
+        root = x;
+    }
     bool valid = c < ncell && ckj[c] <= k;
-    uint32_t root = valid ? parent[c] : kNone;
     uint32_t v = valid ? cmrjk[c] : kNone;
     uint64_t todo = ballot(valid);
     for (int it = 0; it < 2 && todo; it++) {
@@ -865,23 +866,18 @@ __global__ void k_comp_min(const uint32_t *parent, const uint32_t *cmrjk, const 
     if (((todo >> lane_id()) & 1ull) && v < ld_u32(compmin + root)) atomicMin(compmin + root, v);
 }
 
-__global__ void k_root_flags_cells(const uint32_t *parent, const uint32_t *compmin,
-                                   const uint8_t *ckj, uint32_t ncell, int k, uint32_t *bits)
+// Snapshot of level k: root rank of every cell (NONE if the cell holds no
+// core point), and bit r of bits set for every root rank r.
+__global__ void k_snapshot(const uint32_t *parent, const uint32_t *compmin, const uint8_t *ckj,
+                           uint32_t ncell, int k, uint32_t *rootrank, uint32_t *bits)
 {
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= ncell) return;
-    if (ckj[c] > k || parent[c] != c) return;
-    uint32_t r = compmin[c];
-    atomicOr(bits + (r >> 5), 1u << (r & 31));
-}
-
-// root rank of every cell at level k (NONE if the cell holds no core point)
-__global__ void k_snapshot_roots(const uint32_t *parent, const uint32_t *compmin,
-                                 const uint8_t *ckj, uint32_t ncell, int k, uint32_t *rootrank)
-{
-    uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= ncell) return;
-    rootrank[c] = ckj[c] <= k ? compmin[parent[c]] : kNone;
+    bool core = ckj[c] <= k;
+    uint32_t r = parent[c];
+    uint32_t m = core ? compmin[r] : kNone;
+    rootrank[c] = m;
+    if (core && r == c) atomicOr(bits + (m >> 5), 1u << (m & 31));
 }
 
 // Labels of up to kLabelSlots variants sharing minpts j, from their level
@@ -916,12 +912,14 @@ __device__ __forceinline__ uint32_t roots_below(const uint32_t *bits, const uint
 
 // label of a cell's core points at a snapshot: 1 + cluster id of its root,
 // counted from the first root of the cell's dataset
+// (also clears compmin for the next snapshot: it is no longer read)
 __global__ void k_cell_labels(const uint32_t *rootrank, const uint32_t *bits,
                               const uint32_t *pre, const uint32_t *cell_ds, const uint32_t *seg,
-                              uint32_t ncell, uint32_t *celllab)
+                              uint32_t ncell, uint32_t *celllab, uint32_t *compmin)
 {
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= ncell) return;
+    compmin[c] = kNone;
     uint32_t rr = rootrank[c];
     celllab[c] = rr == kNone ? 0u
                              : roots_below(bits, pre, rr) -
@@ -1338,7 +1336,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         if (kmax < 0) continue;
         uint32_t nbits = (uint32_t)K * nw;
         k_init_parent<<<std::max({gc, ceil_div(ncnt, kBlock), ceil_div(nbits, kBlock)}), kBlock, 0,
-                        c.stream()>>>(parent, G.ncell, cnts, ncnt, snap_bits, nbits);
+                        c.stream()>>>(parent, compmin, G.ncell, cnts, ncnt, snap_bits, nbits);
         VDB_HIP(hipGetLastError());
         // pair records bucketed by the level at which they become eligible
         std::vector<uint32_t> bst(K + 2, 0);
@@ -1406,22 +1404,17 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             snap_k.push_back(k);
             uint32_t *sroot = snap_root + (size_t)slot * G.ncell;
             uint32_t *sbits = snap_bits + (size_t)slot * nw, *spre = snap_pre + (size_t)slot * nw;
-            k_cell_roots<<<gc, kBlock, 0, c.stream()>>>(parent, G.ncell, compmin);
-            VDB_HIP(hipGetLastError());
             const uint8_t *ckj = cellkc + (size_t)j * G.ncell;
-            k_comp_min<<<gc, kBlock, 0, c.stream()>>>(
+            k_roots_min<<<gc, kBlock, 0, c.stream()>>>(
                 parent, cmr + ((size_t)j * K + k) * G.ncell, ckj, G.ncell, k, compmin);
             VDB_HIP(hipGetLastError());
-            k_root_flags_cells<<<gc, kBlock, 0, c.stream()>>>(parent, compmin, ckj, G.ncell, k,
-                                                              sbits);
+            k_snapshot<<<gc, kBlock, 0, c.stream()>>>(parent, compmin, ckj, G.ncell, k, sroot,
+                                                      sbits);
             VDB_HIP(hipGetLastError());
             VDB_TRY(scan_popc_u32(c, sbits, spre, nw));
-            k_snapshot_roots<<<gc, kBlock, 0, c.stream()>>>(parent, compmin, ckj, G.ncell, k,
-                                                            sroot);
-            VDB_HIP(hipGetLastError());
             k_cell_labels<<<gc, kBlock, 0, c.stream()>>>(sroot, sbits, spre, G.cell_ds, P.seg,
-                                                         G.ncell,
-                                                         snap_lab + (size_t)slot * G.ncell);
+                                                         G.ncell, snap_lab + (size_t)slot * G.ncell,
+                                                         compmin);
             VDB_HIP(hipGetLastError());
         }
         // labels of every requested variant of this j, kLabelSlots rows per launch
