@@ -54,13 +54,15 @@ constexpr unsigned kMaxPersistent = 1024;
 struct FG {
     const double *xs, *ys, *box;
     const uint32_t *cell_start, *cell_of_pos, *row_lo, *row_hi, *pos_rank;
+    const uint32_t *cell_key, *cell_ds, *sncy;
     int W;
 };
 
 FG fview(const Grid &G)
 {
-    return FG{G.xs, G.ys, G.cell_box, G.cell_start, G.cell_of_pos, G.row_lo, G.row_hi,
-              G.pos_rank, 2 * G.R + 1};
+    return FG{G.xs,       G.ys,       G.cell_box,   G.cell_start, G.cell_of_pos,
+              G.row_lo,   G.row_hi,   G.pos_rank,   G.cell_key,   G.cell_ds,
+              G.sncy,     2 * G.R + 1};
 }
 
 __device__ __forceinline__ uint32_t ld_u32(const uint32_t *p)
@@ -342,45 +344,69 @@ __global__ void __launch_bounds__(kBlock)
         }
         done = true;
     }
-    for (int t = 0; t < g.W && !done; t++) {
-        int d = (t & 1) ? -((t + 1) >> 1) : (t >> 1); // 0, -1, +1, -2, +2, ...
+    // Rows path. The 3 x 3 cells around the point's own cell go first, so
+    // that the threshold is tight before the rest of the window is pruned
+    // against it; the rows then follow centre-out, skipping those cells.
+    // Keys are column-major (key = kbase + cx*ncy + cy) and a window row is
+    // one column cx+d, so the centre cells of row d are the cells of its
+    // range with keys within 1 of k0 + d*ncy. Steps 0..2 visit the centre
+    // cells of rows 0, -1, +1; steps 3.. the window rows 0, -1, +1, -2, ...
+    uint32_t k0 = g.cell_key[c], ncy = g.sncy[g.cell_ds[c]];
+    for (int t = 0; t < g.W + 3 && !done; t++) {
+        bool centre_pass = t < 3;
+        int u = centre_pass ? t : t - 3;
+        int d = (u & 1) ? -((u + 1) >> 1) : (u >> 1); // 0, -1, +1, -2, +2, ...
+        if (centre_pass && (d < -R || d > R)) continue;
         int r = d + R;
         uint32_t lo = g.row_lo[(size_t)c * g.W + r], hi = g.row_hi[(size_t)c * g.W + r];
         if (lo >= hi) continue;
-        for (uint32_t cb = lo; cb < hi && !done; cb += kWave) {
-            // the row's cells that can hold a candidate (exact box bound)
-            uint32_t C = cb + lane;
-            double S = INFINITY;
-            if (C < hi) {
-                const double *b = g.box + 4 * (size_t)C;
-                S = box_dist2_min(px, py, b[0], b[1], b[2], b[3]);
+        uint32_t kc = k0 + (uint32_t)d * ncy;
+        // a row spans at most 2R+1 <= 64 cells
+        uint32_t C = lo + lane;
+        double S = INFINITY;
+        bool centre = false;
+        if (C < hi) {
+            const double *b = g.box + 4 * (size_t)C;
+            S = box_dist2_min(px, py, b[0], b[1], b[2], b[3]);
+            if (d >= -1 && d <= 1) {
+                uint32_t kC = g.cell_key[C];
+                centre = kC + 1u >= kc && kC <= kc + 1u;
             }
-            uint64_t m = ballot(S <= thr);
-            if (!m) continue;
-            // their points are one contiguous span (cells of a row are
-            // consecutive): sweep it whole when short, else cell by cell
-            uint32_t cf = cb + __builtin_ctzll(m), cl = cb + 63 - __builtin_clzll(m);
-            uint32_t a = g.cell_start[cf], e = g.cell_start[cl + 1];
-            if (e - a <= kKnnRowSweep) {
+        }
+        uint64_t mcen = ballot(centre);
+        // the cells to visit: the centre ones, or the others that can hold a
+        // candidate (exact box bound); the centre cells split the others
+        // into a part below and a part above them
+        uint64_t m = centre_pass ? mcen : ballot(S <= thr && !centre);
+        if (!m) continue;
+        uint64_t below = mcen && !centre_pass ? (1ull << __builtin_ctzll(mcen)) - 1ull : ~0ull;
+        uint64_t parts[2] = {m & below, m & ~below};
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            uint64_t mm = parts[h];
+            if (!mm || done) continue;
+            // a part's points are one contiguous span (cells of a row are
+            // consecutive): swept whole when short, else cell by cell
+            uint32_t cf = lo + __builtin_ctzll(mm), cl = lo + 63 - __builtin_clzll(mm);
+            bool whole = centre_pass ||
+                         g.cell_start[cl + 1] - g.cell_start[cf] <= kKnnRowSweep;
+            while (mm && !done) {
+                uint32_t a, e;
+                if (whole) {
+                    a = g.cell_start[cf];
+                    e = g.cell_start[cl + 1];
+                    mm = 0;
+                } else {
+                    int bit = __builtin_ctzll(mm);
+                    mm &= mm - 1;
+                    if (__shfl(S, bit) > thr) continue;
+                    a = g.cell_start[lo + bit];
+                    e = g.cell_start[lo + bit + 1];
+                }
                 for (uint32_t q0 = a; q0 < e && !done; q0 += kWave) {
                     uint32_t q = q0 + lane;
                     double s = INFINITY;
                     if (q < e) s = dist2_exact(px, py, g.xs[q], g.ys[q]);
-                    block(s, q);
-                }
-                continue;
-            }
-            while (m && !done) {
-                int bit = __builtin_ctzll(m);
-                m &= m - 1;
-                double SC = __shfl(S, bit);
-                if (SC > thr) continue;
-                uint32_t Cc = cb + bit;
-                uint32_t ca = g.cell_start[Cc], ce = g.cell_start[Cc + 1];
-                for (uint32_t q0 = ca; q0 < ce && !done; q0 += kWave) {
-                    uint32_t q = q0 + lane;
-                    double s = INFINITY;
-                    if (q < ce) s = dist2_exact(px, py, g.xs[q], g.ys[q]);
                     block(s, q);
                 }
             }
