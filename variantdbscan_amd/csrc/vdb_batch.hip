@@ -453,34 +453,29 @@ __global__ void k_core_levels(const uint32_t *lst_idx, const double *lst_s,
 
 // per cell, per j, per k: smallest rank of a point with kc_j <= k (cells list
 // their points in ascending rank, so it is the first such point), and the
-// first level at which the cell holds a core point. Lane k tracks level k.
-// Both are stored level-major (cmr[(j*K + k)*ncell + c], cellkc[j*ncell + c])
-// so that the per-level passes over all cells read them contiguously.
-__global__ void __launch_bounds__(kBlock)
-    k_cell_core(const uint32_t *cell_start, uint32_t ncell, const uint8_t *kc, int J, int K,
-                const uint32_t *pos_rank, uint32_t *cmr, uint8_t *cellkc)
+// first level at which the cell holds a core point. Thread per cell: one pass
+// over the cell's points keeps the lowest level seen so far, and a point that
+// lowers it is the first for every level it newly covers. Both tables are
+// stored level-major (cmr[(j*K + k)*ncell + c], cellkc[j*ncell + c]) so that
+// the per-level passes over all cells read them contiguously.
+__global__ void k_cell_core(const uint32_t *cell_start, uint32_t ncell, const uint8_t *kc, int J,
+                            int K, const uint32_t *pos_rank, uint32_t *cmr, uint8_t *cellkc)
 {
-    uint32_t c = blockIdx.x * kWPB + wave_id_in_block();
+    uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= ncell) return;
-    int lane = lane_id();
     uint32_t a = cell_start[c], e = cell_start[c + 1];
     for (int j = 0; j < J; j++) {
-        uint32_t first = kNone; // level `lane`
-        bool all = false;
-        for (uint32_t q0 = a; q0 < e && !all; q0 += kWave) {
-            uint32_t q = q0 + lane;
-            int lv = q < e ? (int)kc[(size_t)q * J + j] : K;
-            for (int k = 0; k < K; k++) {
-                uint64_t m = ballot(lv <= k);
-                if (lane == k && first == kNone && m) first = q0 + __builtin_ctzll(m);
-            }
-            // level 0 found: every level is resolved
-            all = __shfl(first, 0) != kNone;
+        uint32_t *row = cmr + (size_t)j * K * ncell + c;
+        int mn = K; // levels [mn, K) have their first point
+        for (uint32_t q = a; q < e && mn > 0; q++) {
+            int lv = kc[(size_t)q * J + j];
+            if (lv >= mn) continue;
+            uint32_t r = pos_rank[q];
+            for (int k = lv; k < mn; k++) row[(size_t)k * ncell] = r;
+            mn = lv;
         }
-        if (lane < K)
-            cmr[((size_t)j * K + lane) * ncell + c] = first == kNone ? kNone : pos_rank[first];
-        uint64_t have = ballot(lane < K && first != kNone);
-        if (lane == 0) cellkc[(size_t)j * ncell + c] = (uint8_t)(have ? __builtin_ctzll(have) : K);
+        for (int k = 0; k < mn; k++) row[(size_t)k * ncell] = kNone;
+        cellkc[(size_t)j * ncell + c] = (uint8_t)mn;
     }
 }
 
@@ -1263,8 +1258,9 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_HIP(hipGetLastError());
     VDB_TRY(c.get("b.cmr", (size_t)G.ncell * J * K, &cmr));
     VDB_TRY(c.get("b.cellkc", (size_t)G.ncell * J, &cellkc));
-    k_cell_core<<<ceil_div(G.ncell, kWPB), kBlock, 0, c.stream()>>>(G.cell_start, G.ncell, kc, J,
-                                                                    K, G.pos_rank, cmr, cellkc);
+    unsigned gc0 = ceil_div(G.ncell, kBlock);
+    k_cell_core<<<gc0, kBlock, 0, c.stream()>>>(G.cell_start, G.ncell, kc, J, K, G.pos_rank, cmr,
+                                                cellkc);
     VDB_HIP(hipGetLastError());
     if (st) st->ms_count += tknn.ms();
 
