@@ -186,33 +186,61 @@ __global__ void k_cell_tail(uint32_t *cell_start, uint32_t ncell, uint32_t n)
     if (threadIdx.x == 0 && blockIdx.x == 0) cell_start[ncell] = n;
 }
 
-// bbox of each cell from its points; one wave per cell
+// Bounding box of every cell's points. Thread per cell for the common small
+// cell (its points read serially); cells holding more than kBoxSerial points
+// are then reduced by the whole wave, one after another.
+constexpr uint32_t kBoxSerial = 16;
 __global__ void k_cell_box(const double *xs, const double *ys, const uint32_t *cell_start,
                            uint32_t ncell, double *box)
 {
-    uint32_t c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-    if (c >= ncell) return;
-    uint32_t a = cell_start[c], b = cell_start[c + 1];
-    double x0 = DBL_MAX, x1 = -DBL_MAX, y0 = DBL_MAX, y1 = -DBL_MAX;
-    for (uint32_t p = a + lane_id(); p < b; p += kWave) {
-        double u = xs[p], v = ys[p];
-        x0 = u < x0 ? u : x0;
-        x1 = u > x1 ? u : x1;
-        y0 = v < y0 ? v : y0;
-        y1 = v > y1 ? v : y1;
+    uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t a = 0, b = 0;
+    if (c < ncell) {
+        a = cell_start[c];
+        b = cell_start[c + 1];
     }
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-        double t;
-        t = __shfl_xor(x0, o); x0 = t < x0 ? t : x0;
-        t = __shfl_xor(x1, o); x1 = t > x1 ? t : x1;
-        t = __shfl_xor(y0, o); y0 = t < y0 ? t : y0;
-        t = __shfl_xor(y1, o); y1 = t > y1 ? t : y1;
-    }
-    if (lane_id() == 0) {
+    bool big = b - a > kBoxSerial;
+    if (c < ncell && !big) {
+        double x0 = DBL_MAX, x1 = -DBL_MAX, y0 = DBL_MAX, y1 = -DBL_MAX;
+        for (uint32_t p = a; p < b; p++) {
+            double u = xs[p], v = ys[p];
+            x0 = u < x0 ? u : x0;
+            x1 = u > x1 ? u : x1;
+            y0 = v < y0 ? v : y0;
+            y1 = v > y1 ? v : y1;
+        }
         box[4 * (size_t)c + 0] = x0;
         box[4 * (size_t)c + 1] = x1;
         box[4 * (size_t)c + 2] = y0;
         box[4 * (size_t)c + 3] = y1;
+    }
+    uint64_t mb = ballot(big);
+    while (mb) {
+        int bit = __builtin_ctzll(mb);
+        mb &= mb - 1;
+        uint32_t cc = c - (uint32_t)lane_id() + (uint32_t)bit;
+        uint32_t ca = __shfl(a, bit), cb = __shfl(b, bit);
+        double x0 = DBL_MAX, x1 = -DBL_MAX, y0 = DBL_MAX, y1 = -DBL_MAX;
+        for (uint32_t p = ca + lane_id(); p < cb; p += kWave) {
+            double u = xs[p], v = ys[p];
+            x0 = u < x0 ? u : x0;
+            x1 = u > x1 ? u : x1;
+            y0 = v < y0 ? v : y0;
+            y1 = v > y1 ? v : y1;
+        }
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            double t;
+            t = __shfl_xor(x0, o); x0 = t < x0 ? t : x0;
+            t = __shfl_xor(x1, o); x1 = t > x1 ? t : x1;
+            t = __shfl_xor(y0, o); y0 = t < y0 ? t : y0;
+            t = __shfl_xor(y1, o); y1 = t > y1 ? t : y1;
+        }
+        if (lane_id() == 0) {
+            box[4 * (size_t)cc + 0] = x0;
+            box[4 * (size_t)cc + 1] = x1;
+            box[4 * (size_t)cc + 2] = y0;
+            box[4 * (size_t)cc + 3] = y1;
+        }
     }
 }
 
@@ -553,7 +581,7 @@ int build_grid(Context &c, const Points &P, double w_min, int R, const char *tag
     VDB_TRY(c.get(slot(tag, "maxpop").c_str(), 1, &maxpop));
     VDB_HIP(hipMemsetAsync(maxpop, 0, sizeof(uint32_t), c.stream()));
     VDB_TRY(c.get(slot(tag, "cell_box").c_str(), (size_t)ncell * 4, &G.cell_box));
-    k_cell_box<<<ceil_div(ncell, kBlock / kWave), kBlock, 0, c.stream()>>>(
+    k_cell_box<<<ceil_div(ncell, kBlock), kBlock, 0, c.stream()>>>(
         G.xs, G.ys, G.cell_start, ncell, G.cell_box);
     VDB_HIP(hipGetLastError());
     k_max_pop<<<std::min<unsigned>(ceil_div(ncell, kBlock), 512), kBlock, 0, c.stream()>>>(
