@@ -50,6 +50,9 @@ constexpr uint32_t kSmallPairWork = 16;
 constexpr unsigned kPairsPerBlock = 8192;
 constexpr unsigned kMinPersistent = 512;
 constexpr unsigned kMaxPersistent = 1024;
+// the cell-pair test kernel: waves per test list size, and its largest grid
+constexpr unsigned kTestsPerBlock = 64;
+constexpr unsigned kMaxTestBlocks = 2048;
 
 struct FG {
     const double *xs, *ys, *box;
@@ -793,7 +796,10 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-// wave per test pair: any core-core pair within eps_k connects the two cells
+// Wave per test pair: any core-core pair within eps_k connects the two cells.
+// The smaller cell A is screened 64 points at a time (core at level k and
+// within eps of B's box); only the survivors are tested, one after another,
+// against B. B's points are held in registers when B fits one wave.
 __device__ void test_pair(const LevelArgs &a, const PairRec &pr, uint32_t *parent,
                           PairRec *list_out, uint32_t *nout)
 {
@@ -807,26 +813,53 @@ __device__ void test_pair(const LevelArgs &a, const PairRec &pr, uint32_t *paren
     }
     bool found = cf_same(parent, A, B);
     if (!found) {
+        int lane = lane_id();
         uint32_t a0 = a.cell_start[A], a1 = a.cell_start[A + 1];
         uint32_t b0 = a.cell_start[B], b1 = a.cell_start[B + 1];
         const double *bB = a.box + 4 * (size_t)B;
         double bx0 = bB[0], bx1 = bB[1], by0 = bB[2], by1 = bB[3];
-        for (uint32_t p = a0; p < a1 && !found; p++) {
-            if (a.kc[(size_t)p * a.J + a.j] > a.k) continue;
-            double px = a.xs[p], py = a.ys[p];
-            // no point of B can be within eps of p
-            if (box_dist2_min(px, py, bx0, bx1, by0, by1) > a.T) continue;
-            for (uint32_t q0 = b0; q0 < b1; q0 += kWave) {
-                uint32_t q = q0 + lane_id();
-                bool hit = q < b1 && a.kc[(size_t)q * a.J + a.j] <= a.k &&
-                           dist2_exact(px, py, a.xs[q], a.ys[q]) <= a.T;
-                if (ballot(hit)) {
-                    found = true;
-                    break;
+        // B in one wave: lane q holds point b0+q (NaN when absent or not core)
+        bool bsmall = b1 - b0 <= (uint32_t)kWave;
+        double qx = __longlong_as_double(0x7ff8000000000000ll), qy = qx;
+        if (bsmall) {
+            uint32_t q = b0 + lane;
+            if (q < b1 && a.kc[(size_t)q * a.J + a.j] <= a.k) {
+                qx = a.xs[q];
+                qy = a.ys[q];
+            }
+        }
+        for (uint32_t p0 = a0; p0 < a1 && !found; p0 += kWave) {
+            uint32_t p = p0 + lane;
+            double px = 0.0, py = 0.0;
+            bool cand = false;
+            if (p < a1 && a.kc[(size_t)p * a.J + a.j] <= a.k) {
+                px = a.xs[p];
+                py = a.ys[p];
+                // some point of B can be within eps of p
+                cand = box_dist2_min(px, py, bx0, bx1, by0, by1) <= a.T;
+            }
+            uint64_t cm = ballot(cand);
+            while (cm && !found) {
+                int bit = __builtin_ctzll(cm);
+                cm &= cm - 1;
+                double cx = __shfl(px, bit), cy = __shfl(py, bit);
+                if (bsmall) {
+                    // NaN coordinates never pass the test
+                    found = ballot(dist2_exact(cx, cy, qx, qy) <= a.T) != 0;
+                    continue;
+                }
+                for (uint32_t q0 = b0; q0 < b1; q0 += kWave) {
+                    uint32_t q = q0 + lane;
+                    bool hit = q < b1 && a.kc[(size_t)q * a.J + a.j] <= a.k &&
+                               dist2_exact(cx, cy, a.xs[q], a.ys[q]) <= a.T;
+                    if (ballot(hit)) {
+                        found = true;
+                        break;
+                    }
                 }
             }
         }
-        if (lane_id() == 0) {
+        if (lane == 0) {
             if (found) cf_unite(parent, A, B);
             else list_out[atomicAdd(nout, 1u)] = pr;
         }
@@ -836,13 +869,17 @@ __device__ void test_pair(const LevelArgs &a, const PairRec &pr, uint32_t *paren
 // the near tests, then the far ones, in one grid-stride pass (waves take
 // them in order, so most far tests start after the near ones)
 __global__ void __launch_bounds__(kBlock)
-    k_test(LevelArgs a, const uint32_t *bstart, const PairRec *near, const uint32_t *nnear,
+    k_test(LevelArgs a, const PairRec *near, const uint32_t *nnear,
            const PairRec *far, const uint32_t *nfar, uint32_t *parent, PairRec *list_out,
            uint32_t *nout)
 {
-    uint32_t nb = level_blocks(bstart, a.k);
-    if (blockIdx.x >= nb) return;
+    // tests are latency-bound waves: one block per kTestsPerBlock tests,
+    // up to the launched grid
     uint32_t nn = *nnear, nt = nn + *nfar;
+    uint32_t nb = (nt + kTestsPerBlock - 1) / kTestsPerBlock;
+    nb = nb > kMinPersistent ? nb : kMinPersistent;
+    nb = nb < gridDim.x ? nb : gridDim.x;
+    if (blockIdx.x >= nb) return;
     for (uint32_t w = blockIdx.x * kWPB + wave_id_in_block(); w < nt; w += nb * kWPB)
         test_pair(a, w < nn ? near[w] : far[w - nn], parent, list_out, nout);
 }
@@ -1403,7 +1440,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
                 a, lin, c_keep + k, rec, bstart, parent, lout, c_keep + k + 1, tests, c_near + k,
                 tests2, c_far + k);
             VDB_HIP(hipGetLastError());
-            k_test<<<kMaxPersistent, kBlock, 0, c.stream()>>>(a, bstart, tests, c_near + k, tests2,
+            k_test<<<kMaxTestBlocks, kBlock, 0, c.stream()>>>(a, tests, c_near + k, tests2,
                                                               c_far + k, parent, lout,
                                                               c_keep + k + 1);
             VDB_HIP(hipGetLastError());
