@@ -121,14 +121,18 @@ __device__ void cf_unite(uint32_t *parent, uint32_t a, uint32_t b)
             a = b;
             b = t;
         }
-        if (atomicCAS(parent + b, b, a) == b) return;
+        uint32_t ret = atomicCAS(parent + b, b, a);
+        // hooked, or another wave hooked b under the same root
+        if (ret == b || ret == a) return;
     }
 }
 
-// link root x under m < x, or unite the two if x stopped being a root
+// link root x under m < x, or unite the two if x stopped being a root (the
+// common loss: another wave hooked x under the same m, then nothing is left)
 __device__ __forceinline__ void cf_hook(uint32_t *parent, uint32_t x, uint32_t m)
 {
-    if (atomicCAS(parent + x, x, m) != x) cf_unite(parent, x, m);
+    uint32_t ret = atomicCAS(parent + x, x, m);
+    if (ret != x && ret != m) cf_unite(parent, ret, m);
 }
 
 // ------------------------------------------------------------- cliques
