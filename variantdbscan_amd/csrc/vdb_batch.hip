@@ -113,8 +113,11 @@ __device__ __forceinline__ bool cf_roots(uint32_t *parent, uint32_t a, uint32_t 
 __device__ void cf_unite(uint32_t *parent, uint32_t a, uint32_t b)
 {
     while (true) {
-        a = cf_find(parent, a);
-        b = cf_find(parent, b);
+        // both parents in one round of loads: equal parents settle it
+        uint32_t pa = ld_u32(parent + a), pb = ld_u32(parent + b);
+        if (pa == pb) return;
+        if (pa != a) a = cf_find(parent, pa);
+        if (pb != b) b = cf_find(parent, pb);
         if (a == b) return;
         if (a > b) {
             uint32_t t = a;
@@ -124,6 +127,7 @@ __device__ void cf_unite(uint32_t *parent, uint32_t a, uint32_t b)
         uint32_t ret = atomicCAS(parent + b, b, a);
         // hooked, or another wave hooked b under the same root
         if (ret == b || ret == a) return;
+        b = ret; // b was hooked elsewhere: go on from its new parent
     }
 }
 
