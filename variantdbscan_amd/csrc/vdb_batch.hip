@@ -323,12 +323,15 @@ __global__ void __launch_bounds__(kBlock)
         if (cnt == L && kth <= T0) done = true;
     };
     int R = g.W / 2;
-    // All window rows' point ranges in one round of loads (lane r: row r).
-    // A window holding few points is swept as one flat list of candidates,
-    // 64 per step, instead of row by row.
+    // All window rows' point ranges in one round of loads, lane l holding
+    // the l-th row centre-out (0, -1, +1, -2, ...). A window holding few
+    // points is swept as one flat list of candidates in that row order, 64
+    // per step, instead of row by row.
     uint32_t ra = 0, rn = 0;
     if (lane < g.W) {
-        uint32_t lo = g.row_lo[(size_t)c * g.W + lane], hi = g.row_hi[(size_t)c * g.W + lane];
+        int dl = (lane & 1) ? -((lane + 1) >> 1) : (lane >> 1);
+        size_t t = (size_t)c * g.W + (dl + g.W / 2);
+        uint32_t lo = g.row_lo[t], hi = g.row_hi[t];
         if (lo < hi) {
             ra = g.cell_start[lo];
             rn = g.cell_start[hi] - ra;
