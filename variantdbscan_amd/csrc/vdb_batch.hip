@@ -327,14 +327,15 @@ __global__ void __launch_bounds__(kBlock)
     // the l-th row centre-out (0, -1, +1, -2, ...). A window holding few
     // points is swept as one flat list of candidates in that row order, 64
     // per step, instead of row by row.
-    uint32_t ra = 0, rn = 0;
+    uint32_t ra = 0, rn = 0, rlo = 0, rhi = 0;
     if (lane < g.W) {
         int dl = (lane & 1) ? -((lane + 1) >> 1) : (lane >> 1);
         size_t t = (size_t)c * g.W + (dl + g.W / 2);
-        uint32_t lo = g.row_lo[t], hi = g.row_hi[t];
-        if (lo < hi) {
-            ra = g.cell_start[lo];
-            rn = g.cell_start[hi] - ra;
+        rlo = g.row_lo[t];
+        rhi = g.row_hi[t];
+        if (rlo < rhi) {
+            ra = g.cell_start[rlo];
+            rn = g.cell_start[rhi] - ra;
         }
     }
     uint32_t pre = rn; // inclusive prefix over rows
@@ -371,8 +372,9 @@ __global__ void __launch_bounds__(kBlock)
         int u = centre_pass ? t : t - 3;
         int d = (u & 1) ? -((u + 1) >> 1) : (u >> 1); // 0, -1, +1, -2, +2, ...
         if (centre_pass && (d < -R || d > R)) continue;
-        int r = d + R;
-        uint32_t lo = g.row_lo[(size_t)c * g.W + r], hi = g.row_hi[(size_t)c * g.W + r];
+        // the row's cell range, held by lane u since the first loads
+        uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)rlo, u);
+        uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)rhi, u);
         if (lo >= hi) continue;
         uint32_t kc = k0 + (uint32_t)d * ncy;
         // a row spans at most 2R+1 <= 64 cells
