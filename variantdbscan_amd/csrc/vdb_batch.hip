@@ -718,6 +718,52 @@ __device__ __forceinline__ void wave_append_rec(bool take, const PairRec &v, Pai
     if (take) list[base + mask_rank(m)] = v;
 }
 
+// Test lists are appended in chunks: a wave reserves kRecChunk records with
+// one atomic and fills them over many steps, so the list counter sees one
+// atomic per chunk instead of one per wave per step (on a level where most
+// pairs need a test, those atomics on one address dominated the sweep). A
+// chunk closed early, and the chunk open when the sweep ends, get sentinel
+// records (a = NONE) in their unused slots. A closed chunk holds at least
+// kRecChunk - 63 records, so a list of A records reserves at most
+// A * kRecChunk / (kRecChunk - 63) + kRecChunk per wave.
+constexpr uint32_t kRecChunk = 256;
+// levels with fewer pairs becoming eligible append exactly (one atomic per
+// wave and step; few steps per wave, and the test lists stay dense)
+constexpr uint32_t kChunkedFrom = 1u << 22;
+struct RecChunk {
+    uint32_t at = 0, left = 0; // next free record, records left (wave-uniform)
+};
+
+__device__ __forceinline__ void chunk_close(RecChunk &ch, PairRec *list)
+{
+    for (uint32_t o = lane_id(); o < ch.left; o += kWave) list[ch.at + o] = PairRec{kNone, kNone, 0};
+    ch.left = 0;
+}
+
+__device__ __forceinline__ void chunk_append(bool take, const PairRec &v, PairRec *list,
+                                             uint32_t *count, uint32_t cap, uint32_t *ovf,
+                                             RecChunk &ch)
+{
+    uint64_t m = ballot(take);
+    if (!m) return;
+    uint32_t h = (uint32_t)popc64(m);
+    if (h > ch.left) {
+        chunk_close(ch, list);
+        uint32_t at = 0;
+        if (lane_id() == 0) at = atomicAdd(count, kRecChunk);
+        at = __shfl(at, 0);
+        if (at > cap - kRecChunk) { // cannot happen within the sizing above
+            if (lane_id() == 0) atomicOr(ovf, 1u);
+            return;
+        }
+        ch.at = at;
+        ch.left = kRecChunk;
+    }
+    if (take) list[ch.at + mask_rank(m)] = v;
+    ch.at += h;
+    ch.left -= h;
+}
+
 // Blocks of a worklist grid that take part at level k: one per
 // kPairsPerBlock pairs becoming eligible at the level, within
 // [kMinPersistent, launched grid] (more waves than the work needs only
@@ -739,8 +785,11 @@ __device__ __forceinline__ uint32_t level_blocks(const uint32_t *bstart, int k)
 __global__ void __launch_bounds__(kBlock)
     k_sweep(LevelArgs a, const PairRec *carried, const uint32_t *ncarried, const PairRec *rec,
             const uint32_t *bstart, uint32_t *parent, PairRec *keep, uint32_t *nkeep,
-            PairRec *near, uint32_t *nnear, PairRec *far, uint32_t *nfar)
+            PairRec *near, uint32_t *nnear, PairRec *far, uint32_t *nfar, uint32_t tcap,
+            uint32_t *ovf)
 {
+    RecChunk chn, chf;
+    bool chunked = bstart[a.k + 1] - bstart[a.k] >= kChunkedFrom;
     uint32_t nb = level_blocks(bstart, a.k);
     if (blockIdx.x >= nb) return;
     const PairRec *fresh = rec + bstart[a.k];
@@ -804,9 +853,16 @@ __global__ void __launch_bounds__(kBlock)
         if (uni && rb != m_l) cf_hook(parent, rb, m_l);
         if (lead && ra != m_l) cf_hook(parent, ra, m_l);
         wave_append_rec(kp, pr, keep, nkeep);
-        wave_append_rec(tnear, pr, near, nnear);
-        wave_append_rec(tfar, pr, far, nfar);
+        if (chunked) {
+            chunk_append(tnear, pr, near, nnear, tcap, ovf, chn);
+            chunk_append(tfar, pr, far, nfar, tcap, ovf, chf);
+        } else {
+            wave_append_rec(tnear, pr, near, nnear);
+            wave_append_rec(tfar, pr, far, nfar);
+        }
     }
+    chunk_close(chn, near);
+    chunk_close(chf, far);
 }
 
 // Wave per test pair: any core-core pair within eps_k connects the two cells.
@@ -816,6 +872,7 @@ __global__ void __launch_bounds__(kBlock)
 __device__ void test_pair(const LevelArgs &a, const PairRec &pr, uint32_t *parent,
                           PairRec *list_out, uint32_t *nout)
 {
+    if (pr.a == kNone) return; // unused slot of a test chunk
     uint32_t A = pr.a, B = pr.b;
     uint32_t na = a.cell_start[A + 1] - a.cell_start[A];
     uint32_t nb = a.cell_start[B + 1] - a.cell_start[B];
@@ -882,19 +939,45 @@ __device__ void test_pair(const LevelArgs &a, const PairRec &pr, uint32_t *paren
 // the near tests, then the far ones, in one grid-stride pass (waves take
 // them in order, so most far tests start after the near ones)
 __global__ void __launch_bounds__(kBlock)
-    k_test(LevelArgs a, const PairRec *near, const uint32_t *nnear,
+    k_test(LevelArgs a, const uint32_t *bstart, const PairRec *near, const uint32_t *nnear,
            const PairRec *far, const uint32_t *nfar, uint32_t *parent, PairRec *list_out,
-           uint32_t *nout)
+           uint32_t *nout, uint32_t tcap, const uint32_t *ovf)
 {
+    // a test list that ran out of room (reported to the host) is not read
+    if (*ovf) return;
     // tests are latency-bound waves: one block per kTestsPerBlock tests,
     // up to the launched grid
-    uint32_t nn = *nnear, nt = nn + *nfar;
+    uint32_t nn = *nnear, nf = *nfar;
+    nn = nn < tcap ? nn : tcap;
+    nf = nf < tcap ? nf : tcap;
+    uint32_t nt = nn + nf;
     uint32_t nb = (nt + kTestsPerBlock - 1) / kTestsPerBlock;
     nb = nb > kMinPersistent ? nb : kMinPersistent;
     nb = nb < gridDim.x ? nb : gridDim.x;
     if (blockIdx.x >= nb) return;
-    for (uint32_t w = blockIdx.x * kWPB + wave_id_in_block(); w < nt; w += nb * kWPB)
-        test_pair(a, w < nn ? near[w] : far[w - nn], parent, list_out, nout);
+    if (bstart[a.k + 1] - bstart[a.k] < kChunkedFrom) {
+        // exact lists: a record per wave
+        for (uint32_t w = blockIdx.x * kWPB + wave_id_in_block(); w < nt; w += nb * kWPB)
+            test_pair(a, w < nn ? near[w] : far[w - nn], parent, list_out, nout);
+        return;
+    }
+    // chunked lists: a wave takes 64 consecutive records at a time (one load
+    // each), skips the unused chunk slots with one ballot and tests the rest
+    // in turn
+    uint32_t ng = (nt + kWave - 1) / kWave;
+    for (uint32_t gi = blockIdx.x * kWPB + wave_id_in_block(); gi < ng; gi += nb * kWPB) {
+        uint32_t w = gi * kWave + lane_id();
+        PairRec r{kNone, kNone, 0};
+        if (w < nt) r = w < nn ? near[w] : far[w - nn];
+        uint64_t real = ballot(r.a != kNone);
+        while (real) {
+            int bit = __builtin_ctzll(real);
+            real &= real - 1;
+            PairRec pr{(uint32_t)__shfl((int)r.a, bit), (uint32_t)__shfl((int)r.b, bit),
+                       (uint32_t)__shfl((int)r.k, bit)};
+            test_pair(a, pr, parent, list_out, nout);
+        }
+    }
 }
 
 // ---------------------------------------------------------- snapshots
@@ -1350,8 +1433,20 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_TRY(c.get("b.rec", psz, &rec));
     VDB_TRY(c.get("b.la", psz, &la));
     VDB_TRY(c.get("b.lb", psz, &lb));
-    VDB_TRY(c.get("b.tests", psz, &tests));
-    VDB_TRY(c.get("b.tests2", psz, &tests2));
+    // test lists are filled in chunks (see kRecChunk): room for the records
+    // plus the chunks' unused slots
+    size_t tcap64 = psz + psz / 2 + (size_t)kMaxPersistent * kWPB * kRecChunk + kRecChunk;
+    if (tcap64 > 0xffffffffull) {
+        set_error("too many cell pairs");
+        return kErrUnsupported;
+    }
+    uint32_t tcap = (uint32_t)tcap64;
+    VDB_TRY(c.get("b.tests", tcap, &tests));
+    VDB_TRY(c.get("b.tests2", tcap, &tests2));
+    uint32_t *tovf;
+    VDB_TRY(c.get("b.tovf", 1, &tovf));
+    k_fill_u32<<<1, kBlock, 0, c.stream()>>>(tovf, 0u, 1);
+    VDB_HIP(hipGetLastError());
     // per level: carried-in list size, near and far test counts (level k
     // reads carried[k], writes carried[k+1])
     const uint32_t ncnt = 3 * ((uint32_t)K + 1);
@@ -1400,6 +1495,9 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_TRY(c.get("b.snap_pre", (size_t)K * nw, &snap_pre));
     Ev tuf(c.stream());
     double ms_labels = 0;
+    // label-pass events per j, read once the batch is done (reading them
+    // inside the loop would make the host wait once per minpts value)
+    std::vector<hipEvent_t> lab_ev;
     int jg0 = -1, jgn = 0; // minpts values whose keys ekey/eval hold
     for (int j = 0; j < J; j++) {
         int kmax = -1;
@@ -1451,11 +1549,11 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             uint32_t nfresh = bst[k + 1] - bst[k]; // verbose only
             k_sweep<<<kMaxPersistent, kBlock, 0, c.stream()>>>(
                 a, lin, c_keep + k, rec, bstart, parent, lout, c_keep + k + 1, tests, c_near + k,
-                tests2, c_far + k);
+                tests2, c_far + k, tcap, tovf);
             VDB_HIP(hipGetLastError());
-            k_test<<<kMaxTestBlocks, kBlock, 0, c.stream()>>>(a, tests, c_near + k, tests2,
+            k_test<<<kMaxTestBlocks, kBlock, 0, c.stream()>>>(a, bstart, tests, c_near + k, tests2,
                                                               c_far + k, parent, lout,
-                                                              c_keep + k + 1);
+                                                              c_keep + k + 1, tcap, tovf);
             VDB_HIP(hipGetLastError());
             if (verbose) {
                 // worklist sizes of this level (a sync per level: verbose only)
@@ -1490,7 +1588,12 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             VDB_HIP(hipGetLastError());
         }
         // labels of every requested variant of this j, kLabelSlots rows per launch
-        Ev tl(c.stream());
+        hipEvent_t le0, le1;
+        VDB_HIP(hipEventCreate(&le0));
+        VDB_HIP(hipEventCreate(&le1));
+        lab_ev.push_back(le0);
+        lab_ev.push_back(le1);
+        VDB_HIP(hipEventRecord(le0, c.stream()));
         std::vector<std::pair<int, uint32_t>> rows_j; // (slot, variant)
         for (size_t sl = 0; sl < snap_k.size(); sl++)
             for (uint32_t v : want[{snap_k[sl], j}]) rows_j.push_back({(int)sl, v});
@@ -1520,9 +1623,21 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             k_rows_from_pos<<<gn, kBlock, 0, c.stream()>>>(S, pos_of_caller, P.nsets, n);
             VDB_HIP(hipGetLastError());
         }
-        ms_labels += tl.ms();
+        VDB_HIP(hipEventRecord(le1, c.stream()));
     }
     double ms_uf_all = tuf.ms();
+    for (size_t e = 0; e + 1 < lab_ev.size(); e += 2) {
+        float m = 0;
+        VDB_HIP(hipEventElapsedTime(&m, lab_ev[e], lab_ev[e + 1]));
+        ms_labels += m;
+    }
+    for (hipEvent_t e : lab_ev) (void)hipEventDestroy(e);
+    uint32_t htovf = 0;
+    VDB_TRY(read_u32(c, tovf, &htovf));
+    if (htovf) {
+        set_error("cell-pair test list overflow");
+        return kErrDevice;
+    }
     if (st) {
         st->ms_uf += ms_uf_all - ms_labels;
         st->ms_labels += ms_labels;
