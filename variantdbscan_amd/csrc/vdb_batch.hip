@@ -245,7 +245,9 @@ __device__ __forceinline__ void wave_sort(double &es, uint32_t &eid, int cnt)
     }
 }
 
-__global__ void __launch_bounds__(kBlock)
+// 8 waves per SIMD: the search is latency-bound, so occupancy pays more
+// than the scalar registers it costs (their spills go to vector lanes)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
     k_knn(FG g, const uint32_t *__restrict__ list_pos, uint32_t nlist, int L, double Tmax,
           double T0, double *__restrict__ lst_s, uint32_t *__restrict__ lst_id,
           uint32_t *__restrict__ lst_n)
