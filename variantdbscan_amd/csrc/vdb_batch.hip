@@ -56,16 +56,15 @@ constexpr unsigned kMaxTestBlocks = 2048;
 
 struct FG {
     const double *xs, *ys, *box;
-    const uint32_t *cell_start, *cell_of_pos, *row_lo, *row_hi, *pos_rank;
+    const uint32_t *cell_start, *cell_of_pos, *row_lo, *row_hi;
     const uint32_t *cell_key, *cell_ds, *sncy;
     int W;
 };
 
 FG fview(const Grid &G)
 {
-    return FG{G.xs,       G.ys,       G.cell_box,   G.cell_start, G.cell_of_pos,
-              G.row_lo,   G.row_hi,   G.pos_rank,   G.cell_key,   G.cell_ds,
-              G.sncy,     2 * G.R + 1};
+    return FG{G.xs,     G.ys,       G.cell_box, G.cell_start, G.cell_of_pos, G.row_lo,
+              G.row_hi, G.cell_key, G.cell_ds,  G.sncy,       2 * G.R + 1};
 }
 
 __device__ __forceinline__ uint32_t ld_u32(const uint32_t *p)
@@ -547,12 +546,6 @@ __global__ void k_pairs(const double *box, const uint32_t *row_lo, const uint32_
     if (!WRITE) cnt[A] = k;
 }
 
-__global__ void k_iota(uint32_t *a, uint32_t n)
-{
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) a[i] = i;
-}
-
 // parent[i] = i, compmin[i] = NONE, and two arrays cleared (counters,
 // snapshot root bits)
 __global__ void k_init_parent(uint32_t *parent, uint32_t *compmin, uint32_t n, uint32_t *z1,
@@ -591,7 +584,6 @@ struct LevelArgs {
     const uint32_t *cell_start, *cell_key;
     const uint32_t *cell_ds, *skbase, *sncy; // dataset of a cell, its key base and ncy
     const uint8_t *kc;
-    const uint8_t *cellkc;
     const uint32_t *pa, *pb;
     const uint16_t *pk;
     int J, j, k;
@@ -1085,14 +1077,13 @@ __global__ void k_cell_labels(const uint32_t *rootrank, const uint32_t *bits,
 // Thread per cell-order position: the point's data are read in sequence, the
 // labels are scattered to caller order.
 __global__ void __launch_bounds__(kBlock)
-    k_labels_j(LabelSlots S, const uint32_t *caller_of_pos, const uint32_t *cell_of_pos,
+    k_labels_j(LabelSlots S, const uint32_t *cell_of_pos,
                const uint32_t *cell_ds, const uint32_t *pos_rank, const uint8_t *kc, int J, int j,
                const uint32_t *lst_idx, const double *lst_s, const uint32_t *lst_id,
                const uint32_t *lst_n, int L, uint32_t n)
 {
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= n) return;
-    uint32_t i = caller_of_pos[p];
     uint32_t r = pos_rank[p];
     uint32_t C = cell_of_pos[p];
     int kcp = kc[(size_t)p * J + j];
@@ -1170,7 +1161,6 @@ __global__ void __launch_bounds__(kBlock)
     uint4 *dst = (uint4 *)(S.tmp + (size_t)p * kLabelSlots);
     dst[0] = make_uint4(lbl[0], lbl[1], lbl[2], lbl[3]);
     dst[1] = make_uint4(lbl[4], lbl[5], lbl[6], lbl[7]);
-    (void)i;
 }
 
 // Label rows from position order to caller order: thread per caller index,
@@ -1544,9 +1534,8 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         PairRec *lin = la, *lout = lb;
         std::vector<int> snap_k;
         for (int k = 0; k <= kmax; k++) {
-            LevelArgs a{G.xs,  G.ys,  G.cell_box, G.cell_start, G.cell_key, G.cell_ds,
-                        G.skbase, G.sncy, kc,     cellkc,       pa,         pb,
-                        pk,    J,     j,          k,            Th[k]};
+            LevelArgs a{G.xs, G.ys, G.cell_box, G.cell_start, G.cell_key, G.cell_ds, G.skbase,
+                        G.sncy, kc, pa, pb, pk, J, j, k, Th[k]};
             // worklist sizes stay on the device: bounded grids stride over them
             uint32_t nfresh = bst[k + 1] - bst[k]; // verbose only
             k_sweep<<<kMaxPersistent, kBlock, 0, c.stream()>>>(
@@ -1618,7 +1607,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             S.nrows = nrows;
             S.seg = P.seg;
             S.tmp = labtmp;
-            k_labels_j<<<gn, kBlock, 0, c.stream()>>>(S, caller_of_pos, G.cell_of_pos, G.cell_ds,
+            k_labels_j<<<gn, kBlock, 0, c.stream()>>>(S, G.cell_of_pos, G.cell_ds,
                                                       G.pos_rank, kc, J, j, lst_idx, lst_s,
                                                       lst_id, lst_n, (int)L, n);
             VDB_HIP(hipGetLastError());
