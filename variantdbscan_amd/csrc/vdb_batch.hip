@@ -61,6 +61,16 @@ struct FG {
     int W;
 };
 
+// Truncated lists are stored in blocks of 64 lists, entry-major inside a
+// block: entry e of list i sits at ((i/64)*L + e)*64 + i%64. The passes that
+// read them walk 64 consecutive lists in step, so each entry step of a wave
+// reads one contiguous 64-entry run; a block's entries stay within 64*L
+// slots, so the list kernel's writes stay close together.
+__host__ __device__ __forceinline__ size_t lst_at(uint32_t i, uint32_t e, int L)
+{
+    return ((size_t)(i >> 6) * (uint32_t)L + e) * 64u + (i & 63u);
+}
+
 FG fview(const Grid &G)
 {
     return FG{G.xs,     G.ys,       G.cell_box, G.cell_start, G.cell_of_pos, G.row_lo,
@@ -431,8 +441,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 
     }
     if (appending) settle();
     if (lane < L) {
-        lst_s[(size_t)i * L + lane] = es;
-        lst_id[(size_t)i * L + lane] = eid;
+        lst_s[lst_at(i, lane, L)] = es;
+        lst_id[lst_at(i, lane, L)] = eid;
     }
     if (lane == 0) lst_n[i] = (uint32_t)cnt;
 }
@@ -449,7 +459,7 @@ __global__ void k_core_levels(const uint32_t *lst_idx, const double *lst_s,
     if (p >= n) return;
     uint32_t i = lst_idx[p];
     uint32_t m = i != kNone ? lst_n[i] : 0;
-    const double *s = lst_s + (size_t)(i != kNone ? i : 0) * L;
+    uint32_t il = i != kNone ? i : 0;
     for (int j = 0; j < J; j++) {
         uint32_t need = mj[j];
         int lev = K;
@@ -459,7 +469,7 @@ __global__ void k_core_levels(const uint32_t *lst_idx, const double *lst_s,
             } else if (need == 0) {
                 lev = 0;
             } else if (need <= m) {
-                double sv = s[need - 1];
+                double sv = lst_s[lst_at(il, need - 1, L)];
                 lev = 0;
                 while (lev < K && sv > Tk[lev]) lev++;
             }
@@ -1104,8 +1114,8 @@ __global__ void __launch_bounds__(kBlock)
         uint32_t li = lst_idx[p];
         if (li != kNone) {
             uint32_t m = lst_n[li];
-            const double *sv = lst_s + (size_t)li * L;
-            const uint32_t *iv = lst_id + (size_t)li * L;
+            const double *sv = lst_s + lst_at(li, 0, L); // entry e at sv[64 e]
+            const uint32_t *iv = lst_id + lst_at(li, 0, L);
             // four entries per step, their loads issued together (the list
             // is sorted, so the walk ends at the first entry beyond Tmax)
             constexpr int kU = 4;
@@ -1115,8 +1125,8 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
                 for (int u = 0; u < kU; u++) {
                     bool ok = e0 + u < m;
-                    se[u] = ok ? sv[e0 + u] : INFINITY;
-                    q[u] = ok ? iv[e0 + u] : 0u;
+                    se[u] = ok ? sv[(size_t)(e0 + u) * 64u] : INFINITY;
+                    q[u] = ok ? iv[(size_t)(e0 + u) * 64u] : 0u;
                 }
                 int kq[kU];
                 uint32_t cq[kU];
@@ -1366,7 +1376,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_HIP(hipGetLastError());
     double *lst_s;
     uint32_t *lst_id, *lst_n;
-    size_t lsz = (size_t)(nlist ? nlist : 1) * L;
+    size_t lsz = (size_t)(nlist ? (nlist + 63) / 64 * 64 : 64) * L; // whole blocks of 64 lists
     VDB_TRY(c.get("b.lst_s", lsz, &lst_s));
     VDB_TRY(c.get("b.lst_id", lsz, &lst_id));
     VDB_TRY(c.get("b.lst_n", nlist ? nlist : 1, &lst_n));
