@@ -484,9 +484,12 @@ __global__ void k_core_levels(const uint32_t *lst_idx, const double *lst_s,
 // over the cell's points keeps the lowest level seen so far, and a point that
 // lowers it is the first for every level it newly covers. Both tables are
 // stored level-major (cmr[(j*K + k)*ncell + c], cellkc[j*ncell + c]) so that
-// the per-level passes over all cells read them contiguously.
+// the per-level passes over all cells read them contiguously; cellkc also
+// goes out cell-major (cellkc_cm[c*J + j]) for the pass over cell pairs,
+// which reads both cells' levels for several minpts values at once.
 __global__ void k_cell_core(const uint32_t *cell_start, uint32_t ncell, const uint8_t *kc, int J,
-                            int K, const uint32_t *pos_rank, uint32_t *cmr, uint8_t *cellkc)
+                            int K, const uint32_t *pos_rank, uint32_t *cmr, uint8_t *cellkc,
+                            uint8_t *cellkc_cm)
 {
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= ncell) return;
@@ -503,6 +506,7 @@ __global__ void k_cell_core(const uint32_t *cell_start, uint32_t ncell, const ui
         }
         for (int k = 0; k < mn; k++) row[(size_t)k * ncell] = kNone;
         cellkc[(size_t)j * ncell + c] = (uint8_t)mn;
+        cellkc_cm[(size_t)c * J + j] = (uint8_t)mn; // cell-major copy for pair passes
     }
 }
 
@@ -626,7 +630,7 @@ __device__ bool small_pair_connected(const LevelArgs &a, uint32_t A, uint32_t B)
 // point distances are computed once for the whole group.
 constexpr int kEligGroup = 8;
 __global__ void k_pair_elig(const uint32_t *pa, const uint32_t *pb, const uint16_t *pk,
-                            const uint8_t *cellkc, uint32_t ncell, const uint32_t *cell_start,
+                            const uint8_t *cellkc_cm, const uint32_t *cell_start,
                             const double *xs, const double *ys, const uint8_t *kc,
                             const double *Tk, int J, int j0, int nj, int K, uint32_t npairs,
                             uint32_t *key, uint32_t *val)
@@ -644,8 +648,9 @@ __global__ void k_pair_elig(const uint32_t *pa, const uint32_t *pb, const uint16
     for (int g = 0; g < kEligGroup; g++) {
         e[g] = K;
         if (g < nj) {
-            const uint8_t *ckj = cellkc + (size_t)(j0 + g) * ncell;
-            int v = e0, ca = ckj[A], cb = ckj[B];
+            // a cell's levels for all minpts values lie side by side
+            int v = e0;
+            int ca = cellkc_cm[(size_t)A * J + j0 + g], cb = cellkc_cm[(size_t)B * J + j0 + g];
             v = v > ca ? v : ca;
             v = v > cb ? v : cb;
             e[g] = v;
@@ -1393,9 +1398,11 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_HIP(hipGetLastError());
     VDB_TRY(c.get("b.cmr", (size_t)G.ncell * J * K, &cmr));
     VDB_TRY(c.get("b.cellkc", (size_t)G.ncell * J, &cellkc));
+    uint8_t *cellkc_cm;
+    VDB_TRY(c.get("b.cellkc_cm", (size_t)G.ncell * J, &cellkc_cm));
     unsigned gc0 = ceil_div(G.ncell, kBlock);
     k_cell_core<<<gc0, kBlock, 0, c.stream()>>>(G.cell_start, G.ncell, kc, J, K, G.pos_rank, cmr,
-                                                cellkc);
+                                                cellkc, cellkc_cm);
     VDB_HIP(hipGetLastError());
     if (st) st->ms_count += tknn.ms();
 
@@ -1438,7 +1445,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     // test lists are filled in chunks (see kRecChunk): room for the records
     // plus the chunks' unused slots
     size_t tcap64 = psz + psz / 2 + (size_t)kMaxPersistent * kWPB * kRecChunk + kRecChunk;
-    if (tcap64 > 0xffffffffull) {
+    if (tcap64 > 0x7fffffffull) { // the two lists' counts add up in 32 bits
         set_error("too many cell pairs");
         return kErrUnsupported;
     }
@@ -1499,7 +1506,14 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     double ms_labels = 0;
     // label-pass events per j, read once the batch is done (reading them
     // inside the loop would make the host wait once per minpts value)
-    std::vector<hipEvent_t> lab_ev;
+    struct EventList {
+        std::vector<hipEvent_t> ev;
+        ~EventList()
+        {
+            for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        }
+    } lab;
+    std::vector<hipEvent_t> &lab_ev = lab.ev;
     int jg0 = -1, jgn = 0; // minpts values whose keys ekey/eval hold
     for (int j = 0; j < J; j++) {
         int kmax = -1;
@@ -1517,7 +1531,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
                 jg0 = j;
                 jgn = std::min(kEligGroup, J - j);
                 k_pair_elig<<<ceil_div(npairs, kBlock), kBlock, 0, c.stream()>>>(
-                    pa, pb, pk, cellkc, G.ncell, G.cell_start, G.xs, G.ys, kc, dTk, J, jg0, jgn,
+                    pa, pb, pk, cellkc_cm, G.cell_start, G.xs, G.ys, kc, dTk, J, jg0, jgn,
                     K, npairs, ekey, eval);
                 VDB_HIP(hipGetLastError());
             }
@@ -1591,8 +1605,8 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         // labels of every requested variant of this j, kLabelSlots rows per launch
         hipEvent_t le0, le1;
         VDB_HIP(hipEventCreate(&le0));
-        VDB_HIP(hipEventCreate(&le1));
         lab_ev.push_back(le0);
+        VDB_HIP(hipEventCreate(&le1));
         lab_ev.push_back(le1);
         VDB_HIP(hipEventRecord(le0, c.stream()));
         std::vector<std::pair<int, uint32_t>> rows_j; // (slot, variant)
@@ -1632,7 +1646,6 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         VDB_HIP(hipEventElapsedTime(&m, lab_ev[e], lab_ev[e + 1]));
         ms_labels += m;
     }
-    for (hipEvent_t e : lab_ev) (void)hipEventDestroy(e);
     uint32_t htovf = 0;
     VDB_TRY(read_u32(c, tovf, &htovf));
     if (htovf) {
