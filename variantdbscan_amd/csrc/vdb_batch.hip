@@ -227,7 +227,7 @@ __device__ __forceinline__ void knn_block(double s, uint32_t q, bool hit, int L,
 // per-cell pruning
 constexpr uint32_t kKnnRowSweep = 512;
 // windows holding at most this many points are swept as one flat list
-constexpr uint32_t kKnnFlat = 256;
+constexpr uint32_t kKnnFlat = 512;
 
 // sort the lane-resident entries (lane e holds entry e) by (s, id), ascending.
 // Only lanes below cnt hold entries (the rest hold (INF, NONE)), so the
