@@ -266,12 +266,49 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *a, uint32_t n, uint32_t v)
+// Smallest i in [0, n] with a[i] >= v (a ascending), searched outward from
+// hint with doubling steps, then by bisection: a window row's cells lie near
+// the centre cell in key order, so this reads O(log distance) keys instead
+// of O(log n).
+__device__ __forceinline__ uint32_t lower_bound_near(const uint32_t *a, uint32_t n, uint32_t v,
+                                                     uint32_t hint)
 {
-    uint32_t lo = 0, hi = n;
+    uint32_t lo, hi;
+    if (hint < n && a[hint] < v) {
+        // answer in (hint, n]
+        lo = hint + 1;
+        hi = n;
+        uint64_t off = 1;
+        while (true) {
+            uint64_t p = (uint64_t)hint + off;
+            if (p >= n) break;
+            if (a[p] >= v) {
+                hi = (uint32_t)p;
+                break;
+            }
+            lo = (uint32_t)p + 1;
+            off <<= 1;
+        }
+    } else {
+        // answer in [0, min(hint, n)]
+        hi = hint < n ? hint : n;
+        lo = 0;
+        uint32_t h = hi;
+        uint64_t off = 1;
+        while (off <= h) {
+            uint32_t p = h - (uint32_t)off;
+            if (a[p] < v) {
+                lo = p + 1;
+                break;
+            }
+            hi = p;
+            off <<= 1;
+        }
+    }
     while (lo < hi) {
-        uint32_t m = (lo + hi) >> 1;
-        if (a[m] < v) lo = m + 1; else hi = m;
+        uint32_t m = lo + ((hi - lo) >> 1);
+        if (a[m] < v) lo = m + 1;
+        else hi = m;
     }
     return lo;
 }
@@ -301,8 +338,8 @@ __global__ void k_row_ranges(const uint32_t *cell_key, uint32_t ncell, const uin
         int64_t y1 = cy + R > (int64_t)ncy - 1 ? (int64_t)ncy - 1 : cy + R;
         uint64_t k0 = kb + (uint64_t)(rx * ncy + y0);
         uint64_t k1 = kb + (uint64_t)(rx * ncy + y1);
-        lo = lower_bound_u32(cell_key, ncell, (uint32_t)k0);
-        hi = k1 >= 0xffffffffull ? ncell : lower_bound_u32(cell_key, ncell, (uint32_t)(k1 + 1));
+        lo = lower_bound_near(cell_key, ncell, (uint32_t)k0, c);
+        hi = k1 >= 0xffffffffull ? ncell : lower_bound_near(cell_key, ncell, (uint32_t)(k1 + 1), lo);
     }
     row_lo[t] = lo;
     row_hi[t] = hi;
