@@ -629,8 +629,8 @@ __global__ void __launch_bounds__(kBlock)
     for (int b = 0; b < kOrigins; b++) {
         uint32_t ob = o0 + (b < (int)nb ? b : nb - 1);
         bool isc = b < (int)nb && core[ob];
-        ox[b] = isc ? g.xs[ob] : __longlong_as_double(0x7ff8000000000000ll);
-        oy[b] = g.ys[ob];
+        ox[b] = sgpr_d(isc ? g.xs[ob] : __longlong_as_double(0x7ff8000000000000ll));
+        oy[b] = sgpr_d(g.ys[ob]);
         rp[b] = isc ? uf_find(parent, ob) : kNone;
         any |= isc;
     }
@@ -714,8 +714,8 @@ __global__ void __launch_bounds__(kBlock)
     for (int b = 0; b < kOrigins; b++) {
         uint32_t ob = o0 + (b < (int)nb ? b : nb - 1);
         bool nc = b < (int)nb && !core[ob];
-        ox[b] = nc ? g.xs[ob] : __longlong_as_double(0x7ff8000000000000ll);
-        oy[b] = g.ys[ob];
+        ox[b] = sgpr_d(nc ? g.xs[ob] : __longlong_as_double(0x7ff8000000000000ll));
+        oy[b] = sgpr_d(g.ys[ob]);
         bst[b] = kNone;
         any |= nc;
     }
