@@ -57,12 +57,29 @@ int libVDBSCAN(double *inputx, double *inputy, unsigned int datasetSize,
 #define VDB_PREC_FP64 0      /* every distance test in binary64 */
 #define VDB_PREC_FP32_RECHECK 1 /* float32 screening, binary64 inside the error band */
 
+/* Labelling of vdb_cluster / vdb_cluster_multi.
+ * VDB_BORDER_REFERENCE: the reference's labels (0 = noise, clusters 1.. in
+ *   its visit order, a border point joins a cluster only when that cluster
+ *   was found before the outer loop reached the point).
+ * VDB_BORDER_STANDARD: textbook DBSCAN labels as int32 values in the uint32
+ *   array: -1 (0xffffffff) = noise, clusters 0..k-1 numbered in ascending
+ *   order of their smallest core point's caller index, and every non-core
+ *   point within eps of a core point joins the lowest-numbered such cluster.
+ *   These are the labels of scikit-learn's DBSCAN for the same eps and
+ *   min_samples = minpts (it visits points in index order). Core points and
+ *   clusters are the same in both modes. No counterpart in the reference;
+ *   computed by the per-variant engines (VDB_PATH_AUTO runs fused,
+ *   VDB_PATH_BATCH is refused with VDB_EUNSUPPORTED). */
+#define VDB_BORDER_REFERENCE 0
+#define VDB_BORDER_STANDARD 1
+
 typedef struct vdb_options {
     int path;                   /* VDB_PATH_* */
     int precision;              /* VDB_PREC_* */
     uint64_t table_budget_bytes;/* 0 = most of the free device memory */
     int verbose;
     int cells_per_eps;          /* per-variant grid: cells per eps (1..4), 0 = library's choice */
+    int border;                 /* VDB_BORDER_* */
 } vdb_options;
 
 typedef struct vdb_stats {

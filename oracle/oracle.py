@@ -140,6 +140,46 @@ def libvdbscan(x, y, eps, minpts, mbbsize=70, closed=True):
     return ret.reshape(v, n)
 
 
+def standard_labels(x, y, eps, minpts):
+    """Textbook DBSCAN labels (int32, -1 = noise, clusters 0..) under the
+    reference's eps predicate, for the library's standard border rule.
+
+    A literal restatement of scikit-learn 1.7's ``dbscan_inner``
+    (sklearn/cluster/_dbscan_inner.pyx): visit points in index order; an
+    unlabelled core point starts a new cluster, grown depth-first; every
+    point reached gets the cluster, core points reached are expanded. The
+    neighbourhoods (self included) come from neighbor_csr, so only the
+    labelling rule differs from libvdbscan. Core: at least minpts
+    neighbours (minpts above INT_MAX: none, as in the reference's int).
+    Pure Python over the table: small inputs only.
+    """
+    off, ids = neighbor_csr(x, y, eps)
+    n = len(off) - 1
+    cnt = np.diff(off)
+    mp = int(minpts)
+    core = (cnt >= mp) if mp <= 0x7fffffff else np.zeros(n, dtype=bool)
+    lab = np.full(n, -1, dtype=np.int32)
+    ids = ids.tolist()
+    off = off.tolist()
+    core = core.tolist()
+    nxt = 0
+    for i in range(n):
+        if lab[i] != -1 or not core[i]:
+            continue
+        stack = [i]
+        while stack:
+            p = stack.pop()
+            if lab[p] != -1:
+                continue
+            lab[p] = nxt
+            if core[p]:
+                for q in ids[off[p]:off[p + 1]]:
+                    if lab[q] == -1:
+                        stack.append(q)
+        nxt += 1
+    return lab
+
+
 # ------------------------------------------------------- compiled reference
 
 def ref_lib_path(threads=1):

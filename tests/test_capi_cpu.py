@@ -94,3 +94,33 @@ def test_reference_package_names_resolve():
             assert m.VariantDBSCAN is variantdbscan_amd.VariantDBSCAN, mod
     finally:
         sys.path.remove(compat)
+
+
+def _cluster_rc(opt):
+    L = _lib.load()
+    x = np.array([0.0, 1.0, 2.0])
+    out = np.zeros(3, dtype=np.uint32)
+    eps = np.array([1.0])
+    mp = np.array([2], dtype=np.uint32)
+    return L.vdb_cluster(_lib._ptr(x), _lib._ptr(x), 3, _lib._ptr(eps), _lib._ptr(mp), 1,
+                         _lib._ptr(out), 0, ctypes.byref(opt), None)
+
+
+def test_options_checked_before_device_work():
+    # unknown values are refused, and the standard border rule on the
+    # batched path is refused as unsupported, before any device call
+    O = _lib.VdbOptions
+    assert _cluster_rc(O(7, 0, 0, 0, 0, 0)) == 1
+    assert _cluster_rc(O(0, 2, 0, 0, 0, 0)) == 1
+    assert _cluster_rc(O(0, 0, 0, 0, 0, 2)) == 1
+    assert "border" in _lib.last_error()
+    assert _cluster_rc(O(_lib.PATH_BATCH, 0, 0, 0, 0, _lib.BORDER_STANDARD)) == 5
+    assert "standard border" in _lib.last_error()
+
+
+def test_options_struct_matches_header():
+    # the ctypes mirror has the header's fields, in order
+    text = open(os.path.join(ROOT, "include", "vdbscan.h")).read()
+    body = re.search(r"typedef struct vdb_options \{(.*?)\} vdb_options;", text, re.S).group(1)
+    names = re.findall(r"\b([a-z_]+);", body)
+    assert names == [f for f, _ in _lib.VdbOptions._fields_]

@@ -57,6 +57,21 @@ def test_cli_variants_file_and_harness_text(tmp_path):
     assert got1 == l1.tolist() and got2 == l2.tolist()
 
 
+def test_cli_standard_border_labels(tmp_path):
+    # --standard: -1 = noise, clusters 0.., as scikit-learn's DBSCAN labels
+    import oracle
+    pts = os.path.join(GOLDEN, "test_dataset.csv")
+    out = tmp_path / "labels.csv"
+    run(["--points", pts, "--eps", "1.0,0.5", "--minpts", "10,12", "--standard", "--out",
+         str(out)])
+    d = np.loadtxt(str(out), delimiter=",")
+    src = np.loadtxt(pts, delimiter=",")
+    for v, (eps, mp) in enumerate([(1.0, 10), (0.5, 12)]):
+        want = oracle.standard_labels(src[:, 0], src[:, 1], eps, mp)
+        assert (d[:, 2 + v] == want).all(), v
+    assert (d[:, 2:] == -1).any()
+
+
 def test_reference_example_runs_unchanged():
     # src/vtest.py's body, importing `variantdbscan` as the reference's
     # examples do (integer coordinates and minpts, as there)

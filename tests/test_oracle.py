@@ -172,3 +172,61 @@ def test_oracle_against_compiled_reference_random():
         for eps, mp in [(0.2, 4), (0.5, 8), (1.5, 12)]:
             ref, _ = oracle.run_reference(x, y, [eps], [mp])
             assert (oracle.libvdbscan(x, y, [eps], [mp]) == ref).all()
+
+
+# ------------------------------------------------- standard border rule oracle
+
+def _sklearn_labels(x, y, eps, mp):
+    from sklearn.cluster import DBSCAN
+    X = np.column_stack([x, y])
+    return DBSCAN(eps=eps, min_samples=mp, algorithm="kd_tree").fit(X).labels_
+
+
+def test_standard_labels_hand_case():
+    # two chains of 4 core points (minpts 4, eps 1) with a border point 0.9
+    # from the end of each; caller order: chain Y, the border, chain X, a
+    # far point. The border joins the lower-numbered cluster (Y = 0); the
+    # far point is noise.
+    x = np.array([2.7, 3.0, 3.3, 3.6, 1.8, 0.0, 0.3, 0.6, 0.9, 50.0])
+    y = np.zeros_like(x)
+    lab = oracle.standard_labels(x, y, 1.0, 4)
+    assert lab.tolist() == [0, 0, 0, 0, 0, 1, 1, 1, 1, -1]
+    # the same chains with X first in caller order: the border joins X
+    perm = [5, 6, 7, 8, 4, 0, 1, 2, 3, 9]
+    lab = oracle.standard_labels(x[perm], y[perm], 1.0, 4)
+    assert lab.tolist() == [0, 0, 0, 0, 0, 1, 1, 1, 1, -1]
+    assert (oracle.standard_labels(x, y, 1.0, 2 ** 31) == -1).all()
+    assert (oracle.standard_labels(x, y, 1.0, 0) >= 0).all()
+
+
+def test_standard_labels_against_sklearn():
+    # scikit-learn's DBSCAN is the definition of the standard rule; on random
+    # data no pair sits within an ulp of eps, so its eps test (squared
+    # distance against eps^2) and the reference's agree
+    rng = np.random.default_rng(31)
+    for trial in range(4):
+        n = int(rng.integers(300, 3000))
+        x = np.concatenate([rng.normal(0, 0.4, n // 2), rng.uniform(-3, 3, n - n // 2)])
+        y = np.concatenate([rng.normal(0.5, 0.3, n // 2), rng.uniform(-3, 3, n - n // 2)])
+        for eps, mp in [(0.1, 3), (0.2, 8), (0.35, 20), (0.05, 1), (0.6, 60)]:
+            want = _sklearn_labels(x, y, eps, mp)
+            got = oracle.standard_labels(x, y, eps, mp)
+            assert (got == want).all(), (trial, eps, mp)
+
+
+def test_standard_and_reference_rules_share_clusters():
+    # same core points and clusters; a point the reference labels is labelled
+    # by the standard rule too, and only border points can differ in kind
+    rng = np.random.default_rng(32)
+    n = 2500
+    x = np.concatenate([rng.normal(0, 0.3, n // 2), rng.uniform(-3, 3, n - n // 2)])
+    y = np.concatenate([rng.normal(0, 0.3, n // 2), rng.uniform(-3, 3, n - n // 2)])
+    for eps, mp in [(0.1, 4), (0.2, 12), (0.3, 30)]:
+        ref = oracle.libvdbscan(x, y, [eps], [mp])[0]
+        std = oracle.standard_labels(x, y, eps, mp)
+        core = oracle.neighbor_counts(x, y, eps) >= mp
+        pairs = set(zip(ref[core].tolist(), std[core].tolist()))
+        assert len(pairs) == len(set(ref[core].tolist())) == len(set(std[core].tolist()))
+        assert (std[ref > 0] >= 0).all()
+        # the rules differ here: borders the reference leaves as noise
+        assert (std == -1).sum() < (ref == 0).sum()

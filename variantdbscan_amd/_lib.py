@@ -17,6 +17,7 @@ DEVICE_PTRS = 1
 
 PATH_AUTO, PATH_FUSED, PATH_TABLE, PATH_BATCH = 0, 1, 2, 3
 PREC_FP64, PREC_FP32_RECHECK = 0, 1
+BORDER_REFERENCE, BORDER_STANDARD = 0, 1
 
 EXPORTED = ("libVDBSCAN", "vdb_last_error", "vdb_version", "vdb_set_stream", "vdb_release",
             "vdb_cluster", "vdb_rank_order", "vdb_neighbor_counts", "vdb_neighbor_table",
@@ -27,7 +28,7 @@ EXPORTED = ("libVDBSCAN", "vdb_last_error", "vdb_version", "vdb_set_stream", "vd
 class VdbOptions(ctypes.Structure):
     _fields_ = [("path", ctypes.c_int), ("precision", ctypes.c_int),
                 ("table_budget_bytes", ctypes.c_uint64), ("verbose", ctypes.c_int),
-                ("cells_per_eps", ctypes.c_int)]
+                ("cells_per_eps", ctypes.c_int), ("border", ctypes.c_int)]
 
 
 class VdbStats(ctypes.Structure):
@@ -127,10 +128,13 @@ def _u32(a):
 
 
 def cluster(x, y, eps, minpts, path=PATH_AUTO, precision=PREC_FP64, verbose=False,
-            table_budget_bytes=0, return_stats=False, cells_per_eps=0):
-    """Labels (V, n) uint32 for host arrays; retArr layout of libVDBSCAN.
+            table_budget_bytes=0, return_stats=False, cells_per_eps=0, border=BORDER_REFERENCE):
+    """Labels (V, n) for host arrays; retArr layout of libVDBSCAN.
     cells_per_eps (1..4) sets the per-variant grid's cells per eps; 0 lets
-    the library choose."""
+    the library choose. border=BORDER_REFERENCE gives the reference's uint32
+    labels (0 = noise, clusters 1..); BORDER_STANDARD gives int32 labels with
+    textbook DBSCAN borders (-1 = noise, clusters 0.. by smallest core index,
+    scikit-learn's labelling)."""
     L = load()
     x, y = _f64(x), _f64(y)
     eps = _f64(np.atleast_1d(eps))
@@ -139,21 +143,24 @@ def cluster(x, y, eps, minpts, path=PATH_AUTO, precision=PREC_FP64, verbose=Fals
         raise ValueError("length mismatch")
     n, v = len(x), len(eps)
     out = np.zeros(n * v, dtype=np.uint32)
-    opt = VdbOptions(path, precision, table_budget_bytes, 1 if verbose else 0, cells_per_eps)
+    opt = VdbOptions(path, precision, table_budget_bytes, 1 if verbose else 0, cells_per_eps,
+                     border)
     st = VdbStats()
     _check(L.vdb_cluster(_ptr(x), _ptr(y), n, _ptr(eps), _ptr(mp), v, _ptr(out), 0,
                          ctypes.byref(opt), ctypes.byref(st)))
     lab = out.reshape(v, n)
+    if border == BORDER_STANDARD:
+        lab = lab.view(np.int32)
     return (lab, st.as_dict()) if return_stats else lab
 
 
 def cluster_device(x_ptr, y_ptr, n, eps, minpts, labels_ptr, path=PATH_AUTO,
-                   precision=PREC_FP64, stats=True, cells_per_eps=0):
+                   precision=PREC_FP64, stats=True, cells_per_eps=0, border=BORDER_REFERENCE):
     """Device-resident inputs/outputs (raw device pointers, e.g. tensor.data_ptr())."""
     L = load()
     eps = _f64(np.atleast_1d(eps))
     mp = _u32(np.atleast_1d(minpts))
-    opt = VdbOptions(path, precision, 0, 0, cells_per_eps)
+    opt = VdbOptions(path, precision, 0, 0, cells_per_eps, border)
     st = VdbStats()
     _check(L.vdb_cluster(ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), n, _ptr(eps),
                          _ptr(mp), len(eps), ctypes.c_void_p(labels_ptr), DEVICE_PTRS,
@@ -233,9 +240,11 @@ def digest_of_csr(offsets, ids):
     return int(z.sum(dtype=np.uint64))
 
 
-def cluster_multi(sets, eps, minpts, path=PATH_AUTO, precision=PREC_FP64, return_stats=False):
+def cluster_multi(sets, eps, minpts, path=PATH_AUTO, precision=PREC_FP64, return_stats=False,
+                  border=BORDER_REFERENCE):
     """Cluster a list of (x, y) datasets under the same variants in one call.
-    Returns a list of (V, n_d) uint32 label arrays."""
+    Returns a list of (V, n_d) label arrays (uint32; int32 with
+    BORDER_STANDARD)."""
     L = load()
     eps = _f64(np.atleast_1d(eps))
     mp = _u32(np.atleast_1d(minpts))
@@ -246,23 +255,25 @@ def cluster_multi(sets, eps, minpts, path=PATH_AUTO, precision=PREC_FP64, return
     y = _f64(np.concatenate([s[1] for s in sets])) if sets else np.zeros(0)
     V = len(eps)
     out = np.zeros(int(offs[-1]) * V, dtype=np.uint32)
-    opt = VdbOptions(path, precision, 0, 0)
+    opt = VdbOptions(path, precision, 0, 0, 0, border)
     st = VdbStats()
     _check(L.vdb_cluster_multi(_ptr(x), _ptr(y), _ptr(offs), len(sets), _ptr(eps), _ptr(mp), V,
                                _ptr(out), 0, ctypes.byref(opt), ctypes.byref(st)))
+    if border == BORDER_STANDARD:
+        out = out.view(np.int32)
     res = [out[int(offs[d]) * V:int(offs[d + 1]) * V].reshape(V, int(sizes[d]))
            for d in range(len(sets))]
     return (res, st.as_dict()) if return_stats else res
 
 
 def cluster_multi_device(x_ptr, y_ptr, offsets, eps, minpts, labels_ptr, path=PATH_AUTO,
-                         precision=PREC_FP64):
+                         precision=PREC_FP64, border=BORDER_REFERENCE):
     """Device-resident variant of cluster_multi (offsets is a host array)."""
     L = load()
     eps = _f64(np.atleast_1d(eps))
     mp = _u32(np.atleast_1d(minpts))
     offs = np.ascontiguousarray(offsets, dtype=np.uint64)
-    opt = VdbOptions(path, precision, 0, 0)
+    opt = VdbOptions(path, precision, 0, 0, 0, border)
     st = VdbStats()
     _check(L.vdb_cluster_multi(ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), _ptr(offs),
                                len(offs) - 1, _ptr(eps), _ptr(mp), len(eps),

@@ -74,7 +74,8 @@ size_t default_budget(Context &c, const vdb_options *opt)
 // per-variant paths: one grid per distinct eps, counts shared by its variants
 int run_per_variant(Context &c, Points &P, const double *eps, const uint32_t *minpts,
                     const std::vector<uint32_t> &which, uint32_t *labels_dev, int mode,
-                    int precision, int cells, size_t budget, vdb_stats *st, bool verbose)
+                    int border, int precision, int cells, size_t budget, vdb_stats *st,
+                    bool verbose)
 {
     // group by eps (NaN eps compare unequal to themselves: key them by bits)
     std::map<uint64_t, std::vector<uint32_t>> by_eps;
@@ -107,7 +108,7 @@ int run_per_variant(Context &c, Points &P, const double *eps, const uint32_t *mi
                    "largest %u points",
                    e, G.ncx, G.ncy, G.R, G.ncell, G.max_cell);
         for (uint32_t v : kv.second) {
-            VDB_TRY(cluster_variant(c, P, G, I, e, minpts[v], counts, mode, budget,
+            VDB_TRY(cluster_variant(c, P, G, I, e, minpts[v], counts, mode, border, budget,
                                     labels_dev + (size_t)v * P.n, st ? &pt : nullptr));
         }
     }
@@ -139,7 +140,10 @@ int cluster_points(Context &c, Points &P, const char *tag, const double *eps,
 {
     int path = opt ? opt->path : VDB_PATH_AUTO;
     int precision = opt ? opt->precision : VDB_PREC_FP64;
+    int border = opt ? opt->border : VDB_BORDER_REFERENCE;
     bool verbose = opt && opt->verbose;
+    // the batched engine's snapshots encode the reference's rule
+    if (border == VDB_BORDER_STANDARD && path == VDB_PATH_AUTO) path = VDB_PATH_FUSED;
     Timer tprep(c.stream());
     VDB_TRY(build_rank_order(c, P, tag));
     if (st) st->ms_prepare += tprep.ms();
@@ -213,10 +217,27 @@ int cluster_points(Context &c, Points &P, const char *tag, const double *eps,
         }
     } else {
         VDB_TRY(run_per_variant(c, P, eps, minpts, rest, out, path == VDB_PATH_TABLE ? 1 : 0,
-                                precision, cells_per_eps(opt), default_budget(c, opt), st,
-                                verbose));
+                                border, precision, cells_per_eps(opt), default_budget(c, opt),
+                                st, verbose));
     }
     *used_out = used;
+    return kOk;
+}
+
+// Options are checked before any device work.
+int check_options(const vdb_options *opt)
+{
+    if (!opt) return kOk;
+    if (opt->path < VDB_PATH_AUTO || opt->path > VDB_PATH_BATCH ||
+        opt->precision < VDB_PREC_FP64 || opt->precision > VDB_PREC_FP32_RECHECK ||
+        opt->border < VDB_BORDER_REFERENCE || opt->border > VDB_BORDER_STANDARD) {
+        set_error("unknown path, precision or border option");
+        return kErrInvalid;
+    }
+    if (opt->border == VDB_BORDER_STANDARD && opt->path == VDB_PATH_BATCH) {
+        set_error("the standard border rule runs on the per-variant paths (fused, table)");
+        return kErrUnsupported;
+    }
     return kOk;
 }
 
@@ -241,6 +262,7 @@ int cluster_impl(const double *x, const double *y, uint32_t n, const double *eps
         set_error("datasetSize*numVariants exceeds 2^32 labels");
         return kErrInvalid;
     }
+    VDB_TRY(check_options(opt));
     Context *cp = nullptr;
     VDB_TRY(get_context(&cp));
     Context &c = *cp;
@@ -319,6 +341,7 @@ int vdb_cluster_multi(const double *x, const double *y, const uint64_t *offsets,
         set_error("offsets[0] must be 0 and total points*variants below 2^32");
         return kErrInvalid;
     }
+    VDB_TRY(check_options(opt));
     if (ntot == 0) return kOk;
     Context *cp = nullptr;
     VDB_TRY(get_context(&cp));

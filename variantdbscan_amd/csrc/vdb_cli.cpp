@@ -5,7 +5,8 @@
 // the variants (an eps,minpts CSV or --eps/--minpts lists), clusters every
 // variant and writes the labels: x,y,label_0,...,label_{V-1} per point, or
 // with --harness the text the reference's harness prints
-// (src/c_test_prog.cpp:144-153).
+// (src/c_test_prog.cpp:144-153). --standard gives textbook DBSCAN labels
+// instead (-1 = noise, clusters 0.., every border point in a cluster).
 #include "vdbscan.h"
 
 #include <cstdio>
@@ -22,7 +23,7 @@ void usage()
             "usage: vdbscan --points FILE [--lonlat] (--variants FILE | --eps E1,E2,.. "
             "--minpts M1,M2,..)\n"
             "               [--out FILE] [--harness] [--path auto|fused|table|batch] "
-            "[--fp32] [--verbose]\n");
+            "[--fp32] [--standard] [--verbose]\n");
 }
 
 bool parse_list(const char *s, std::vector<double> &out)
@@ -38,6 +39,23 @@ bool parse_list(const char *s, std::vector<double> &out)
         else if (*p) return false;
     }
     return !out.empty();
+}
+
+// labels as signed values (the standard border rule's -1 = noise)
+int write_signed_csv(const char *path, const double *x, const double *y, uint32_t n,
+                     const uint32_t *labels, uint32_t nvar)
+{
+    FILE *f = strcmp(path, "-") == 0 ? stdout : fopen(path, "w");
+    if (!f) return 1;
+    for (uint32_t i = 0; i < n; i++) {
+        fprintf(f, "%.17g,%.17g", x[i], y[i]);
+        for (uint32_t v = 0; v < nvar; v++) fprintf(f, ",%d", (int32_t)labels[(size_t)v * n + i]);
+        fputc('\n', f);
+    }
+    int bad = ferror(f);
+    if (f != stdout) bad |= fclose(f);
+    else fflush(f);
+    return bad ? 1 : 0;
 }
 
 int fail(int rc)
@@ -65,6 +83,7 @@ int main(int argc, char **argv)
         else if (a == "--harness") harness = true;
         else if (a == "--verbose") verbose = true;
         else if (a == "--fp32") opt.precision = VDB_PREC_FP32_RECHECK;
+        else if (a == "--standard") opt.border = VDB_BORDER_STANDARD;
         else if (a == "--eps" || a == "--minpts") {
             const char *v = next();
             if (!v || !parse_list(v, a == "--eps" ? eps_l : mp_l)) {
@@ -123,11 +142,21 @@ int main(int argc, char **argv)
         }
         for (uint32_t v = 0; v < nv; v++) {
             fprintf(f, "\n\nVariant: %u, Epsilon: %f, minpts: %u", v, eps[v], mp[v]);
-            for (uint32_t j = 0; j < n; j++)
-                fprintf(f, "\n%u, %f, %f, %u", j, x[j], y[j], labels[(size_t)v * n + j]);
+            for (uint32_t j = 0; j < n; j++) {
+                uint32_t l = labels[(size_t)v * n + j];
+                if (opt.border == VDB_BORDER_STANDARD)
+                    fprintf(f, "\n%u, %f, %f, %d", j, x[j], y[j], (int32_t)l);
+                else
+                    fprintf(f, "\n%u, %f, %f, %u", j, x[j], y[j], l);
+            }
         }
         fprintf(f, "\n");
         if (f != stdout) fclose(f);
+    } else if (opt.border == VDB_BORDER_STANDARD) {
+        if (write_signed_csv(out, x, y, n, labels.data(), nv)) {
+            fprintf(stderr, "vdbscan: cannot write %s\n", out);
+            return 1;
+        }
     } else {
         rc = vdb_write_labels_csv(out, x, y, n, labels.data(), nv);
         if (rc) return fail(rc);

@@ -59,10 +59,14 @@ int count_neighbors(Context &c, const Grid &G, const Items &I, double eps, int p
 
 // Full per-variant clustering; labels written in caller order to out (device).
 // mode 0 = fused (no table), 1 = CSR table (count/scan/fill, union-find over
-// the table), tiled when the table exceeds table_budget bytes.
+// the table), tiled when the table exceeds table_budget bytes. border =
+// VDB_BORDER_REFERENCE gives the labels above; VDB_BORDER_STANDARD numbers
+// the same clusters 0.. by their smallest core caller index, gives every
+// non-core point with a core neighbour the lowest-numbered adjacent cluster
+// and marks noise 0xffffffff.
 int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, double eps,
-                    uint32_t minpts, const uint32_t *counts, int mode, size_t table_budget,
-                    uint32_t *out, PhaseTimes *pt);
+                    uint32_t minpts, const uint32_t *counts, int mode, int border,
+                    size_t table_budget, uint32_t *out, PhaseTimes *pt);
 
 // Scatter a per-position array to caller order.
 int pos_to_caller(Context &c, const Points &P, const Grid &G, const uint32_t *by_pos,

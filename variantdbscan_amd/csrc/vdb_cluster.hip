@@ -695,7 +695,8 @@ __global__ void k_compress(uint32_t *parent, const uint8_t *core, uint32_t n)
     st_par(parent + p, uf_root(parent, p));
 }
 
-// border: best[p] = smallest root rank among core neighbours of non-core p
+// border: best[p] = smallest rank[root] among core neighbours of non-core p
+// (rank is the binned rank, or any per-root key the caller orders roots by)
 __global__ void __launch_bounds__(kBlock)
     k_border_fused(GridView g, const uint32_t *__restrict__ item_cell,
                    const uint32_t *__restrict__ item_o0, uint32_t nitems, double T,
@@ -835,6 +836,66 @@ __global__ void k_labels(const uint8_t *core, const uint32_t *parent, const uint
         lbl = (b != kNone && b < r) ? cid_by_rank[b] + 1 : 0u;
     }
     out[mapping[r]] = lbl;
+}
+
+// Standard border rule. key[root] = smallest caller index among the core
+// points of root's component (key holds NONE on entry; parent is flat). Lanes
+// of a wave that share the first lane's root reduce first, so a large
+// cluster takes one atomic per wave; the remaining lanes update theirs one
+// atomic each.
+__global__ void k_root_keys(const uint8_t *core, const uint32_t *parent, const uint32_t *rank,
+                            const uint32_t *mapping, uint32_t n, uint32_t *key)
+{
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    bool valid = p < n && core[p];
+    uint32_t root = valid ? parent[p] : kNone;
+    uint32_t v = valid ? mapping[rank[p]] : kNone;
+    uint64_t todo = ballot(valid);
+    if (todo) {
+        int leader = __builtin_ctzll(todo);
+        uint32_t r = __shfl(root, leader);
+        bool same = valid && root == r;
+        todo &= ~ballot(same);
+        uint32_t m = same ? v : kNone;
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            uint32_t t = __shfl_xor(m, o);
+            m = t < m ? t : m;
+        }
+        if (lane_id() == leader) atomicMin(key + r, m);
+    }
+    if ((todo >> lane_id()) & 1ull) atomicMin(key + root, v);
+}
+
+// flag_by_caller[i] = 1 when the point of caller index i is the smallest
+// core point (by caller index) of its component; every entry is written
+__global__ void k_key_flags(const uint8_t *core, const uint32_t *parent, const uint32_t *rank,
+                            const uint32_t *mapping, const uint32_t *key, uint32_t n,
+                            uint32_t *flag_by_caller)
+{
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    uint32_t ci = mapping[rank[p]];
+    flag_by_caller[ci] = (core[p] && key[parent[p]] == ci) ? 1u : 0u;
+}
+
+// Labels of the standard border rule: clusters 0.. in ascending key, noise
+// 0xffffffff (-1 as int32); best[p] is the smallest key among the core
+// neighbours of non-core p.
+__global__ void k_key_labels(const uint8_t *core, const uint32_t *parent, const uint32_t *rank,
+                             const uint32_t *key, const uint32_t *best,
+                             const uint32_t *cid_by_caller, const uint32_t *mapping, uint32_t n,
+                             uint32_t *out)
+{
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    uint32_t lbl;
+    if (core[p]) {
+        lbl = cid_by_caller[key[parent[p]]];
+    } else {
+        uint32_t b = best[p];
+        lbl = b != kNone ? cid_by_caller[b] : kNone;
+    }
+    out[mapping[rank[p]]] = lbl;
 }
 
 __global__ void k_perm_counts(const uint32_t *cnt, const uint32_t *pos_rank,
@@ -1001,13 +1062,25 @@ int count_neighbors(Context &c, const Grid &G, const Items &I, double eps, int p
 }
 
 static int finish_labels(Context &c, const Points &P, const Grid &G, const uint8_t *core,
-                         const uint32_t *parent, const uint32_t *best, uint32_t *out)
+                         const uint32_t *parent, const uint32_t *best, const uint32_t *key,
+                         uint32_t *out)
 {
     uint32_t n = P.n;
     unsigned g = ceil_div(n, kBlock);
     uint32_t *flag, *cid;
     VDB_TRY(c.get("lab.flag", n, &flag));
     VDB_TRY(c.get("lab.cid", n, &cid));
+    if (key) {
+        // standard border rule: clusters numbered by smallest core caller index
+        k_key_flags<<<g, kBlock, 0, c.stream()>>>(core, parent, G.pos_rank, P.mapping, key, n,
+                                                  flag);
+        VDB_HIP(hipGetLastError());
+        VDB_TRY(scan_exclusive_u32(c, flag, cid, n, nullptr));
+        k_key_labels<<<g, kBlock, 0, c.stream()>>>(core, parent, G.pos_rank, key, best, cid,
+                                                   P.mapping, n, out);
+        VDB_HIP(hipGetLastError());
+        return kOk;
+    }
     k_root_flags<<<g, kBlock, 0, c.stream()>>>(core, parent, G.pos_rank, n, flag);
     VDB_HIP(hipGetLastError());
     VDB_TRY(scan_exclusive_u32(c, flag, cid, n, nullptr));
@@ -1018,17 +1091,29 @@ static int finish_labels(Context &c, const Points &P, const Grid &G, const uint8
 }
 
 int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, double eps,
-                    uint32_t minpts, const uint32_t *counts, int mode, size_t table_budget,
-                    uint32_t *out, PhaseTimes *pt)
+                    uint32_t minpts, const uint32_t *counts, int mode, int border,
+                    size_t table_budget, uint32_t *out, PhaseTimes *pt)
 {
     uint32_t n = P.n;
     double T = threshold_for_eps(eps);
     unsigned g = ceil_div(n, kBlock);
     uint8_t *core;
-    uint32_t *parent, *best;
+    uint32_t *parent, *best, *key = nullptr;
     VDB_TRY(c.get("cv.core", n, &core));
     VDB_TRY(c.get("cv.parent", n, &parent));
     VDB_TRY(c.get("cv.best", n, &best));
+    if (border == VDB_BORDER_STANDARD) VDB_TRY(c.get("cv.key", n, &key));
+    // Border kernels take the minimum of bkey[root] over a point's core
+    // neighbours: the root's rank (reference rule), or its component's
+    // smallest core caller index (standard rule), set once the forest is flat.
+    auto root_keys = [&]() -> int {
+        if (!key) return kOk;
+        VDB_HIP(hipMemsetAsync(key, 0xff, sizeof(uint32_t) * n, c.stream()));
+        k_root_keys<<<g, kBlock, 0, c.stream()>>>(core, parent, G.pos_rank, P.mapping, n, key);
+        VDB_HIP(hipGetLastError());
+        return kOk;
+    };
+    const uint32_t *bkey = key ? key : G.pos_rank;
     EvTimer tm(c.stream());
     k_core_init<<<g, kBlock, 0, c.stream()>>>(counts, n, minpts, core, parent, best);
     VDB_HIP(hipGetLastError());
@@ -1041,10 +1126,11 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
         VDB_HIP(hipGetLastError());
         k_compress<<<g, kBlock, 0, c.stream()>>>(parent, core, n);
         VDB_HIP(hipGetLastError());
+        VDB_TRY(root_keys());
         if (pt) pt->uf += tm.stop(), tm.start();
         if (I.n)
             k_border_fused<<<blocks, kBlock, 0, c.stream()>>>(view(G, G.npts), I.cell, I.o0, I.n, T,
-                                                              core, G.pos_rank, parent, best);
+                                                              core, bkey, parent, best);
         VDB_HIP(hipGetLastError());
         if (pt) pt->border += tm.stop();
     } else {
@@ -1113,6 +1199,7 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
         if (pt) tm.start();
         k_compress<<<g, kBlock, 0, c.stream()>>>(parent, core, n);
         VDB_HIP(hipGetLastError());
+        VDB_TRY(root_keys());
         if (pt) pt->uf += tm.stop(), tm.start();
         // border rows are short (fewer than minpts entries): re-read them tile by tile
         i0 = 0;
@@ -1134,7 +1221,7 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
                 VDB_HIP(hipGetLastError());
             }
             k_border_table<<<ceil_div(pend - p0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
-                offsets, base, ids, p0, pend, core, G.pos_rank, parent, best);
+                offsets, base, ids, p0, pend, core, bkey, parent, best);
             VDB_HIP(hipGetLastError());
             i0 = i1;
         }
@@ -1148,7 +1235,7 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
         }
     }
     if (pt) tm.start();
-    VDB_TRY(finish_labels(c, P, G, core, parent, best, out));
+    VDB_TRY(finish_labels(c, P, G, core, parent, best, key, out));
     if (pt) pt->labels += tm.stop();
     return kOk;
 }
