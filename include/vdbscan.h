@@ -68,8 +68,7 @@ int libVDBSCAN(double *inputx, double *inputy, unsigned int datasetSize,
  *   These are the labels of scikit-learn's DBSCAN for the same eps and
  *   min_samples = minpts (it visits points in index order). Core points and
  *   clusters are the same in both modes. No counterpart in the reference;
- *   computed by the per-variant engines (VDB_PATH_AUTO runs fused,
- *   VDB_PATH_BATCH is refused with VDB_EUNSUPPORTED). */
+ *   every path computes it. */
 #define VDB_BORDER_REFERENCE 0
 #define VDB_BORDER_STANDARD 1
 

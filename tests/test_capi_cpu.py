@@ -107,15 +107,12 @@ def _cluster_rc(opt):
 
 
 def test_options_checked_before_device_work():
-    # unknown values are refused, and the standard border rule on the
-    # batched path is refused as unsupported, before any device call
+    # unknown values are refused before any device call
     O = _lib.VdbOptions
     assert _cluster_rc(O(7, 0, 0, 0, 0, 0)) == 1
     assert _cluster_rc(O(0, 2, 0, 0, 0, 0)) == 1
     assert _cluster_rc(O(0, 0, 0, 0, 0, 2)) == 1
     assert "border" in _lib.last_error()
-    assert _cluster_rc(O(_lib.PATH_BATCH, 0, 0, 0, 0, _lib.BORDER_STANDARD)) == 5
-    assert "standard border" in _lib.last_error()
 
 
 def test_options_struct_matches_header():

@@ -1,5 +1,5 @@
-"""Standard border rule (VDB_BORDER_STANDARD): textbook DBSCAN labels from the
-per-variant engines, checked against scikit-learn's DBSCAN on random data and
+"""Standard border rule (VDB_BORDER_STANDARD): textbook DBSCAN labels from
+every engine, checked against scikit-learn's DBSCAN on random data and
 against the oracle's restatement of its labelling (over the reference's exact
 eps predicate) on the ULP-adversarial fixtures."""
 import os
@@ -17,7 +17,7 @@ from variantdbscan_amd import datasets
 pytestmark = pytest.mark.gpu
 
 STD = vd.BORDER_STANDARD
-PATHS = [vd.PATH_FUSED, vd.PATH_TABLE]
+PATHS = [vd.PATH_FUSED, vd.PATH_TABLE, vd.PATH_BATCH]
 PRECS = [vd.PREC_FP64, vd.PREC_FP32_RECHECK]
 
 
@@ -47,8 +47,10 @@ def test_standard_against_sklearn(aniso40k):
     for path in PATHS:
         for prec in PRECS:
             say("aniso 40k standard border, path", path, "prec", prec)
-            got = vd.cluster(x, y, eps, mp, path=path, precision=prec, border=STD)
+            got, st = vd.cluster(x, y, eps, mp, path=path, precision=prec, border=STD,
+                                 return_stats=True)
             assert got.dtype == np.int32
+            assert st["path_used"] == path
             for v in range(len(eps)):
                 assert (got[v] == want[v]).all(), (path, prec, v)
 
@@ -57,11 +59,11 @@ def test_standard_auto_path_and_cells(aniso40k):
     x, y, eps, mp, want = aniso40k
     say("aniso 40k standard border, auto path")
     got, st = vd.cluster(x, y, eps, mp, border=STD, return_stats=True)
-    assert st["path_used"] == vd.PATH_FUSED
+    assert st["path_used"] == vd.PATH_BATCH
     assert (got == want).all()
     for k in (1, 4):
         say("aniso 40k standard border, cells_per_eps", k)
-        got = vd.cluster(x, y, eps[:3], mp[:3], border=STD, cells_per_eps=k)
+        got = vd.cluster(x, y, eps[:3], mp[:3], path=vd.PATH_FUSED, border=STD, cells_per_eps=k)
         assert (got == want[:3]).all(), k
 
 
@@ -70,7 +72,7 @@ def test_standard_same_clusters_as_reference_rule(aniso40k):
     # rule labels every point the reference rule labels, and more borders
     x, y, eps, mp, _ = aniso40k
     say("aniso 40k both rules")
-    ref = vd.cluster(x, y, eps, mp, path=vd.PATH_FUSED)
+    ref = vd.cluster(x, y, eps, mp)
     std = vd.cluster(x, y, eps, mp, border=STD)
     more = 0
     for v in range(len(eps)):
@@ -128,7 +130,7 @@ def test_standard_multi_datasets():
                     (rng.normal(rng.uniform(-60, 60), 1.0, n), rng.normal(rng.uniform(-60, 60), 1.0, n)))
     eps = np.array([0.25, 0.5, 0.5])
     mp = np.array([4, 8, 30], dtype=np.uint32)
-    for path in (vd.PATH_AUTO, vd.PATH_TABLE):
+    for path in (vd.PATH_AUTO, vd.PATH_FUSED, vd.PATH_TABLE):
         say("multi datasets standard border, path", path)
         got = vd.cluster_multi(sets, eps, mp, path=path, border=STD)
         for d, (x, y) in enumerate(sets):
@@ -137,7 +139,19 @@ def test_standard_multi_datasets():
                 assert (got[d][v] == sklearn_labels(x, y, eps[v], mp[v])).all(), (path, d, v)
 
 
-def test_standard_refused_on_batch_path():
-    x, y = datasets.plain_10k(seed=2)
-    with pytest.raises(vd._lib.VdbError):
-        vd.cluster(x, y, [0.5], [4], path=vd.PATH_BATCH, border=STD)
+def test_standard_batched_64_variants():
+    # the batched engine on 64 variants (eps 0.2..0.9 x minpts 4..48) of 30k
+    # anisotropic points, with eps < 0 and NaN variants (no neighbours) mixed in
+    x, y = datasets.aniso(30_000, seed=23)
+    E, M = datasets.BATCH64_EPS, datasets.BATCH64_MINPTS
+    eps = np.array([e for e in E for m in M] + [-1.0, np.nan])
+    mp = np.array([m for e in E for m in M] + [0, 3], dtype=np.uint32)
+    say("aniso 30k x 66 variants standard border, batch")
+    got, st = vd.cluster(x, y, eps, mp, path=vd.PATH_BATCH, border=STD, return_stats=True)
+    assert st["path_used"] == vd.PATH_BATCH
+    for v in range(64):
+        assert (got[v] == sklearn_labels(x, y, eps[v], mp[v])).all(), v
+    assert (got[64] == np.arange(len(x))).all()   # every point its own cluster
+    assert (got[65] == -1).all()
+    say("aniso 30k x 66 variants standard border, fused")
+    assert (vd.cluster(x, y, eps, mp, path=vd.PATH_FUSED, border=STD) == got).all()

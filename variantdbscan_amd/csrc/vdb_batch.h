@@ -18,7 +18,8 @@ struct BatchResult {
 };
 
 // Variant v of dataset d is written to out + seg[d]*nrows + rows[v]*n_d (for
-// one dataset: out + rows[v]*n). Every eps must be finite and > 0
+// one dataset: out + rows[v]*n), labelled by the border rule given
+// (VDB_BORDER_REFERENCE or VDB_BORDER_STANDARD). Every eps must be finite and > 0
 // or have no neighbour at all (eps < 0, NaN), every minpts <= 64 or above
 // INT_MAX, and max eps / min eps <= kBatchEpsRatio. Returns kErrUnsupported
 // when the data defeat the engine (cells that are not cliques, too many
@@ -28,6 +29,6 @@ constexpr uint32_t kBatchMaxMinpts = 64;
 bool batchable(double eps, uint32_t minpts);
 int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t *minpts,
                   const uint32_t *rows, uint32_t nvar, uint32_t *out, uint32_t nrows,
-                  vdb_stats *st, bool verbose, BatchResult *res);
+                  int border, vdb_stats *st, bool verbose, BatchResult *res);
 
 } // namespace vdb

@@ -478,17 +478,20 @@ __global__ void k_core_levels(const uint32_t *lst_idx, const double *lst_s,
     }
 }
 
-// per cell, per j, per k: smallest rank of a point with kc_j <= k (cells list
-// their points in ascending rank, so it is the first such point), and the
-// first level at which the cell holds a core point. Thread per cell: one pass
-// over the cell's points keeps the lowest level seen so far, and a point that
-// lowers it is the first for every level it newly covers. Both tables are
-// stored level-major (cmr[(j*K + k)*ncell + c], cellkc[j*ncell + c]) so that
-// the per-level passes over all cells read them contiguously; cellkc also
-// goes out cell-major (cellkc_cm[c*J + j]) for the pass over cell pairs,
-// which reads both cells' levels for several minpts values at once.
+// per cell, per j, per k: smallest key of a point with kc_j <= k, and the
+// first level at which the cell holds a core point. The key is the binned
+// rank (reference rule: cells list their points in ascending rank, so it is
+// the first such point) or the caller index (standard rule: the smallest over
+// the cell's points). Thread per cell: one pass over the cell's points keeps
+// the lowest level seen so far, and a point that lowers it is the first for
+// every level it newly covers. Both tables are stored level-major
+// (cmr[(j*K + k)*ncell + c], cellkc[j*ncell + c]) so that the per-level
+// passes over all cells read them contiguously; cellkc also goes out
+// cell-major (cellkc_cm[c*J + j]) for the pass over cell pairs, which reads
+// both cells' levels for several minpts values at once.
+template <bool STD>
 __global__ void k_cell_core(const uint32_t *cell_start, uint32_t ncell, const uint8_t *kc, int J,
-                            int K, const uint32_t *pos_rank, uint32_t *cmr, uint8_t *cellkc,
+                            int K, const uint32_t *key, uint32_t *cmr, uint8_t *cellkc,
                             uint8_t *cellkc_cm)
 {
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
@@ -497,12 +500,27 @@ __global__ void k_cell_core(const uint32_t *cell_start, uint32_t ncell, const ui
     for (int j = 0; j < J; j++) {
         uint32_t *row = cmr + (size_t)j * K * ncell + c;
         int mn = K; // levels [mn, K) have their first point
-        for (uint32_t q = a; q < e && mn > 0; q++) {
-            int lv = kc[(size_t)q * J + j];
-            if (lv >= mn) continue;
-            uint32_t r = pos_rank[q];
-            for (int k = lv; k < mn; k++) row[(size_t)k * ncell] = r;
-            mn = lv;
+        if (STD) {
+            for (uint32_t q = a; q < e; q++) {
+                int lv = kc[(size_t)q * J + j];
+                mn = lv < mn ? lv : mn;
+            }
+            for (int k = mn; k < K; k++) {
+                uint32_t m = kNone;
+                for (uint32_t q = a; q < e; q++) {
+                    uint32_t kq = key[q];
+                    if (kc[(size_t)q * J + j] <= k && kq < m) m = kq;
+                }
+                row[(size_t)k * ncell] = m;
+            }
+        } else {
+            for (uint32_t q = a; q < e && mn > 0; q++) {
+                int lv = kc[(size_t)q * J + j];
+                if (lv >= mn) continue;
+                uint32_t r = key[q];
+                for (int k = lv; k < mn; k++) row[(size_t)k * ncell] = r;
+                mn = lv;
+            }
         }
         for (int k = 0; k < mn; k++) row[(size_t)k * ncell] = kNone;
         cellkc[(size_t)j * ncell + c] = (uint8_t)mn;
@@ -1073,12 +1091,13 @@ __device__ __forceinline__ uint32_t roots_below(const uint32_t *bits, const uint
     return pre[w] + (uint32_t)__popc(bits[w] & ((1u << (r & 31)) - 1u));
 }
 
-// label of a cell's core points at a snapshot: 1 + cluster id of its root,
-// counted from the first root of the cell's dataset
+// label of a cell's core points at a snapshot: base + cluster id of its
+// root, counted from the first root of the cell's dataset (base 1: the
+// reference's numbering; 0: the standard rule's)
 // (also clears compmin for the next snapshot: it is no longer read)
 __global__ void k_cell_labels(const uint32_t *rootrank, const uint32_t *bits,
                               const uint32_t *pre, const uint32_t *cell_ds, const uint32_t *seg,
-                              uint32_t ncell, uint32_t *celllab, uint32_t *compmin)
+                              uint32_t ncell, uint32_t base, uint32_t *celllab, uint32_t *compmin)
 {
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= ncell) return;
@@ -1086,11 +1105,14 @@ __global__ void k_cell_labels(const uint32_t *rootrank, const uint32_t *bits,
     uint32_t rr = rootrank[c];
     celllab[c] = rr == kNone ? 0u
                              : roots_below(bits, pre, rr) -
-                                   roots_below(bits, pre, seg[cell_ds[c]]) + 1;
+                                   roots_below(bits, pre, seg[cell_ds[c]]) + base;
 }
 
 // Thread per cell-order position: the point's data are read in sequence, the
-// labels are scattered to caller order.
+// labels are scattered to caller order. STD: snapshots are keyed by caller
+// index, a non-core point with a core neighbour always joins the smallest
+// such root, clusters count from 0 and noise is NONE.
+template <bool STD>
 __global__ void __launch_bounds__(kBlock)
     k_labels_j(LabelSlots S, const uint32_t *cell_of_pos,
                const uint32_t *cell_ds, const uint32_t *pos_rank, const uint8_t *kc, int J, int j,
@@ -1164,6 +1186,11 @@ __global__ void __launch_bounds__(kBlock)
         if (t >= S.n) continue;
         if (kcp <= S.k[t]) {
             lbl[t] = S.celllab[t][C];
+        } else if (STD) {
+            uint32_t b = best[t];
+            lbl[t] = b != kNone ? roots_below(S.cbits[t], S.cpre[t], b) -
+                                      roots_below(S.cbits[t], S.cpre[t], s0)
+                                : kNone;
         } else {
             uint32_t b = best[t];
             lbl[t] = (b != kNone && b < r) ? roots_below(S.cbits[t], S.cpre[t], b) -
@@ -1223,17 +1250,19 @@ __global__ void k_fill_u32(uint32_t *a, uint32_t v, uint32_t n)
 
 // variants with no neighbour at all (eps < 0 or NaN): core iff minpts == 0,
 // and then every point is its own cluster, numbered in its dataset's binned
-// order; the row layout is that of LabelSlots
+// order (standard rule: caller order from 0; noise NONE); the row layout is
+// that of LabelSlots
 __global__ void k_isolated_labels(const uint32_t *mapping, const uint32_t *seg, uint32_t nsets,
-                                  uint32_t n, bool all_core, uint32_t *out, uint32_t nrows,
-                                  uint32_t row)
+                                  uint32_t n, bool all_core, bool std_rule, uint32_t *out,
+                                  uint32_t nrows, uint32_t row)
 {
     uint32_t r = blockIdx.x * kBlock + threadIdx.x;
     if (r >= n) return;
     uint32_t ds = seg_of(seg, nsets, r);
     uint32_t s0 = seg[ds], nd = seg[ds + 1] - s0;
     uint32_t i = mapping[r];
-    out[(size_t)s0 * nrows + (size_t)row * nd + (i - s0)] = all_core ? r - s0 + 1 : 0u;
+    uint32_t lbl = std_rule ? (all_core ? i - s0 : kNone) : (all_core ? r - s0 + 1 : 0u);
+    out[(size_t)s0 * nrows + (size_t)row * nd + (i - s0)] = lbl;
 }
 
 struct Ev {
@@ -1281,8 +1310,9 @@ bool batchable(double eps, uint32_t minpts)
 
 int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t *minpts,
                   const uint32_t *rows, uint32_t nvar, uint32_t *out, uint32_t nrows,
-                  vdb_stats *st, bool verbose, BatchResult *res)
+                  int border, vdb_stats *st, bool verbose, BatchResult *res)
 {
+    const bool std_rule = border == VDB_BORDER_STANDARD;
     // every distance decision of this engine is binary64
     uint32_t n = P.n;
     // ---- levels and minpts values
@@ -1320,7 +1350,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         if (threshold_for_eps(eps[v]) >= 0) continue;
         bool all_core = minpts[v] == 0;
         k_isolated_labels<<<ceil_div(n, kBlock), kBlock, 0, c.stream()>>>(
-            P.mapping, P.seg, P.nsets, n, all_core, out, nrows, rows[v]);
+            P.mapping, P.seg, P.nsets, n, all_core, std_rule, out, nrows, rows[v]);
         VDB_HIP(hipGetLastError());
     }
     if (K == 0) return kOk;
@@ -1401,8 +1431,18 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     uint8_t *cellkc_cm;
     VDB_TRY(c.get("b.cellkc_cm", (size_t)G.ncell * J, &cellkc_cm));
     unsigned gc0 = ceil_div(G.ncell, kBlock);
-    k_cell_core<<<gc0, kBlock, 0, c.stream()>>>(G.cell_start, G.ncell, kc, J, K, G.pos_rank, cmr,
-                                                cellkc, cellkc_cm);
+    // caller index of every position: the standard rule's key, and the label
+    // pass's scatter order
+    uint32_t *caller_of_pos;
+    VDB_TRY(c.get("b.cop", n, &caller_of_pos));
+    k_caller_of_pos<<<gn, kBlock, 0, c.stream()>>>(G.pos_rank, P.mapping, n, caller_of_pos);
+    VDB_HIP(hipGetLastError());
+    if (std_rule)
+        k_cell_core<true><<<gc0, kBlock, 0, c.stream()>>>(G.cell_start, G.ncell, kc, J, K,
+                                                          caller_of_pos, cmr, cellkc, cellkc_cm);
+    else
+        k_cell_core<false><<<gc0, kBlock, 0, c.stream()>>>(G.cell_start, G.ncell, kc, J, K,
+                                                           G.pos_rank, cmr, cellkc, cellkc_cm);
     VDB_HIP(hipGetLastError());
     if (st) st->ms_count += tknn.ms();
 
@@ -1471,10 +1511,6 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_TRY(c.get("b.bstart", (size_t)K + 2, &bstart));
     k_pairs<true><<<gc, kBlock, 0, c.stream()>>>(G.cell_box, G.row_lo, G.row_hi, W, G.ncell,
                                                  dTk, K, nullptr, poff, pa, pb, pk);
-    VDB_HIP(hipGetLastError());
-    uint32_t *caller_of_pos;
-    VDB_TRY(c.get("b.cop", n, &caller_of_pos));
-    k_caller_of_pos<<<gn, kBlock, 0, c.stream()>>>(G.pos_rank, P.mapping, n, caller_of_pos);
     VDB_HIP(hipGetLastError());
     uint32_t *pos_of_caller, *labtmp;
     VDB_TRY(c.get("b.poc", n, &pos_of_caller));
@@ -1598,7 +1634,8 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             VDB_HIP(hipGetLastError());
             VDB_TRY(scan_popc_u32(c, sbits, spre, nw));
             k_cell_labels<<<gc, kBlock, 0, c.stream()>>>(sroot, sbits, spre, G.cell_ds, P.seg,
-                                                         G.ncell, snap_lab + (size_t)slot * G.ncell,
+                                                         G.ncell, std_rule ? 0u : 1u,
+                                                         snap_lab + (size_t)slot * G.ncell,
                                                          compmin);
             VDB_HIP(hipGetLastError());
         }
@@ -1631,9 +1668,14 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             S.nrows = nrows;
             S.seg = P.seg;
             S.tmp = labtmp;
-            k_labels_j<<<gn, kBlock, 0, c.stream()>>>(S, G.cell_of_pos, G.cell_ds,
-                                                      G.pos_rank, kc, J, j, lst_idx, lst_s,
-                                                      lst_id, lst_n, (int)L, n);
+            if (std_rule)
+                k_labels_j<true><<<gn, kBlock, 0, c.stream()>>>(S, G.cell_of_pos, G.cell_ds,
+                                                                G.pos_rank, kc, J, j, lst_idx,
+                                                                lst_s, lst_id, lst_n, (int)L, n);
+            else
+                k_labels_j<false><<<gn, kBlock, 0, c.stream()>>>(S, G.cell_of_pos, G.cell_ds,
+                                                                 G.pos_rank, kc, J, j, lst_idx,
+                                                                 lst_s, lst_id, lst_n, (int)L, n);
             VDB_HIP(hipGetLastError());
             k_rows_from_pos<<<gn, kBlock, 0, c.stream()>>>(S, pos_of_caller, P.nsets, n);
             VDB_HIP(hipGetLastError());

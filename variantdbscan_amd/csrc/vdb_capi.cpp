@@ -142,8 +142,6 @@ int cluster_points(Context &c, Points &P, const char *tag, const double *eps,
     int precision = opt ? opt->precision : VDB_PREC_FP64;
     int border = opt ? opt->border : VDB_BORDER_REFERENCE;
     bool verbose = opt && opt->verbose;
-    // the batched engine's snapshots encode the reference's rule
-    if (border == VDB_BORDER_STANDARD && path == VDB_PATH_AUTO) path = VDB_PATH_FUSED;
     Timer tprep(c.stream());
     VDB_TRY(build_rank_order(c, P, tag));
     if (st) st->ms_prepare += tprep.ms();
@@ -182,7 +180,7 @@ int cluster_points(Context &c, Points &P, const char *tag, const double *eps,
             }
             BatchResult br;
             int s = cluster_batch(c, P, ge.data(), gm.data(), gr.data(), (uint32_t)ge.size(), out,
-                                  nvar, st, verbose, &br);
+                                  nvar, border, st, verbose, &br);
             if (s == kOk) {
                 any_batched = true;
             } else if (s == kErrUnsupported) {
@@ -233,10 +231,6 @@ int check_options(const vdb_options *opt)
         opt->border < VDB_BORDER_REFERENCE || opt->border > VDB_BORDER_STANDARD) {
         set_error("unknown path, precision or border option");
         return kErrInvalid;
-    }
-    if (opt->border == VDB_BORDER_STANDARD && opt->path == VDB_PATH_BATCH) {
-        set_error("the standard border rule runs on the per-variant paths (fused, table)");
-        return kErrUnsupported;
     }
     return kOk;
 }
