@@ -8,6 +8,10 @@ one call clustering all 64 variants.
 
 Also measured in the same run (each in its own timed region, none of them
 `value`):
+  * `batch_standard_border`: the same batched call under the standard border
+    rule (textbook DBSCAN labels, noise -1).
+  * `end_to_end_host`: the drop-in libVDBSCAN call on the same 2M x 64
+    workload from host arrays (upload and the 512 MB label copy included).
   * `roofline`: the CSR neighbour-table fill kernel at eps 0.5 on the same
     points (ids only, FP64), algorithmic bytes per launch over its HIP-event
     duration against 8 TB/s; `table_build` adds the count and scan kernels.
@@ -135,6 +139,29 @@ def main():
                           path=vd.PATH_FUSED)
         lab = labels.view(V, n)[chk]
         out["labels_match_per_variant_engine"] = bool(torch.equal(lab, ref.view(len(chk), n)))
+
+        # the same call under the standard border rule (textbook DBSCAN
+        # labels), with the same spot check
+        for _ in range(args.warmup):
+            vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, eps, mp, labels.data_ptr(),
+                              path=vd.PATH_BATCH, border=vd.BORDER_STANDARD)
+        sa = torch.cuda.Event(enable_timing=True)
+        sb = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        sa.record(stream)
+        for _ in range(args.steps):
+            vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, eps, mp, labels.data_ptr(),
+                              path=vd.PATH_BATCH, border=vd.BORDER_STANDARD)
+        sb.record(stream)
+        torch.cuda.synchronize()
+        ms_std = sa.elapsed_time(sb) / args.steps
+        vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, eps[chk], mp[chk], ref.data_ptr(),
+                          path=vd.PATH_FUSED, border=vd.BORDER_STANDARD)
+        lab = labels.view(V, n)[chk]
+        out["batch_standard_border"] = {
+            "ms_per_step": ms_std, "variants_per_s": V / (ms_std / 1e3),
+            "labels_match_per_variant_engine": bool(torch.equal(lab, ref.view(len(chk), n)))}
+        log("batch, standard border rule: %.3f ms/step" % ms_std)
         del labels, ref
         L.vdb_release()
         torch.cuda.empty_cache()
@@ -231,7 +258,31 @@ def main():
     # ------------------------------------------ other workloads (not `value`)
     if args.only is None and not args.no_extra:
         del dx, dy
+        L.vdb_release()
         torch.cuda.empty_cache()
+        # the drop-in call end to end: host arrays in, labels out through
+        # libVDBSCAN (upload, batched engine, label copy to pageable memory)
+        ret = np.empty(V * n, dtype=np.uint32)
+        hx, hy = np.ascontiguousarray(x), np.ascontiguousarray(y)
+        he = np.ascontiguousarray(eps, dtype=np.float64)
+        hm = np.ascontiguousarray(mp, dtype=np.uint32)
+
+        def drop_in():
+            rc = L.libVDBSCAN(hx.ctypes.data, hy.ctypes.data, n, he.ctypes.data,
+                              hm.ctypes.data, V, 70, ret.ctypes.data, False)
+            assert rc == 0, vd._lib.last_error()
+
+        drop_in()
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            drop_in()
+        e2e = (time.perf_counter() - t0) / reps
+        out["end_to_end_host"] = {"entry": "libVDBSCAN (host arrays, pageable)",
+                                  "s_per_call": e2e, "variants_per_s": V / e2e}
+        log("libVDBSCAN end to end: %.1f ms per call -> %.0f variants/s" % (e2e * 1e3, V / e2e))
+        del ret
+        L.vdb_release()
         # 8M points with a dense clump, eps 0.3, minpts 10: FP32 screening
         # with binary64 recheck against plain FP64
         sx, sy = datasets.skew_8m()
@@ -337,6 +388,7 @@ def main():
         "cpu_baseline": out.get("cpu_baseline"),
     }
     for k in ("ratio_vs_plain", "ratio_vs_plain_table", "plain_fp64", "table_build",
+              "batch_standard_border", "end_to_end_host",
               "skew_8m", "multi_256", "batch_phases_ms",
               "labels_match_per_variant_engine", "wall_s"):
         if k in out:

@@ -478,20 +478,17 @@ __global__ void k_core_levels(const uint32_t *lst_idx, const double *lst_s,
     }
 }
 
-// per cell, per j, per k: smallest key of a point with kc_j <= k, and the
-// first level at which the cell holds a core point. The key is the binned
-// rank (reference rule: cells list their points in ascending rank, so it is
-// the first such point) or the caller index (standard rule: the smallest over
-// the cell's points). Thread per cell: one pass over the cell's points keeps
-// the lowest level seen so far, and a point that lowers it is the first for
-// every level it newly covers. Both tables are stored level-major
-// (cmr[(j*K + k)*ncell + c], cellkc[j*ncell + c]) so that the per-level
-// passes over all cells read them contiguously; cellkc also goes out
-// cell-major (cellkc_cm[c*J + j]) for the pass over cell pairs, which reads
-// both cells' levels for several minpts values at once.
-template <bool STD>
+// per cell, per j, per k: smallest rank of a point with kc_j <= k (cells list
+// their points in ascending rank, so it is the first such point), and the
+// first level at which the cell holds a core point. Thread per cell: one pass
+// over the cell's points keeps the lowest level seen so far, and a point that
+// lowers it is the first for every level it newly covers. Both tables are
+// stored level-major (cmr[(j*K + k)*ncell + c], cellkc[j*ncell + c]) so that
+// the per-level passes over all cells read them contiguously; cellkc also
+// goes out cell-major (cellkc_cm[c*J + j]) for the pass over cell pairs,
+// which reads both cells' levels for several minpts values at once.
 __global__ void k_cell_core(const uint32_t *cell_start, uint32_t ncell, const uint8_t *kc, int J,
-                            int K, const uint32_t *key, uint32_t *cmr, uint8_t *cellkc,
+                            int K, const uint32_t *pos_rank, uint32_t *cmr, uint8_t *cellkc,
                             uint8_t *cellkc_cm)
 {
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
@@ -500,31 +497,63 @@ __global__ void k_cell_core(const uint32_t *cell_start, uint32_t ncell, const ui
     for (int j = 0; j < J; j++) {
         uint32_t *row = cmr + (size_t)j * K * ncell + c;
         int mn = K; // levels [mn, K) have their first point
-        if (STD) {
-            for (uint32_t q = a; q < e; q++) {
-                int lv = kc[(size_t)q * J + j];
-                mn = lv < mn ? lv : mn;
-            }
-            for (int k = mn; k < K; k++) {
-                uint32_t m = kNone;
-                for (uint32_t q = a; q < e; q++) {
-                    uint32_t kq = key[q];
-                    if (kc[(size_t)q * J + j] <= k && kq < m) m = kq;
-                }
-                row[(size_t)k * ncell] = m;
-            }
-        } else {
-            for (uint32_t q = a; q < e && mn > 0; q++) {
-                int lv = kc[(size_t)q * J + j];
-                if (lv >= mn) continue;
-                uint32_t r = key[q];
-                for (int k = lv; k < mn; k++) row[(size_t)k * ncell] = r;
-                mn = lv;
-            }
+        for (uint32_t q = a; q < e && mn > 0; q++) {
+            int lv = kc[(size_t)q * J + j];
+            if (lv >= mn) continue;
+            uint32_t r = pos_rank[q];
+            for (int k = lv; k < mn; k++) row[(size_t)k * ncell] = r;
+            mn = lv;
         }
         for (int k = 0; k < mn; k++) row[(size_t)k * ncell] = kNone;
         cellkc[(size_t)j * ncell + c] = (uint8_t)mn;
         cellkc_cm[(size_t)c * J + j] = (uint8_t)mn; // cell-major copy for pair passes
+    }
+}
+
+// The same tables keyed by caller index (standard border rule). Caller order
+// is not sorted inside a cell, so the smallest key per level takes a scan of
+// the whole cell: a wave per cell, each lane keeping in its own LDS column
+// the smallest key of its points at each exact level; a running minimum over
+// the levels then gives "kc_j <= k", reduced across the wave level by level.
+// Dynamic LDS: kWPB * K * kWave words.
+__global__ void __launch_bounds__(kBlock)
+    k_cell_core_std(const uint32_t *cell_start, uint32_t ncell, const uint8_t *kc, int J, int K,
+                    const uint32_t *key, uint32_t *cmr, uint8_t *cellkc, uint8_t *cellkc_cm)
+{
+    extern __shared__ uint32_t lmin[]; // [wave][level][lane]
+    uint32_t c = blockIdx.x * kWPB + wave_id_in_block();
+    if (c >= ncell) return; // wave-uniform
+    int lane = lane_id();
+    uint32_t *col = lmin + (size_t)wave_id_in_block() * K * kWave + lane;
+    uint32_t a = cell_start[c], e = cell_start[c + 1];
+    for (int j = 0; j < J; j++) {
+        for (int k = 0; k < K; k++) col[k * kWave] = kNone;
+        for (uint32_t q = a + lane; q < e; q += kWave) {
+            int lv = kc[(size_t)q * J + j];
+            if (lv < K) {
+                uint32_t kq = key[q];
+                uint32_t cur = col[lv * kWave];
+                col[lv * kWave] = kq < cur ? kq : cur;
+            }
+        }
+        uint32_t *row = cmr + (size_t)j * K * ncell + c;
+        uint32_t run = kNone;
+        int mn = K;
+        for (int k = 0; k < K; k++) {
+            uint32_t v = col[k * kWave];
+            run = v < run ? v : run;
+            uint32_t m = run;
+            for (int o = kWave / 2; o > 0; o >>= 1) {
+                uint32_t t = __shfl_xor(m, o);
+                m = t < m ? t : m;
+            }
+            if (m != kNone && mn == K) mn = k;
+            if (lane == 0) row[(size_t)k * ncell] = m;
+        }
+        if (lane == 0) {
+            cellkc[(size_t)j * ncell + c] = (uint8_t)mn;
+            cellkc_cm[(size_t)c * J + j] = (uint8_t)mn;
+        }
     }
 }
 
@@ -1438,11 +1467,12 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     k_caller_of_pos<<<gn, kBlock, 0, c.stream()>>>(G.pos_rank, P.mapping, n, caller_of_pos);
     VDB_HIP(hipGetLastError());
     if (std_rule)
-        k_cell_core<true><<<gc0, kBlock, 0, c.stream()>>>(G.cell_start, G.ncell, kc, J, K,
-                                                          caller_of_pos, cmr, cellkc, cellkc_cm);
+        k_cell_core_std<<<ceil_div(G.ncell, kWPB), kBlock,
+                          sizeof(uint32_t) * kWPB * (size_t)K * kWave, c.stream()>>>(
+            G.cell_start, G.ncell, kc, J, K, caller_of_pos, cmr, cellkc, cellkc_cm);
     else
-        k_cell_core<false><<<gc0, kBlock, 0, c.stream()>>>(G.cell_start, G.ncell, kc, J, K,
-                                                           G.pos_rank, cmr, cellkc, cellkc_cm);
+        k_cell_core<<<gc0, kBlock, 0, c.stream()>>>(G.cell_start, G.ncell, kc, J, K, G.pos_rank,
+                                                    cmr, cellkc, cellkc_cm);
     VDB_HIP(hipGetLastError());
     if (st) st->ms_count += tknn.ms();
 
