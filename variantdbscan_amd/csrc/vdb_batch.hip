@@ -750,18 +750,6 @@ __global__ void k_bucket_starts(const uint32_t *key, uint32_t n, int K, uint32_t
     for (uint32_t k = prev; k <= cur && k <= (uint32_t)K + 1; k++) bstart[k] = i;
 }
 
-// records of the pairs that ever become eligible (the first *nelig of the
-// sorted order; the grid covers all pairs)
-__global__ void k_pack_pairs(const uint32_t *order, const uint32_t *pa, const uint32_t *pb,
-                             const uint16_t *pk, const uint32_t *nelig, PairRec *rec)
-{
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= *nelig) return;
-    uint32_t v = order[i] & 0x7fffffffu;
-    uint32_t exact = order[i] >> 31;
-    rec[i] = PairRec{pa[v], pb[v], (uint32_t)pk[v] | (exact << 16)};
-}
-
 __device__ __forceinline__ void wave_append_rec(bool take, const PairRec &v, PairRec *list,
                                                 uint32_t *count)
 {
@@ -833,13 +821,16 @@ __device__ __forceinline__ uint32_t level_blocks(const uint32_t *bstart, int k)
 }
 
 // Grid-stride over the carried pairs (count on the device) followed by the
-// pairs that become eligible at this level. Each eligible pair is settled
+// pairs that become eligible at this level, read through the level-sorted
+// pair order (order[i] = pair index | exact-level flag in bit 31; a stable
+// sort keeps the indices of one level ascending, so the gathers of a wave
+// stay close together). Each eligible pair is settled
 // (already connected; fully inside eps; small and tested here), kept for the
 // next level, or handed to a wave test. Tests of grid-adjacent cells go to
 // `near`, the rest to `far`; the near tests run first so most far pairs are
 // already connected when theirs run.
 __global__ void __launch_bounds__(kBlock)
-    k_sweep(LevelArgs a, const PairRec *carried, const uint32_t *ncarried, const PairRec *rec,
+    k_sweep(LevelArgs a, const PairRec *carried, const uint32_t *ncarried, const uint32_t *order,
             const uint32_t *bstart, uint32_t *parent, PairRec *keep, uint32_t *nkeep,
             PairRec *near, uint32_t *nnear, PairRec *far, uint32_t *nfar, uint32_t tcap,
             uint32_t *ovf)
@@ -848,7 +839,7 @@ __global__ void __launch_bounds__(kBlock)
     bool chunked = bstart[a.k + 1] - bstart[a.k] >= kChunkedFrom;
     uint32_t nb = level_blocks(bstart, a.k);
     if (blockIdx.x >= nb) return;
-    const PairRec *fresh = rec + bstart[a.k];
+    const uint32_t *fresh = order + bstart[a.k];
     uint32_t nfresh = bstart[a.k + 1] - bstart[a.k];
     uint32_t nc = *ncarried;
     uint32_t total = nc + nfresh;
@@ -856,7 +847,14 @@ __global__ void __launch_bounds__(kBlock)
         uint32_t i = base + threadIdx.x;
         bool valid = i < total;
         PairRec pr{0, 0, 0};
-        if (valid) pr = i < nc ? carried[i] : fresh[i - nc];
+        if (valid) {
+            if (i < nc) {
+                pr = carried[i];
+            } else {
+                uint32_t o = fresh[i - nc], v = o & 0x7fffffffu;
+                pr = PairRec{a.pa[v], a.pb[v], (uint32_t)a.pk[v] | ((o >> 31) << 16)};
+            }
+        }
         bool kp = false, tnear = false, tfar = false, uni = false;
         uint32_t ra = 0, rb = 0;
         if (valid) {
@@ -1504,12 +1502,11 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     uint32_t npairs = (uint32_t)npairs64;
     uint32_t *pa, *pb, *cnts;
     uint16_t *pk;
-    PairRec *rec, *la, *lb, *tests, *tests2;
+    PairRec *la, *lb, *tests, *tests2;
     size_t psz = npairs ? npairs : 1;
     VDB_TRY(c.get("b.pa", psz, &pa));
     VDB_TRY(c.get("b.pb", psz, &pb));
     VDB_TRY(c.get("b.pk", psz, &pk));
-    VDB_TRY(c.get("b.rec", psz, &rec));
     VDB_TRY(c.get("b.la", psz, &la));
     VDB_TRY(c.get("b.lb", psz, &lb));
     // test lists are filled in chunks (see kRecChunk): room for the records
@@ -1592,6 +1589,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         VDB_HIP(hipGetLastError());
         // pair records bucketed by the level at which they become eligible
         std::vector<uint32_t> bst(K + 2, 0);
+        const uint32_t *order = stv; // level-sorted pair order (unread without pairs)
         if (npairs) {
             if (j < jg0 || j >= jg0 + jgn) {
                 jg0 = j;
@@ -1608,9 +1606,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             k_bucket_starts<<<ceil_div(npairs + 1, kBlock), kBlock, 0, c.stream()>>>(sk, npairs, K,
                                                                                    bstart);
             VDB_HIP(hipGetLastError());
-            k_pack_pairs<<<ceil_div(npairs, kBlock), kBlock, 0, c.stream()>>>(sv, pa, pb, pk,
-                                                                            bstart + K, rec);
-            VDB_HIP(hipGetLastError());
+            order = sv;
             if (verbose) {
                 // bucket sizes (a sync per minpts value: verbose only)
                 VDB_HIP(hipMemcpyAsync(bst.data(), bstart, sizeof(uint32_t) * (K + 2),
@@ -1629,7 +1625,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             // worklist sizes stay on the device: bounded grids stride over them
             uint32_t nfresh = bst[k + 1] - bst[k]; // verbose only
             k_sweep<<<kMaxPersistent, kBlock, 0, c.stream()>>>(
-                a, lin, c_keep + k, rec, bstart, parent, lout, c_keep + k + 1, tests, c_near + k,
+                a, lin, c_keep + k, order, bstart, parent, lout, c_keep + k + 1, tests, c_near + k,
                 tests2, c_far + k, tcap, tovf);
             VDB_HIP(hipGetLastError());
             k_test<<<kMaxTestBlocks, kBlock, 0, c.stream()>>>(a, bstart, tests, c_near + k, tests2,
