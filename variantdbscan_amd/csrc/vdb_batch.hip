@@ -1035,7 +1035,8 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---------------------------------------------------------- snapshots
-// Flattens the forest (every cell points at its root) and sets
+// Flattens the forest (every cell with a core point at level k points at its
+// root) and sets
 // compmin[root] = smallest core rank of the component (compmin holds NONE on
 // entry: k_init_parent and k_cell_labels clear it). Lanes of a wave that
 // share a root (the common case inside a large cluster) reduce first, so a
@@ -1047,7 +1048,10 @@ __global__ void k_roots_min(uint32_t *parent, const uint32_t *cmrjk, const uint8
 {
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     uint32_t root = kNone;
-    if (c < ncell) {
+    // a cell without a core point at level k is in no eligible pair yet, so
+    // nothing points at it and it points at itself: nothing to flatten
+    bool valid = c < ncell && ckj[c] <= k;
+    if (valid) {
         uint32_t x = c, p = ld_u32(parent + x);
         while (p != x) { // read-only walk; only true roots are stored
             x = p;
@@ -1056,7 +1060,6 @@ __global__ void k_roots_min(uint32_t *parent, const uint32_t *cmrjk, const uint8
         st_u32(parent + c, x);
         root = x;
     }
-    bool valid = c < ncell && ckj[c] <= k;
     uint32_t v = valid ? cmrjk[c] : kNone;
     uint64_t todo = ballot(valid);
     for (int it = 0; it < 2 && todo; it++) {
@@ -1082,7 +1085,7 @@ __global__ void k_snapshot(const uint32_t *parent, const uint32_t *compmin, cons
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= ncell) return;
     bool core = ckj[c] <= k;
-    uint32_t r = parent[c];
+    uint32_t r = core ? parent[c] : c;
     uint32_t m = core ? compmin[r] : kNone;
     rootrank[c] = m;
     if (core && r == c) atomicOr(bits + (m >> 5), 1u << (m & 31));
