@@ -29,7 +29,7 @@ Also measured in the same run (each in its own timed region, none of them
     variants in one call, on the host's cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-       [--no-table] [--no-plain] [--no-extra] [--only batch|table]
+       [--no-table] [--no-plain] [--no-extra] [--no-standard] [--only batch|table]
 """
 import argparse
 import json
@@ -73,6 +73,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the skew_8m and multi_256 side measurements")
     ap.add_argument("--only", choices=["batch", "table"], default=None)
+    ap.add_argument("--no-standard", action="store_true",
+                    help="skip the standard-border-rule timing of the batch leg")
     ap.add_argument("--table-steps", type=int, default=2)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy in the table leg (HBM counter calibration)")
@@ -142,26 +144,27 @@ def main():
 
         # the same call under the standard border rule (textbook DBSCAN
         # labels), with the same spot check
-        for _ in range(args.warmup):
+        for _ in range(0 if args.no_standard else args.warmup):
             vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, eps, mp, labels.data_ptr(),
                               path=vd.PATH_BATCH, border=vd.BORDER_STANDARD)
-        sa = torch.cuda.Event(enable_timing=True)
-        sb = torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        sa.record(stream)
-        for _ in range(args.steps):
-            vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, eps, mp, labels.data_ptr(),
-                              path=vd.PATH_BATCH, border=vd.BORDER_STANDARD)
-        sb.record(stream)
-        torch.cuda.synchronize()
-        ms_std = sa.elapsed_time(sb) / args.steps
-        vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, eps[chk], mp[chk], ref.data_ptr(),
-                          path=vd.PATH_FUSED, border=vd.BORDER_STANDARD)
-        lab = labels.view(V, n)[chk]
-        out["batch_standard_border"] = {
-            "ms_per_step": ms_std, "variants_per_s": V / (ms_std / 1e3),
-            "labels_match_per_variant_engine": bool(torch.equal(lab, ref.view(len(chk), n)))}
-        log("batch, standard border rule: %.3f ms/step" % ms_std)
+        if not args.no_standard:
+            sa = torch.cuda.Event(enable_timing=True)
+            sb = torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            sa.record(stream)
+            for _ in range(args.steps):
+                vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, eps, mp, labels.data_ptr(),
+                                  path=vd.PATH_BATCH, border=vd.BORDER_STANDARD)
+            sb.record(stream)
+            torch.cuda.synchronize()
+            ms_std = sa.elapsed_time(sb) / args.steps
+            vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, eps[chk], mp[chk], ref.data_ptr(),
+                              path=vd.PATH_FUSED, border=vd.BORDER_STANDARD)
+            lab = labels.view(V, n)[chk]
+            out["batch_standard_border"] = {
+                "ms_per_step": ms_std, "variants_per_s": V / (ms_std / 1e3),
+                "labels_match_per_variant_engine": bool(torch.equal(lab, ref.view(len(chk), n)))}
+            log("batch, standard border rule: %.3f ms/step" % ms_std)
         del labels, ref
         L.vdb_release()
         torch.cuda.empty_cache()

@@ -1049,10 +1049,12 @@ __global__ void k_roots_min(uint32_t *parent, const uint32_t *cmrjk, const uint8
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     uint32_t root = kNone;
     // a cell without a core point at level k is in no eligible pair yet, so
-    // nothing points at it and it points at itself: nothing to flatten
+    // nothing points at it and it points at itself: nothing to flatten (its
+    // parent is loaded anyway, beside its level, to keep the chain short)
+    uint32_t p = c < ncell ? ld_u32(parent + c) : 0u;
     bool valid = c < ncell && ckj[c] <= k;
     if (valid) {
-        uint32_t x = c, p = ld_u32(parent + x);
+        uint32_t x = c;
         while (p != x) { // read-only walk; only true roots are stored
             x = p;
             p = ld_u32(parent + x);
