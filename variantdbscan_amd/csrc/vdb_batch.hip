@@ -843,18 +843,27 @@ __global__ void __launch_bounds__(kBlock)
     uint32_t nfresh = bstart[a.k + 1] - bstart[a.k];
     uint32_t nc = *ncarried;
     uint32_t total = nc + nfresh;
-    for (uint32_t base = blockIdx.x * kBlock; base < total; base += nb * kBlock) {
-        uint32_t i = base + threadIdx.x;
-        bool valid = i < total;
-        PairRec pr{0, 0, 0};
-        if (valid) {
+    auto load_rec = [&](uint32_t i) {
+        PairRec r{0, 0, 0};
+        if (i < total) {
             if (i < nc) {
-                pr = carried[i];
+                r = carried[i];
             } else {
                 uint32_t o = fresh[i - nc], v = o & 0x7fffffffu;
-                pr = PairRec{a.pa[v], a.pb[v], (uint32_t)a.pk[v] | ((o >> 31) << 16)};
+                r = PairRec{a.pa[v], a.pb[v], (uint32_t)a.pk[v] | ((o >> 31) << 16)};
             }
         }
+        return r;
+    };
+    const uint32_t step = nb * kBlock;
+    // the next step's record is loaded while this one is settled: its loads
+    // (order, then the pair arrays) do not wait behind this step's parent loads
+    PairRec next = load_rec(blockIdx.x * kBlock + threadIdx.x);
+    for (uint32_t base = blockIdx.x * kBlock; base < total; base += step) {
+        uint32_t i = base + threadIdx.x;
+        bool valid = i < total;
+        PairRec pr = next;
+        next = load_rec(base + step < total ? i + step : total);
         bool kp = false, tnear = false, tfar = false, uni = false;
         uint32_t ra = 0, rb = 0;
         if (valid) {
