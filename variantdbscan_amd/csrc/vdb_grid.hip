@@ -190,6 +190,7 @@ __global__ void k_cell_tail(uint32_t *cell_start, uint32_t ncell, uint32_t n)
 // cell (its points read serially); cells holding more than kBoxSerial points
 // are then reduced by the whole wave, one after another.
 constexpr uint32_t kBoxSerial = 16;
+constexpr int kBoxUnroll = 8;
 __global__ void k_cell_box(const double *xs, const double *ys, const uint32_t *cell_start,
                            uint32_t ncell, double *box)
 {
@@ -221,7 +222,25 @@ __global__ void k_cell_box(const double *xs, const double *ys, const uint32_t *c
         uint32_t cc = c - (uint32_t)lane_id() + (uint32_t)bit;
         uint32_t ca = __shfl(a, bit), cb = __shfl(b, bit);
         double x0 = DBL_MAX, x1 = -DBL_MAX, y0 = DBL_MAX, y1 = -DBL_MAX;
-        for (uint32_t p = ca + lane_id(); p < cb; p += kWave) {
+        // kBoxUnroll strides per step, all loads issued before the first
+        // min/max: a dense cell (thousands of points) is a long walk
+        uint32_t p = ca + lane_id();
+        for (; p + (kBoxUnroll - 1) * kWave < cb; p += kBoxUnroll * kWave) {
+            double u[kBoxUnroll], v[kBoxUnroll];
+#pragma unroll
+            for (int t = 0; t < kBoxUnroll; t++) {
+                u[t] = xs[p + t * kWave];
+                v[t] = ys[p + t * kWave];
+            }
+#pragma unroll
+            for (int t = 0; t < kBoxUnroll; t++) {
+                x0 = u[t] < x0 ? u[t] : x0;
+                x1 = u[t] > x1 ? u[t] : x1;
+                y0 = v[t] < y0 ? v[t] : y0;
+                y1 = v[t] > y1 ? v[t] : y1;
+            }
+        }
+        for (; p < cb; p += kWave) {
             double u = xs[p], v = ys[p];
             x0 = u < x0 ? u : x0;
             x1 = u > x1 ? u : x1;
