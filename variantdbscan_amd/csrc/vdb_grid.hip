@@ -334,34 +334,45 @@ __device__ __forceinline__ uint32_t lower_bound_near(const uint32_t *a, uint32_t
 
 // window rows: for cell c and row offset d in [-R, R], the cells of grid row
 // cx+d whose cy lies in [cy-R, cy+R] form one contiguous run of the sorted keys
-// rows stay inside the cell's own dataset grid
+// rows stay inside the cell's own dataset grid. A thread takes one row offset
+// for kRowRun consecutive cells: each search starts from the previous cell's
+// answer, a few positions away, instead of galloping across up to R columns.
+constexpr uint32_t kRowRun = 8;
 __global__ void k_row_ranges(const uint32_t *cell_key, uint32_t ncell, const uint32_t *sncx,
                              const uint32_t *sncy, const uint32_t *skbase, uint32_t nsets, int R,
                              uint32_t *row_lo, uint32_t *row_hi, uint32_t *cell_ds)
 {
     uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     uint32_t W = 2 * R + 1;
-    if (t >= ncell * W) return;
-    uint32_t c = t / W;
+    uint32_t nrun = (ncell + kRowRun - 1) / kRowRun;
+    if (t >= nrun * W) return;
+    uint32_t run = t / W;
     int dr = (int)(t % W) - R;
-    uint32_t key = cell_key[c];
-    uint32_t ds = seg_of(skbase, nsets, key);
-    if (dr == 0) cell_ds[c] = ds;
-    uint32_t ncx = sncx[ds], ncy = sncy[ds], kb = skbase[ds];
-    uint32_t local = key - kb;
-    int64_t cx = local / ncy, cy = local % ncy;
-    int64_t rx = cx + dr;
-    uint32_t lo = 0, hi = 0;
-    if (rx >= 0 && rx < (int64_t)ncx) {
-        int64_t y0 = cy - R < 0 ? 0 : cy - R;
-        int64_t y1 = cy + R > (int64_t)ncy - 1 ? (int64_t)ncy - 1 : cy + R;
-        uint64_t k0 = kb + (uint64_t)(rx * ncy + y0);
-        uint64_t k1 = kb + (uint64_t)(rx * ncy + y1);
-        lo = lower_bound_near(cell_key, ncell, (uint32_t)k0, c);
-        hi = k1 >= 0xffffffffull ? ncell : lower_bound_near(cell_key, ncell, (uint32_t)(k1 + 1), lo);
+    uint32_t c0 = run * kRowRun;
+    uint32_t c1 = c0 + kRowRun < ncell ? c0 + kRowRun : ncell;
+    uint32_t hint = c0;
+    for (uint32_t c = c0; c < c1; c++) {
+        uint32_t key = cell_key[c];
+        uint32_t ds = seg_of(skbase, nsets, key);
+        if (dr == 0) cell_ds[c] = ds;
+        uint32_t ncx = sncx[ds], ncy = sncy[ds], kb = skbase[ds];
+        uint32_t local = key - kb;
+        int64_t cx = local / ncy, cy = local % ncy;
+        int64_t rx = cx + dr;
+        uint32_t lo = 0, hi = 0;
+        if (rx >= 0 && rx < (int64_t)ncx) {
+            int64_t y0 = cy - R < 0 ? 0 : cy - R;
+            int64_t y1 = cy + R > (int64_t)ncy - 1 ? (int64_t)ncy - 1 : cy + R;
+            uint64_t k0 = kb + (uint64_t)(rx * ncy + y0);
+            uint64_t k1 = kb + (uint64_t)(rx * ncy + y1);
+            lo = lower_bound_near(cell_key, ncell, (uint32_t)k0, hint);
+            hi = k1 >= 0xffffffffull ? ncell
+                                     : lower_bound_near(cell_key, ncell, (uint32_t)(k1 + 1), lo);
+            hint = lo;
+        }
+        row_lo[(size_t)c * W + (dr + R)] = lo;
+        row_hi[(size_t)c * W + (dr + R)] = hi;
     }
-    row_lo[t] = lo;
-    row_hi[t] = hi;
 }
 
 std::string slot(const char *tag, const char *name) { return std::string(tag) + "." + name; }
@@ -648,7 +659,7 @@ int build_grid(Context &c, const Points &P, double w_min, int R, const char *tag
     VDB_TRY(c.get(slot(tag, "row_lo").c_str(), ncell * W, &G.row_lo));
     VDB_TRY(c.get(slot(tag, "row_hi").c_str(), ncell * W, &G.row_hi));
     VDB_TRY(c.get(slot(tag, "cell_ds").c_str(), ncell, &G.cell_ds));
-    k_row_ranges<<<ceil_div(ncell * W, kBlock), kBlock, 0, c.stream()>>>(
+    k_row_ranges<<<ceil_div(ceil_div(ncell, kRowRun) * W, kBlock), kBlock, 0, c.stream()>>>(
         G.cell_key, ncell, G.sncx, G.sncy, G.skbase, ns, R, G.row_lo, G.row_hi, G.cell_ds);
     VDB_HIP(hipGetLastError());
 
