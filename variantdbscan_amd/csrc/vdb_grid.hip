@@ -337,7 +337,7 @@ __device__ __forceinline__ uint32_t lower_bound_near(const uint32_t *a, uint32_t
 // rows stay inside the cell's own dataset grid. A thread takes one row offset
 // for kRowRun consecutive cells: each search starts from the previous cell's
 // answer, a few positions away, instead of galloping across up to R columns.
-constexpr uint32_t kRowRun = 8;
+constexpr uint32_t kRowRun = 4;
 __global__ void k_row_ranges(const uint32_t *cell_key, uint32_t ncell, const uint32_t *sncx,
                              const uint32_t *sncy, const uint32_t *skbase, uint32_t nsets, int R,
                              uint32_t *row_lo, uint32_t *row_hi, uint32_t *cell_ds)
