@@ -1613,14 +1613,21 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
                     K, npairs, ekey, eval);
                 VDB_HIP(hipGetLastError());
             }
-            uint32_t *sk, *sv;
             size_t g = (size_t)(j - jg0) * npairs;
-            VDB_TRY(radix_sort_pairs(c, ekey + g, eval + g, stk, stv, npairs, bits_for(K + 1),
-                                     &sk, &sv));
-            k_bucket_starts<<<ceil_div(npairs + 1, kBlock), kBlock, 0, c.stream()>>>(sk, npairs, K,
-                                                                                   bstart);
-            VDB_HIP(hipGetLastError());
-            order = sv;
+            if (K + 1 <= 16) {
+                // levels 0..K fit one counting pass, which also gives the
+                // bucket starts
+                VDB_TRY(bucket_sort_values(c, ekey + g, eval + g, stv, npairs, K + 1, bstart));
+                order = stv;
+            } else {
+                uint32_t *sk, *sv;
+                VDB_TRY(radix_sort_pairs(c, ekey + g, eval + g, stk, stv, npairs,
+                                         bits_for(K + 1), &sk, &sv));
+                k_bucket_starts<<<ceil_div(npairs + 1, kBlock), kBlock, 0, c.stream()>>>(
+                    sk, npairs, K, bstart);
+                VDB_HIP(hipGetLastError());
+                order = sv;
+            }
             if (verbose) {
                 // bucket sizes (a sync per minpts value: verbose only)
                 VDB_HIP(hipMemcpyAsync(bst.data(), bstart, sizeof(uint32_t) * (K + 2),
