@@ -579,32 +579,91 @@ __device__ __forceinline__ void pair_levels(const double *A, const double *B, co
     *kfull = b;
 }
 
-template <bool WRITE>
-__global__ void k_pairs(const double *box, const uint32_t *row_lo, const uint32_t *row_hi, int W,
-                        uint32_t ncell, const double *Tk, int K, uint32_t *cnt,
-                        const uint64_t *off, uint32_t *pa, uint32_t *pb, uint16_t *pk)
+// Pairs (A, B > A) that some eps level may connect, counted per cell A with
+// a thread per cell (the same acceptance test as k_pairs below, which stores
+// them).
+__global__ void k_pair_count(const double *box, const uint32_t *row_lo, const uint32_t *row_hi,
+                             int W, uint32_t ncell, const double *Tk, int K, uint32_t *cnt)
 {
     uint32_t A = blockIdx.x * kBlock + threadIdx.x;
     if (A >= ncell) return;
     const double *bA = box + 4 * (size_t)A;
     uint32_t k = 0;
-    uint64_t o = WRITE ? off[A] : 0;
     for (int r = 0; r < W; r++) {
         uint32_t lo = row_lo[(size_t)A * W + r], hi = row_hi[(size_t)A * W + r];
         if (lo <= A) lo = A + 1;
         for (uint32_t B = lo; B < hi; B++) {
             int kmin, kfull;
             pair_levels(bA, box + 4 * (size_t)B, Tk, K, &kmin, &kfull);
-            if (kmin >= K) continue;
-            if (WRITE) {
-                pa[o + k] = A;
-                pb[o + k] = B;
-                pk[o + k] = (uint16_t)(kmin | (kfull << 8));
-            }
-            k++;
+            k += kmin < K ? 1u : 0u;
         }
     }
-    if (!WRITE) cnt[A] = k;
+    cnt[A] = k;
+}
+
+// Cell pairs (A, B > A) that some eps level may connect: a wave per cell A.
+// Lane r < W holds window row r's run of cells after A; a scan of the run
+// lengths flattens the window, and each step the lanes take the next 64
+// candidates in row order (a lane finds its row by binary search over the
+// scanned run ends). Accepted pairs of A are stored in row order, B
+// ascending within a row, by one ballot-compacted store per step (a thread
+// per cell, storing its own run, made 64 scattered stores per instruction).
+// A's share of the output starts at off[A] (k_pair_count, scanned). Needs
+// W <= 64.
+__global__ void __launch_bounds__(kBlock)
+    k_pairs(const double *box, const uint32_t *row_lo, const uint32_t *row_hi, int W,
+            uint32_t ncell, const double *Tk, int K, const uint64_t *off, uint32_t *pa,
+            uint32_t *pb, uint16_t *pk)
+{
+    uint32_t A = blockIdx.x * kWPB + wave_id_in_block();
+    if (A >= ncell) return; // wave-uniform
+    int lane = lane_id();
+    uint32_t lo = 0, len = 0;
+    if (lane < W) {
+        uint32_t l = row_lo[(size_t)A * W + lane], h = row_hi[(size_t)A * W + lane];
+        if (l <= A) l = A + 1;
+        lo = l;
+        len = h > l ? h - l : 0u;
+    }
+    uint32_t incl = len;
+    for (int o = 1; o < kWave; o <<= 1) {
+        uint32_t t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    uint32_t excl = incl - len;
+    uint32_t total = __shfl(incl, kWave - 1);
+    const double *bA = box + 4 * (size_t)A;
+    uint64_t o = off[A];
+    uint32_t k = 0;
+    for (uint32_t t0 = 0; t0 < total; t0 += kWave) {
+        uint32_t t = t0 + (uint32_t)lane;
+        // row of t: the first row whose run ends after t (6 halvings cover
+        // 64 rows; every lane shuffles, finished or not)
+        int rlo = 0, rhi = W - 1;
+#pragma unroll
+        for (int step = 0; step < 6; step++) {
+            int mid = (rlo + rhi) >> 1;
+            uint32_t im = __shfl(incl, mid);
+            if (rlo < rhi) {
+                if (im > t) rhi = mid;
+                else rlo = mid + 1;
+            }
+        }
+        uint32_t rs = __shfl(excl, rlo), rl = __shfl(lo, rlo);
+        bool v = t < total;
+        uint32_t B = rl + (t - rs);
+        int kmin = K, kfull = K;
+        if (v) pair_levels(bA, box + 4 * (size_t)B, Tk, K, &kmin, &kfull);
+        bool acc = v && kmin < K;
+        uint64_t m = ballot(acc);
+        if (acc) {
+            uint64_t d = o + k + (uint32_t)mask_rank(m);
+            pa[d] = A;
+            pb[d] = B;
+            pk[d] = (uint16_t)(kmin | (kfull << 8));
+        }
+        k += (uint32_t)popc64(m);
+    }
 }
 
 // parent[i] = i, compmin[i] = NONE, and two arrays cleared (counters,
@@ -1495,10 +1554,10 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_TRY(c.get("b.pcnt", G.ncell, &pcnt));
     VDB_TRY(c.get("b.poff", (size_t)G.ncell + 1, &poff));
     unsigned gc = ceil_div(G.ncell, kBlock);
+    unsigned gcw = ceil_div(G.ncell, kWPB); // a wave per cell
     int W = 2 * R + 1;
-    k_pairs<false><<<gc, kBlock, 0, c.stream()>>>(G.cell_box, G.row_lo, G.row_hi, W, G.ncell,
-                                                  dTk, K, pcnt, nullptr, nullptr, nullptr,
-                                                  nullptr);
+    k_pair_count<<<gc, kBlock, 0, c.stream()>>>(G.cell_box, G.row_lo, G.row_hi, W, G.ncell, dTk,
+                                                K, pcnt);
     VDB_HIP(hipGetLastError());
     VDB_TRY(scan_exclusive_u32_u64(c, pcnt, poff, G.ncell, poff + G.ncell));
     uint64_t npairs64 = 0;
@@ -1550,8 +1609,8 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_TRY(c.get("b.stk", psz, &stk));
     VDB_TRY(c.get("b.stv", psz, &stv));
     VDB_TRY(c.get("b.bstart", (size_t)K + 2, &bstart));
-    k_pairs<true><<<gc, kBlock, 0, c.stream()>>>(G.cell_box, G.row_lo, G.row_hi, W, G.ncell,
-                                                 dTk, K, nullptr, poff, pa, pb, pk);
+    k_pairs<<<gcw, kBlock, 0, c.stream()>>>(G.cell_box, G.row_lo, G.row_hi, W, G.ncell, dTk, K,
+                                            poff, pa, pb, pk);
     VDB_HIP(hipGetLastError());
     uint32_t *pos_of_caller, *labtmp;
     VDB_TRY(c.get("b.poc", n, &pos_of_caller));
