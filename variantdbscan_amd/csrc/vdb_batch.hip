@@ -511,48 +511,86 @@ __global__ void k_cell_core(const uint32_t *cell_start, uint32_t ncell, const ui
 }
 
 // The same tables keyed by caller index (standard border rule). Caller order
-// is not sorted inside a cell, so the smallest key per level takes a scan of
-// the whole cell: a wave per cell, each lane keeping in its own LDS column
-// the smallest key of its points at each exact level; a running minimum over
-// the levels then gives "kc_j <= k", reduced across the wave level by level.
-// Dynamic LDS: kWPB * K * kWave words.
+// is not sorted inside a cell, so the smallest key per level takes the whole
+// cell. A thread per cell of at most kCoreSerial points, held in registers;
+// larger cells are then taken by the whole wave one after another, each lane
+// keeping in its own LDS column the smallest key of its points at each exact
+// level, and a running minimum over the levels, reduced across the wave,
+// giving "kc_j <= k". Dynamic LDS: kWPB * K * kWave words.
+constexpr uint32_t kCoreSerial = 16;
 __global__ void __launch_bounds__(kBlock)
     k_cell_core_std(const uint32_t *cell_start, uint32_t ncell, const uint8_t *kc, int J, int K,
                     const uint32_t *key, uint32_t *cmr, uint8_t *cellkc, uint8_t *cellkc_cm)
 {
     extern __shared__ uint32_t lmin[]; // [wave][level][lane]
-    uint32_t c = blockIdx.x * kWPB + wave_id_in_block();
-    if (c >= ncell) return; // wave-uniform
-    int lane = lane_id();
-    uint32_t *col = lmin + (size_t)wave_id_in_block() * K * kWave + lane;
-    uint32_t a = cell_start[c], e = cell_start[c + 1];
-    for (int j = 0; j < J; j++) {
-        for (int k = 0; k < K; k++) col[k * kWave] = kNone;
-        for (uint32_t q = a + lane; q < e; q += kWave) {
-            int lv = kc[(size_t)q * J + j];
-            if (lv < K) {
-                uint32_t kq = key[q];
-                uint32_t cur = col[lv * kWave];
-                col[lv * kWave] = kq < cur ? kq : cur;
+    uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t a = 0, e = 0;
+    if (c < ncell) {
+        a = cell_start[c];
+        e = cell_start[c + 1];
+    }
+    bool big = e - a > kCoreSerial;
+    if (c < ncell && !big) {
+        uint32_t np = e - a;
+        uint32_t kq[kCoreSerial];
+#pragma unroll
+        for (int i = 0; i < (int)kCoreSerial; i++) kq[i] = (uint32_t)i < np ? key[a + i] : kNone;
+        for (int j = 0; j < J; j++) {
+            int lv[kCoreSerial];
+            int mn = K;
+#pragma unroll
+            for (int i = 0; i < (int)kCoreSerial; i++) {
+                lv[i] = (uint32_t)i < np ? (int)kc[(size_t)(a + i) * J + j] : K;
+                mn = lv[i] < mn ? lv[i] : mn;
             }
-        }
-        uint32_t *row = cmr + (size_t)j * K * ncell + c;
-        uint32_t run = kNone;
-        int mn = K;
-        for (int k = 0; k < K; k++) {
-            uint32_t v = col[k * kWave];
-            run = v < run ? v : run;
-            uint32_t m = run;
-            for (int o = kWave / 2; o > 0; o >>= 1) {
-                uint32_t t = __shfl_xor(m, o);
-                m = t < m ? t : m;
+            uint32_t *row = cmr + (size_t)j * K * ncell + c;
+            for (int k = 0; k < K; k++) {
+                uint32_t m = kNone;
+#pragma unroll
+                for (int i = 0; i < (int)kCoreSerial; i++)
+                    if (lv[i] <= k && kq[i] < m) m = kq[i];
+                row[(size_t)k * ncell] = m;
             }
-            if (m != kNone && mn == K) mn = k;
-            if (lane == 0) row[(size_t)k * ncell] = m;
-        }
-        if (lane == 0) {
             cellkc[(size_t)j * ncell + c] = (uint8_t)mn;
             cellkc_cm[(size_t)c * J + j] = (uint8_t)mn;
+        }
+    }
+    uint64_t mb = ballot(big);
+    int lane = lane_id();
+    uint32_t *col = lmin + (size_t)wave_id_in_block() * K * kWave + lane;
+    while (mb) {
+        int bit = __builtin_ctzll(mb);
+        mb &= mb - 1;
+        uint32_t cc = c - (uint32_t)lane + (uint32_t)bit;
+        uint32_t ca = __shfl(a, bit), ce = __shfl(e, bit);
+        for (int j = 0; j < J; j++) {
+            for (int k = 0; k < K; k++) col[k * kWave] = kNone;
+            for (uint32_t q = ca + lane; q < ce; q += kWave) {
+                int lv = kc[(size_t)q * J + j];
+                if (lv < K) {
+                    uint32_t kv = key[q];
+                    uint32_t cur = col[lv * kWave];
+                    col[lv * kWave] = kv < cur ? kv : cur;
+                }
+            }
+            uint32_t *row = cmr + (size_t)j * K * ncell + cc;
+            uint32_t run = kNone;
+            int mn = K;
+            for (int k = 0; k < K; k++) {
+                uint32_t v = col[k * kWave];
+                run = v < run ? v : run;
+                uint32_t m = run;
+                for (int o = kWave / 2; o > 0; o >>= 1) {
+                    uint32_t t = __shfl_xor(m, o);
+                    m = t < m ? t : m;
+                }
+                if (m != kNone && mn == K) mn = k;
+                if (lane == 0) row[(size_t)k * ncell] = m;
+            }
+            if (lane == 0) {
+                cellkc[(size_t)j * ncell + cc] = (uint8_t)mn;
+                cellkc_cm[(size_t)cc * J + j] = (uint8_t)mn;
+            }
         }
     }
 }
@@ -1538,8 +1576,8 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     k_caller_of_pos<<<gn, kBlock, 0, c.stream()>>>(G.pos_rank, P.mapping, n, caller_of_pos);
     VDB_HIP(hipGetLastError());
     if (std_rule)
-        k_cell_core_std<<<ceil_div(G.ncell, kWPB), kBlock,
-                          sizeof(uint32_t) * kWPB * (size_t)K * kWave, c.stream()>>>(
+        k_cell_core_std<<<gc0, kBlock, sizeof(uint32_t) * kWPB * (size_t)K * kWave,
+                          c.stream()>>>(
             G.cell_start, G.ncell, kc, J, K, caller_of_pos, cmr, cellkc, cellkc_cm);
     else
         k_cell_core<<<gc0, kBlock, 0, c.stream()>>>(G.cell_start, G.ncell, kc, J, K, G.pos_rank,
