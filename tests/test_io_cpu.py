@@ -87,3 +87,19 @@ def test_missing_file_is_an_error(tmp_path):
         _lib.read_points_csv(str(tmp_path / "nope.csv"))
     with pytest.raises(_lib.VdbError, match="cannot open"):
         _lib.read_variants_csv(str(tmp_path / "nope.csv"))
+
+
+def test_cli_usage_errors_need_no_device(tmp_path):
+    # argument errors end before any device call: exit 2 with the usage text,
+    # which names every option (the standard border rule's included)
+    import subprocess
+    cli = os.path.join(os.path.dirname(_lib.LIB_PATH), os.pardir, "bin", "vdbscan")
+    for args in (["--bogus"], ["--points"], ["--points", "x.csv"],
+                 ["--points", "x.csv", "--eps", "0.5"], ["--path", "sideways"]):
+        r = subprocess.run([cli] + args, capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2, args
+        assert "usage: vdbscan" in r.stderr
+    r = subprocess.run([cli, "--bogus"], capture_output=True, text=True, timeout=60)
+    for opt in ("--points", "--lonlat", "--variants", "--eps", "--minpts", "--out",
+                "--harness", "--path", "--fp32", "--standard", "--verbose"):
+        assert opt in r.stderr, opt
