@@ -59,6 +59,12 @@ def lib():
         L.vdbo_libVDBSCAN.argtypes = [dp, dp, ctypes.c_uint32, dp, u32p, ctypes.c_uint32,
                                       ctypes.c_int, u32p, ctypes.c_int]
         L.vdbo_libVDBSCAN.restype = ctypes.c_int
+        L.vdbo_levels_closed.argtypes = [dp, dp, ctypes.c_uint32, ctypes.c_double, u32p,
+                                         ctypes.c_uint32, u32p]
+        L.vdbo_levels_closed.restype = ctypes.c_int
+        L.vdbo_libVDBSCAN_reuse.argtypes = [dp, dp, ctypes.c_uint32, dp, u32p, ctypes.c_uint32,
+                                            u32p]
+        L.vdbo_libVDBSCAN_reuse.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -138,6 +144,38 @@ def libvdbscan(x, y, eps, minpts, mbbsize=70, closed=True):
     if rc:
         raise ValueError("oracle libVDBSCAN failed (rc=%d)" % rc)
     return ret.reshape(v, n)
+
+
+def libvdbscan_reuse(x, y, eps, minpts):
+    """The reference's multi-variant call with greedy cluster reuse as it runs
+    with one search thread (deterministic): retArr layout, shape (V, n)."""
+    x, y = _f64(x), _f64(y)
+    eps = _f64(np.atleast_1d(eps))
+    mp = np.ascontiguousarray(np.atleast_1d(minpts), dtype=np.uint32)
+    n, v = len(x), len(eps)
+    ret = np.zeros(n * v, dtype=np.uint32)
+    rc = lib().vdbo_libVDBSCAN_reuse(_p(x, ctypes.c_double), _p(y, ctypes.c_double), n,
+                                     _p(eps, ctypes.c_double), _p(mp, ctypes.c_uint32), v,
+                                     _p(ret, ctypes.c_uint32))
+    if rc:
+        raise ValueError("oracle libVDBSCAN_reuse failed (rc=%d)" % rc)
+    return ret.reshape(v, n)
+
+
+def levels_closed(x, y, eps, minpts):
+    """Labels (len(minpts), n) of the variants (eps, m) for every m in minpts,
+    retArr layout; the same labels as libvdbscan(x, y, [eps]*k, minpts) but
+    without a neighbour table (O(n * k) memory), for full-size workloads."""
+    x, y = _f64(x), _f64(y)
+    mp = np.ascontiguousarray(np.atleast_1d(minpts), dtype=np.uint32)
+    n = len(x)
+    out = np.zeros(len(mp) * n, dtype=np.uint32)
+    rc = lib().vdbo_levels_closed(_p(x, ctypes.c_double), _p(y, ctypes.c_double), n,
+                                  float(eps), _p(mp, ctypes.c_uint32), len(mp),
+                                  _p(out, ctypes.c_uint32))
+    if rc:
+        raise ValueError("oracle levels_closed failed (rc=%d)" % rc)
+    return out.reshape(len(mp), n)
 
 
 def standard_labels(x, y, eps, minpts):

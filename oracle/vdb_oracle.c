@@ -21,6 +21,10 @@
  *                        oracle/_ref single-variant runs on seeded datasets.
  *   vdbo_dbscan_closed   equal to vdbo_dbscan_literal on every dataset tested.
  *   vdbo_libVDBSCAN      oracle/_ref single-variant calls, bit-exact labels.
+ *   vdbo_levels_closed   equal to vdbo_libVDBSCAN (literal and closed) on the
+ *                        test datasets; equal to oracle/_ref on the 500,000-point
+ *                        64-variant and 1,000,000-point skewed workloads
+ *                        (tests/golden/make_full_digests.py checks it there).
  *
  * Known divergence from the reference, by construction: the reference only
  * distance-tests points whose R-tree box group (MBBsize consecutive binned
@@ -143,8 +147,19 @@ static void grid_free(grid_t *g)
     free(g->cell);
 }
 
+/* cells of side max(eps / k, range / cap); a query then scans R =
+ * ceil(eps / side) + 1 cells each way (grid_radius) */
+static int grid_build_k(const double *x, const double *y, uint32_t n, double eps, int k,
+                        double cap, grid_t *g);
+
 static int grid_build(const double *x, const double *y, uint32_t n, double eps,
                       grid_t *g)
+{
+    return grid_build_k(x, y, n, eps, 1, 2048.0, g);
+}
+
+static int grid_build_k(const double *x, const double *y, uint32_t n, double eps, int k,
+                        double cap, grid_t *g)
 {
     memset(g, 0, sizeof(*g));
     double minx = INFINITY, maxx = -INFINITY, miny = INFINITY, maxy = -INFINITY;
@@ -156,7 +171,7 @@ static int grid_build(const double *x, const double *y, uint32_t n, double eps,
     }
     if (n == 0) { minx = maxx = miny = maxy = 0.0; }
     double range = fmax(maxx - minx, maxy - miny);
-    double cs = fmax(eps, range / 2048.0);
+    double cs = fmax(eps / k, range / cap);
     g->minx = minx;
     g->miny = miny;
     if (!(cs > 0.0) || !isfinite(cs) || !isfinite(range)) {
@@ -196,15 +211,15 @@ static int grid_build(const double *x, const double *y, uint32_t n, double eps,
 
 /* Visit every j with dist2(i,j) <= T; callback-free: writes ids to out (if
  * non-NULL) and returns the count. */
-static int64_t grid_query(const grid_t *g, const double *x, const double *y,
-                          uint32_t i, double T, int32_t *out)
+static int64_t grid_query_r(const grid_t *g, const double *x, const double *y,
+                            uint32_t i, double T, int64_t R, int32_t *out)
 {
     if (T < 0.0) return 0;
     int64_t cx = g->cell[i] / g->ncy, cy = g->cell[i] % g->ncy;
     int64_t k = 0;
-    for (int64_t ax = cx - 2; ax <= cx + 2; ax++) {
+    for (int64_t ax = cx - R; ax <= cx + R; ax++) {
         if (ax < 0 || ax >= g->ncx) continue;
-        for (int64_t ay = cy - 2; ay <= cy + 2; ay++) {
+        for (int64_t ay = cy - R; ay <= cy + R; ay++) {
             if (ay < 0 || ay >= g->ncy) continue;
             int64_t c = ax * g->ncy + ay;
             for (int64_t p = g->start[c]; p < g->start[c + 1]; p++) {
@@ -217,6 +232,21 @@ static int64_t grid_query(const grid_t *g, const double *x, const double *y,
         }
     }
     return k;
+}
+
+static int64_t grid_query(const grid_t *g, const double *x, const double *y,
+                          uint32_t i, double T, int32_t *out)
+{
+    return grid_query_r(g, x, y, i, T, 2, out);
+}
+
+/* window radius covering eps on a grid of side cs: an accepted pair has
+ * |dx| <= eps (1 + 2^-50) and floor() moves a coordinate by less than one
+ * cell, so ceil(eps / cs) + 1 cells each way hold every neighbour */
+static int64_t grid_radius(const grid_t *g, double eps)
+{
+    if (g->ncx <= 1 && g->ncy <= 1) return 1;
+    return (int64_t)ceil(eps / g->cs) + 1;
 }
 
 static int cmp_i32(const void *a, const void *b)
@@ -378,6 +408,118 @@ int vdbo_dbscan_closed(const double *xb, const double *yb, uint32_t n,
     free(cid);
     grid_free(&g);
     return (int)ncl + 1;
+}
+
+/* ------------------------------------- several minpts at one eps, closed */
+
+int vdbo_levels_closed(const double *x, const double *y, uint32_t n, double eps,
+                       const uint32_t *minpts, uint32_t nmp, uint32_t *labels)
+{
+    if (n == 0 || nmp == 0) return 0;
+    double T = vdbo_threshold(eps);
+    int32_t *mapping = (int32_t *)malloc(sizeof(int32_t) * n);
+    double *xb = (double *)malloc(sizeof(double) * n);
+    double *yb = (double *)malloc(sizeof(double) * n);
+    uint32_t *cnt = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    int32_t *nb = (int32_t *)malloc(sizeof(int32_t) * n);
+    int32_t *parent = (int32_t *)malloc(sizeof(int32_t) * (size_t)n * nmp);
+    uint32_t *cid = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    grid_t g;
+    memset(&g, 0, sizeof g);
+    int rc = 0;
+    if (!mapping || !xb || !yb || !cnt || !nb || !parent || !cid) { rc = -2; goto done; }
+    if (vdbo_bin_order(x, y, n, 300, mapping)) { rc = -1; goto done; }   /* src/main.cpp:107 */
+    for (uint32_t k = 0; k < n; k++) {
+        xb[k] = x[mapping[k]];
+        yb[k] = y[mapping[k]];
+    }
+    if (grid_build_k(xb, yb, n, eps, 4, 8192.0, &g)) { rc = -2; goto done; }
+    int64_t R = grid_radius(&g, eps);
+    /* neighbour counts, self included (src/DBScan.cpp:547) */
+    for (uint32_t i = 0; i < n; i++) cnt[i] = (uint32_t)grid_query_r(&g, xb, yb, i, T, R, NULL);
+    for (size_t t = 0; t < (size_t)n * nmp; t++) parent[t] = (int32_t)(t % n);
+    /* core components of every minpts value in one pass over the pairs; a
+     * root is its component's minimum binned index */
+    for (uint32_t i = 0; i < n; i++) {
+        int anycore = 0;
+        for (uint32_t j = 0; j < nmp; j++) anycore |= is_core(cnt[i], minpts[j]);
+        if (!anycore) continue;
+        int64_t c = grid_query_r(&g, xb, yb, i, T, R, nb);
+        for (uint32_t j = 0; j < nmp; j++) {
+            if (!is_core(cnt[i], minpts[j])) continue;
+            int32_t *par = parent + (size_t)j * n;
+            for (int64_t k = 0; k < c; k++) {
+                int32_t q = nb[k];
+                /* the predicate is symmetric: each unordered pair once */
+                if (q <= (int32_t)i || !is_core(cnt[q], minpts[j])) continue;
+                int32_t a = uf_find(par, (int32_t)i), b = uf_find(par, q);
+                if (a == b) continue;
+                if (a < b) par[b] = a; else par[a] = b;
+            }
+        }
+    }
+    for (uint32_t j = 0; j < nmp; j++) {
+        int32_t *par = parent + (size_t)j * n;
+        uint32_t *out = labels + (size_t)j * n;
+        uint32_t ncl = 0;
+        for (uint32_t i = 0; i < n; i++)
+            if (is_core(cnt[i], minpts[j]) && uf_find(par, (int32_t)i) == (int32_t)i)
+                cid[i] = ++ncl;
+        for (uint32_t i = 0; i < n; i++) {
+            uint32_t lab;
+            if (is_core(cnt[i], minpts[j])) {
+                lab = cid[uf_find(par, (int32_t)i)];
+            } else {
+                /* border: smallest adjacent root, only if it opens before i */
+                int64_t c = grid_query_r(&g, xb, yb, i, T, R, nb);
+                int32_t best = -1;
+                for (int64_t k = 0; k < c; k++) {
+                    int32_t q = nb[k];
+                    if (!is_core(cnt[q], minpts[j])) continue;
+                    int32_t r = uf_find(par, q);
+                    if (best < 0 || r < best) best = r;
+                }
+                lab = (best >= 0 && (uint32_t)best < i) ? cid[best] : 0;
+            }
+            out[mapping[i]] = lab;                               /* src/main.cpp:255-262 */
+        }
+    }
+done:
+    free(mapping);
+    free(xb);
+    free(yb);
+    free(cnt);
+    free(nb);
+    free(parent);
+    free(cid);
+    grid_free(&g);
+    return rc;
+}
+
+/* ------------------------------------- grid handle for other oracle files */
+
+void *vdbo_grid_new(const double *x, const double *y, uint32_t n, double eps)
+{
+    grid_t *g = (grid_t *)malloc(sizeof(grid_t));
+    if (!g) return NULL;
+    if (grid_build(x, y, n, eps, g)) {
+        free(g);
+        return NULL;
+    }
+    return g;
+}
+
+int64_t vdbo_grid_neighbors(const void *g, const double *x, const double *y, uint32_t i,
+                            double T, int32_t *out)
+{
+    return grid_query((const grid_t *)g, x, y, i, T, out);
+}
+
+void vdbo_grid_free(void *g)
+{
+    if (!g) return;
+    grid_free((grid_t *)g);
+    free(g);
 }
 
 /* --------------------------------------------------------- libVDBSCAN */

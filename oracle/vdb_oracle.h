@@ -63,6 +63,15 @@ int vdbo_dbscan_literal(const double *xb, const double *yb, uint32_t n,
 int vdbo_dbscan_closed(const double *xb, const double *yb, uint32_t n,
                        double eps, uint32_t minpts, uint32_t *labels);
 
+/* The same labels as vdbo_libVDBSCAN(use_closed=1) for the nmp variants
+ * (eps, minpts[j]) of one eps, without storing a neighbour table: one count
+ * pass and one union pass over the pairs serve every minpts value, memory is
+ * O(n * nmp). labels[j*n + i] = label of caller point i under minpts[j]. For
+ * workloads whose pair count rules out a table (or the reference's own
+ * expansion stack, which holds every core point's neighbour list). */
+int vdbo_levels_closed(const double *x, const double *y, uint32_t n, double eps,
+                       const uint32_t *minpts, uint32_t nmp, uint32_t *labels);
+
 /* Full libVDBSCAN restatement (src/main.cpp:50-299) with every variant
  * clustered from scratch (the reference's answer for a single-variant call).
  * retArr[v*n + i] = label of caller point i under caller variant v.
@@ -70,6 +79,24 @@ int vdbo_dbscan_closed(const double *xb, const double *yb, uint32_t n,
 int vdbo_libVDBSCAN(const double *x, const double *y, uint32_t n,
                     const double *eps, const uint32_t *minpts, uint32_t nvar,
                     int mbb_size, uint32_t *retArr, int use_closed);
+
+/* Multi-variant libVDBSCAN with the reference's greedy cluster reuse, as the
+ * reference runs it with one search thread (NSEARCHTHREADS=1, where it is
+ * deterministic): variants in compareByEpsilon order, each one seeded from
+ * the last finished variant with minpts >= its own (Schedule::schedGreedy,
+ * src/schedule.cpp:68-116), clustered by assignPointsToPredefinedCluster
+ * (src/DBScan.cpp:272-416, 157-205, 226-267, 429-518), or from scratch when
+ * there is none. Literal restatement (vdb_oracle_reuse.cpp). retArr layout
+ * as vdbo_libVDBSCAN. Returns 0, or < 0 on error. */
+int vdbo_libVDBSCAN_reuse(const double *x, const double *y, uint32_t n, const double *eps,
+                          const uint32_t *minpts, uint32_t nvar, uint32_t *retArr);
+
+/* Grid over points for other oracle files: every j with
+ * vdbo_dist2(i, j) <= T, written to out (n entries) when non-NULL. */
+void *vdbo_grid_new(const double *x, const double *y, uint32_t n, double eps);
+int64_t vdbo_grid_neighbors(const void *g, const double *x, const double *y, uint32_t i,
+                            double T, int32_t *out);
+void vdbo_grid_free(void *g);
 
 #ifdef __cplusplus
 }

@@ -6,7 +6,7 @@ the output of the reference's own ``libVDBSCAN`` (one variant per process, the
 reference's exact scratch semantics), so the fixtures pin results without
 the reference being present at test time.
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [--reuse-only]
 """
 import os
 import sys
@@ -79,9 +79,51 @@ def edge_sets():
     return sets
 
 
+def reuse_sets():
+    """Multi-variant datasets for the reuse emulation: name -> (x, y, eps, minpts).
+    Variant orders are shuffled so the reference's sort decides the schedule."""
+    sets = {}
+    x, y = datasets.plain_10k(seed=1)
+    sets["plain10k"] = (x, y, np.array([1.0, 0.5, 0.5, 1.0, 1.5, 2.0]),
+                        np.array([4, 4, 12, 10, 8, 30], dtype=np.uint32))
+    rng = np.random.default_rng(11)
+    for k, n in enumerate([8_000, 20_000]):
+        x, y = datasets.aniso(n, seed=100 + k, n_blobs=12 + 10 * k)
+        E = [0.25, 0.4, 0.7, 1.1][: 3 + k]
+        M = [3, 6, 12, 24][: 3 + k]
+        eps = np.array([e for e in E for m in M])
+        mp = np.array([m for e in E for m in M], dtype=np.uint32)
+        p = rng.permutation(len(eps))
+        sets["aniso%dk" % (n // 1000)] = (x, y, eps[p], mp[p])
+    # equal variants, minpts 0 and 1, and minpts above INT_MAX (negative int)
+    x, y = datasets.blobs_isotropic(6_000, 8, 0.4, 0.8, seed=77)
+    sets["special"] = (x, y, np.array([0.5, 0.5, 0.8, 0.3, 0.8, 0.5]),
+                       np.array([6, 6, 0, 1, 0x80000000, 3], dtype=np.uint32))
+    return sets
+
+
+def write_reuse_fixture():
+    """tests/golden/reuse_t1.npz: multi-variant labels of the reference built
+    with NSEARCHTHREADS=1 (its greedy reuse is then deterministic)."""
+    out = {}
+    for name, (x, y, eps, mp) in reuse_sets().items():
+        lab, _ = oracle.run_reference(x, y, eps, mp, threads=1)
+        out[name + "_x"] = np.asarray(x, dtype=np.float64)
+        out[name + "_y"] = np.asarray(y, dtype=np.float64)
+        out[name + "_eps"] = eps
+        out[name + "_minpts"] = mp
+        out[name + "_labels"] = lab
+    np.savez_compressed(os.path.join(HERE, "reuse_t1.npz"), **out)
+    print("wrote reuse_t1.npz")
+
+
 def main():
     if not oracle.ref_available(1) or not oracle.ref_available(8):
         raise SystemExit("build oracle/_ref first: make -C oracle ref")
+    if "--reuse-only" in sys.argv:
+        write_reuse_fixture()
+        return
+    write_reuse_fixture()
     out = {}
     # the shipped 225-point example is kept as CSV (test_dataset.csv / test_output.csv)
     x, y = datasets.plain_10k(seed=1)

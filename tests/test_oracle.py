@@ -160,6 +160,39 @@ def test_fixture_edge_cases_against_oracle(golden_dir):
                 assert (o == e[nm + "_labels"][v]).all(), (nm, v, closed)
 
 
+def test_levels_closed_equals_libvdbscan(golden_dir):
+    # the table-free multi-minpts form against the per-variant restatements,
+    # on the golden fixtures (their labels come from the compiled reference)
+    # and on random blobs at several eps, minpts including 0, 1 and > INT_MAX
+    g = np.load(os.path.join(golden_dir, "plain_10k.npz"))
+    x, y = g["plain10k_x"], g["plain10k_y"]
+    for eps in sorted(set(g["plain10k_eps"].tolist())):
+        sel = np.flatnonzero(g["plain10k_eps"] == eps)
+        mp = g["plain10k_minpts"][sel]
+        got = oracle.levels_closed(x, y, eps, mp)
+        for r, v in enumerate(sel):
+            assert (got[r] == g["plain10k_labels"][v]).all(), (eps, v)
+    e = np.load(os.path.join(golden_dir, "edge_cases.npz"))
+    for nm in sorted(set(k.rsplit("_", 1)[0] for k in e.files)):
+        for v in range(len(e[nm + "_eps"])):
+            got = oracle.levels_closed(e[nm + "_x"], e[nm + "_y"], e[nm + "_eps"][v],
+                                       [e[nm + "_minpts"][v]])[0]
+            assert (got == e[nm + "_labels"][v]).all(), (nm, v)
+    rng = np.random.default_rng(33)
+    for trial in range(3):
+        n = int(rng.integers(1000, 5000))
+        x = np.concatenate([rng.normal(rng.uniform(-50, 50), 0.5, n // 2),
+                            rng.uniform(-60, 60, n - n // 2)])
+        y = np.concatenate([rng.normal(rng.uniform(-50, 50), 0.5, n // 2),
+                            rng.uniform(-60, 60, n - n // 2)])
+        mps = [0, 1, 3, 8, 17, 0x80000000]
+        for eps in (0.05, 0.3, 1.1):
+            got = oracle.levels_closed(x, y, eps, mps)
+            for r, m in enumerate(mps):
+                lit = oracle.libvdbscan(x, y, [eps], [m], closed=False)[0]
+                assert (got[r] == lit).all(), (trial, eps, m)
+
+
 @pytest.mark.skipif(not oracle.ref_available(1), reason="oracle/_ref not built")
 def test_oracle_against_compiled_reference_random():
     rng = np.random.default_rng(21)
@@ -172,6 +205,36 @@ def test_oracle_against_compiled_reference_random():
         for eps, mp in [(0.2, 4), (0.5, 8), (1.5, 12)]:
             ref, _ = oracle.run_reference(x, y, [eps], [mp])
             assert (oracle.libvdbscan(x, y, [eps], [mp]) == ref).all()
+
+
+def test_reuse_oracle_against_reference_fixture(golden_dir):
+    # tests/golden/reuse_t1.npz: multi-variant labels of the compiled reference
+    # built with NSEARCHTHREADS=1, where its greedy reuse is deterministic
+    g = np.load(os.path.join(golden_dir, "reuse_t1.npz"))
+    names = sorted(set(k.rsplit("_", 1)[0] for k in g.files))
+    assert len(names) >= 4
+    differs = 0
+    for nm in names:
+        x, y, eps, mp = g[nm + "_x"], g[nm + "_y"], g[nm + "_eps"], g[nm + "_minpts"]
+        got = oracle.libvdbscan_reuse(x, y, eps, mp)
+        assert (got == g[nm + "_labels"]).all(), nm
+        differs += int((got != oracle.libvdbscan(x, y, eps, mp)).any(axis=1).sum())
+    # the fixtures exercise the reuse path: many variants differ from scratch
+    assert differs >= 10
+
+
+@pytest.mark.skipif(not oracle.ref_available(1), reason="oracle/_ref not built")
+def test_reuse_oracle_against_compiled_reference_random():
+    rng = np.random.default_rng(5)
+    for trial in range(3):
+        n = int(rng.integers(2000, 6000))
+        c = rng.uniform(-20, 20, (8, 2))[rng.integers(0, 8, n // 2)]
+        x = np.concatenate([rng.normal(c[:, 0], 0.3), rng.uniform(-25, 25, n - n // 2)])
+        y = np.concatenate([rng.normal(c[:, 1], 0.3), rng.uniform(-25, 25, n - n // 2)])
+        eps = rng.choice([0.2, 0.35, 0.5, 0.8], 6)
+        mp = rng.choice([2, 4, 7, 15, 30], 6).astype(np.uint32)
+        ref, _ = oracle.run_reference(x, y, eps, mp, threads=1)
+        assert (oracle.libvdbscan_reuse(x, y, eps, mp) == ref).all(), trial
 
 
 # ------------------------------------------------- standard border rule oracle
