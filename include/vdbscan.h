@@ -47,6 +47,14 @@ int libVDBSCAN(double *inputx, double *inputy, unsigned int datasetSize,
 
 /* flags */
 #define VDB_DEVICE_PTRS 1u /* point / output arrays are device pointers */
+/* Multi-variant calls: the labels of the reference's greedy cluster reuse
+ * (src/main.cpp:170-251, src/schedule.cpp:68-116, src/DBScan.cpp:157-518) as
+ * the reference gives them with one search thread (NSEARCHTHREADS=1, where
+ * its schedule is deterministic): variants in compareByEpsilon order, each
+ * one seeded from the last earlier variant with minpts >= its own. Without
+ * the flag every variant gets its one-variant (scratch) labels. Reference
+ * border rule and finite eps only; vdb_cluster only. */
+#define VDB_REUSE_REFERENCE 2u
 
 /* path selection for vdb_cluster */
 #define VDB_PATH_AUTO 0    /* batched engine when applicable, else fused */
@@ -104,6 +112,22 @@ int vdb_version(void);
 int vdb_set_stream(void *hip_stream);
 /* Free the pooled device memory of the current device. */
 int vdb_release(void);
+
+/* Times the launches of one kernel of the library (its name, e.g. "k_knn",
+ * "k_sweep", "k_fill"; NULL or "" stops) with a pair of HIP events on the
+ * stream it runs on; vdb_kernel_time returns the summed duration in ms and
+ * the launch count since the last read (or since vdb_time_kernel). For
+ * roofline figures of the kernel over a measured region. */
+int vdb_time_kernel(const char *name);
+int vdb_kernel_time(double *ms, uint64_t *launches);
+
+/* Autotuner state as JSON: {"cache": "<file>", "tuned_in_process": N,
+ * "entries": {"<device>:<workload key>": choice, ...}}, the choices read from
+ * the tuning cache file and those made in this process (the per-variant
+ * grid's cells per eps, the batched engine's grid shape). Writes at most cap
+ * bytes (NUL-terminated) and sets *len to the full length. No counterpart in
+ * the reference. */
+int vdb_tuning(char *buf, uint64_t cap, uint64_t *len);
 
 /* Cluster n points under nvar variants. labels has nvar*n entries laid out
  * as retArr of libVDBSCAN. flags & VDB_DEVICE_PTRS: x, y, labels are device

@@ -1,9 +1,13 @@
 import os
 import sys
+import tempfile
 
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+# the library's tuning cache: one file per test session, not the user's
+os.environ.setdefault("VDB_TUNE_CACHE",
+                      os.path.join(tempfile.gettempdir(), "vdb_tune_tests_%d.json" % os.getpid()))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 

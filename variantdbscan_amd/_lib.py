@@ -14,6 +14,7 @@ LIB_NAME = "libSharedVDBSCAN.so"
 LIB_PATH = os.path.join(LIB_DIR, LIB_NAME)
 
 DEVICE_PTRS = 1
+REUSE_REFERENCE = 2
 
 PATH_AUTO, PATH_FUSED, PATH_TABLE, PATH_BATCH = 0, 1, 2, 3
 PREC_FP64, PREC_FP32_RECHECK = 0, 1
@@ -22,7 +23,8 @@ BORDER_REFERENCE, BORDER_STANDARD = 0, 1
 EXPORTED = ("libVDBSCAN", "vdb_last_error", "vdb_version", "vdb_set_stream", "vdb_release",
             "vdb_cluster", "vdb_rank_order", "vdb_neighbor_counts", "vdb_neighbor_table",
             "vdb_neighbor_table_digest", "vdb_cluster_multi", "vdb_read_xy_csv",
-            "vdb_read_lonlat_csv", "vdb_read_variants_csv", "vdb_write_labels_csv", "vdb_free")
+            "vdb_read_lonlat_csv", "vdb_read_variants_csv", "vdb_write_labels_csv", "vdb_free",
+            "vdb_time_kernel", "vdb_kernel_time", "vdb_tuning")
 
 
 class VdbOptions(ctypes.Structure):
@@ -105,6 +107,13 @@ def load():
     L.vdb_write_labels_csv.restype = ctypes.c_int
     L.vdb_free.argtypes = [vp]
     L.vdb_free.restype = None
+    L.vdb_time_kernel.argtypes = [ctypes.c_char_p]
+    L.vdb_time_kernel.restype = ctypes.c_int
+    L.vdb_kernel_time.argtypes = [ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_uint64)]
+    L.vdb_kernel_time.restype = ctypes.c_int
+    L.vdb_tuning.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+    L.vdb_tuning.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -128,13 +137,16 @@ def _u32(a):
 
 
 def cluster(x, y, eps, minpts, path=PATH_AUTO, precision=PREC_FP64, verbose=False,
-            table_budget_bytes=0, return_stats=False, cells_per_eps=0, border=BORDER_REFERENCE):
+            table_budget_bytes=0, return_stats=False, cells_per_eps=0, border=BORDER_REFERENCE,
+            reuse=False):
     """Labels (V, n) for host arrays; retArr layout of libVDBSCAN.
     cells_per_eps (1..4) sets the per-variant grid's cells per eps; 0 lets
     the library choose. border=BORDER_REFERENCE gives the reference's uint32
     labels (0 = noise, clusters 1..); BORDER_STANDARD gives int32 labels with
     textbook DBSCAN borders (-1 = noise, clusters 0.. by smallest core index,
-    scikit-learn's labelling)."""
+    scikit-learn's labelling). reuse=True gives the labels of the
+    reference's greedy cluster reuse with one search thread
+    (VDB_REUSE_REFERENCE)."""
     L = load()
     x, y = _f64(x), _f64(y)
     eps = _f64(np.atleast_1d(eps))
@@ -146,8 +158,8 @@ def cluster(x, y, eps, minpts, path=PATH_AUTO, precision=PREC_FP64, verbose=Fals
     opt = VdbOptions(path, precision, table_budget_bytes, 1 if verbose else 0, cells_per_eps,
                      border)
     st = VdbStats()
-    _check(L.vdb_cluster(_ptr(x), _ptr(y), n, _ptr(eps), _ptr(mp), v, _ptr(out), 0,
-                         ctypes.byref(opt), ctypes.byref(st)))
+    _check(L.vdb_cluster(_ptr(x), _ptr(y), n, _ptr(eps), _ptr(mp), v, _ptr(out),
+                         REUSE_REFERENCE if reuse else 0, ctypes.byref(opt), ctypes.byref(st)))
     lab = out.reshape(v, n)
     if border == BORDER_STANDARD:
         lab = lab.view(np.int32)
@@ -166,6 +178,29 @@ def cluster_device(x_ptr, y_ptr, n, eps, minpts, labels_ptr, path=PATH_AUTO,
                          _ptr(mp), len(eps), ctypes.c_void_p(labels_ptr), DEVICE_PTRS,
                          ctypes.byref(opt), ctypes.byref(st) if stats else None))
     return st.as_dict()
+
+
+def time_kernel(name):
+    """Start timing every launch of the named library kernel (None stops)."""
+    _check(load().vdb_time_kernel(name.encode() if name else None))
+
+
+def kernel_time():
+    """(ms, launches) of the timed kernel since the last read."""
+    ms, n = ctypes.c_double(0), ctypes.c_uint64(0)
+    _check(load().vdb_kernel_time(ctypes.byref(ms), ctypes.byref(n)))
+    return ms.value, n.value
+
+
+def tuning():
+    """The autotuner's state: {"cache": path, "tuned_in_process": N,
+    "entries": {key: choice}}."""
+    import json
+    n = ctypes.c_uint64(0)
+    _check(load().vdb_tuning(None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    _check(load().vdb_tuning(buf, n.value + 1, ctypes.byref(n)))
+    return json.loads(buf.value.decode())
 
 
 def rank_order(x, y):

@@ -15,6 +15,7 @@ struct BatchResult {
     uint32_t list_len = 0;      // truncated list length (largest minpts)
     uint64_t listed_points = 0; // points that needed a neighbour list
     uint64_t cell_pairs = 0;
+    int shape = 0;              // grid shape the autotuner chose (index)
 };
 
 // Variant v of dataset d is written to out + seg[d]*nrows + rows[v]*n_d (for

@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -43,16 +44,22 @@ constexpr int kMaxLevels = 64;
 constexpr int kMaxMinpts = 64;
 // pairs whose cells hold at most this many point pairs are tested by one thread
 constexpr uint32_t kSmallPairWork = 16;
-// grid of the worklist kernels, whose sizes live on the device: one block
-// per kPairsPerBlock pairs becoming eligible at the level, within
-// [kMinPersistent, kMaxPersistent] (more waves than the work needs only
-// contend on the union-find)
-constexpr unsigned kPairsPerBlock = 8192;
-constexpr unsigned kMinPersistent = 512;
-constexpr unsigned kMaxPersistent = 1024;
-// the cell-pair test kernel: waves per test list size, and its largest grid
-constexpr unsigned kTestsPerBlock = 64;
-constexpr unsigned kMaxTestBlocks = 2048;
+// Grids of the worklist kernels, whose sizes live on the device (one grid
+// row per minpts value): the sweep takes one block per `ppb` pairs becoming
+// eligible at the level, within [minb, maxb] (more waves than the work needs
+// only contend on the union-find); the test kernel one block per `tpb` tests,
+// within [minb, maxtb]. The autotuner picks one of kBatchShapes per workload.
+struct BatchShape {
+    unsigned maxb, minb, ppb, tpb, maxtb;
+};
+constexpr BatchShape kBatchShapes[] = {
+    {1024, 512, 8192, 64, 2048}, // best with one minpts value per launch
+    {512, 256, 8192, 64, 1024},
+    {256, 128, 8192, 32, 1024},
+    {1024, 256, 4096, 32, 2048},
+};
+constexpr int kNumBatchShapes = sizeof(kBatchShapes) / sizeof(kBatchShapes[0]);
+constexpr unsigned kMaxPersistent = 1024; // largest maxb of kBatchShapes
 
 struct FG {
     const double *xs, *ys, *box;
@@ -704,18 +711,21 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-// parent[i] = i, compmin[i] = NONE, and two arrays cleared (counters,
-// snapshot root bits)
-__global__ void k_init_parent(uint32_t *parent, uint32_t *compmin, uint32_t n, uint32_t *z1,
-                              uint32_t n1, uint32_t *z2, uint32_t n2)
+// For every minpts value j (blockIdx.y): parent[j][c] = c and compmin[j][c] =
+// NONE over its ncell cells; two flat arrays cleared (counters, snapshot root
+// bits) by a grid-stride over all blocks.
+__global__ void k_init_parent(uint32_t *parent, uint32_t *compmin, uint32_t ncell, uint32_t *z1,
+                              size_t n1, uint32_t *z2, size_t n2)
 {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) {
-        parent[i] = i;
-        compmin[i] = kNone;
+    if (i < ncell) {
+        parent[(size_t)blockIdx.y * ncell + i] = i;
+        compmin[(size_t)blockIdx.y * ncell + i] = kNone;
     }
-    if (i < n1) z1[i] = 0;
-    if (i < n2) z2[i] = 0;
+    size_t f = ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * kBlock + threadIdx.x;
+    size_t stride = (size_t)gridDim.y * gridDim.x * kBlock;
+    for (size_t t = f; t < n1; t += stride) z1[t] = 0;
+    for (size_t t = f; t < n2; t += stride) z2[t] = 0;
 }
 
 // Append v to list[] for every lane with take set: one atomic per wave.
@@ -748,6 +758,51 @@ struct LevelArgs {
     double T;
 };
 
+// The union-find state of every minpts value of a level pass: minpts value j
+// (blockIdx.y of the level kernels) owns its parents, carried and test lists
+// and counters at fixed strides. The pairs are sorted once for all minpts
+// values by (j, eligibility level): order holds pair indices (bit 31: exact
+// level) and bucket j*(K+1)+k of bstart is the pairs of j becoming eligible
+// at level k.
+struct LevelLists {
+    uint32_t *parent;        // j: parent + j*ncell
+    const PairRec *lin;      // carried into the level: j at lin + j*lstride
+    PairRec *lout;           // carried out of it
+    PairRec *near, *far;     // test lists: j at + j*tcap
+    uint32_t *cnt;           // j at cnt + j*3*(K+1): keep[K+1], near[K+1], far[K+1]
+    const uint32_t *order;   // sorted pair order of all minpts values
+    const uint32_t *bstart;  // J*(K+1)+1 bucket starts into order
+    size_t lstride;
+    uint32_t ncell, tcap, K;
+    uint32_t *ovf;
+    uint32_t ppb, minb, tpb; // grid shape (BatchShape)
+};
+
+struct LevelJ {
+    uint32_t *parent;
+    const PairRec *carried;
+    PairRec *keep, *near, *far;
+    uint32_t *nc, *nkeep, *nnear, *nfar;
+    const uint32_t *bstart; // buckets of this j: bstart[k]..bstart[k+1]
+};
+
+__device__ __forceinline__ LevelJ level_j(const LevelLists &L, uint32_t j, int k)
+{
+    LevelJ r;
+    r.parent = L.parent + (size_t)j * L.ncell;
+    r.carried = L.lin + (size_t)j * L.lstride;
+    r.keep = L.lout + (size_t)j * L.lstride;
+    r.near = L.near + (size_t)j * L.tcap;
+    r.far = L.far + (size_t)j * L.tcap;
+    uint32_t *c = L.cnt + (size_t)j * 3u * (L.K + 1);
+    r.nc = c + k;
+    r.nkeep = c + k + 1;
+    r.nnear = c + (L.K + 1) + k;
+    r.nfar = c + 2 * (L.K + 1) + k;
+    r.bstart = L.bstart + (size_t)j * (L.K + 1);
+    return r;
+}
+
 __device__ bool small_pair_connected(const LevelArgs &a, uint32_t A, uint32_t B)
 {
     uint32_t a0 = a.cell_start[A], a1 = a.cell_start[A + 1];
@@ -766,7 +821,9 @@ __device__ bool small_pair_connected(const LevelArgs &a, uint32_t A, uint32_t B)
 // Eligibility level of every pair for the minpts values j0 .. j0+nj-1
 // (nj <= kEligGroup): the first level at which both cells hold a core point
 // and some pair of their boxes is within eps. K marks pairs that never
-// become eligible. Output g of the group goes to key/val + g*npairs.
+// become eligible. Output g of the group goes to key/val + g*npairs, the key
+// being (jrel0 + g)*(K+1) + level: the pass's sort key over minpts value and
+// level.
 // For pairs of small cells (at most kSmallPairWork point pairs) the level is
 // exact instead: the first level at which some core-core pair is within eps,
 //   min over (p, q) of max(level(s(p,q)), kc_j(p), kc_j(q)),
@@ -777,7 +834,7 @@ __global__ void k_pair_elig(const uint32_t *pa, const uint32_t *pb, const uint16
                             const uint8_t *cellkc_cm, const uint32_t *cell_start,
                             const double *xs, const double *ys, const uint8_t *kc,
                             const double *Tk, int J, int j0, int nj, int K, uint32_t npairs,
-                            uint32_t *key, uint32_t *val)
+                            uint32_t jrel0, uint32_t *key, uint32_t *val)
 {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= npairs) return;
@@ -831,7 +888,9 @@ __global__ void k_pair_elig(const uint32_t *pa, const uint32_t *pb, const uint16
 #pragma unroll
     for (int g = 0; g < kEligGroup; g++) {
         if (g >= nj) break;
-        key[(size_t)g * npairs + i] = (uint32_t)(e[g] < K ? e[g] : K);
+        // one sort key for all minpts values of the pass: (j, level)
+        key[(size_t)g * npairs + i] =
+            (jrel0 + (uint32_t)g) * (uint32_t)(K + 1) + (uint32_t)(e[g] < K ? e[g] : K);
         val[(size_t)g * npairs + i] = i | (((exact >> g) & 1u) ? 0x80000000u : 0u);
     }
 }
@@ -856,6 +915,26 @@ __device__ __forceinline__ void wave_append_rec(bool take, const PairRec &v, Pai
     uint32_t base = 0;
     if (lane_id() == leader) base = atomicAdd(count, (uint32_t)popc64(m));
     base = __shfl(base, leader);
+    if (take) list[base + mask_rank(m)] = v;
+}
+
+// wave_append_rec into a list of cap records: records that would not fit are
+// dropped and *ovf is set (the host then hands the batch to another engine).
+// The counter may pass cap; readers clamp it.
+__device__ __forceinline__ void wave_append_cap(bool take, const PairRec &v, PairRec *list,
+                                                uint32_t *count, uint32_t cap, uint32_t *ovf)
+{
+    uint64_t m = ballot(take);
+    if (!m) return;
+    int leader = __builtin_ctzll(m);
+    uint32_t h = (uint32_t)popc64(m);
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(count, h);
+    base = __shfl(base, leader);
+    if (base > cap || cap - base < h) {
+        if (lane_id() == leader) atomicOr(ovf, 1u);
+        return;
+    }
     if (take) list[base + mask_rank(m)] = v;
 }
 
@@ -893,7 +972,9 @@ __device__ __forceinline__ void chunk_append(bool take, const PairRec &v, PairRe
         uint32_t at = 0;
         if (lane_id() == 0) at = atomicAdd(count, kRecChunk);
         at = __shfl(at, 0);
-        if (at > cap - kRecChunk) { // cannot happen within the sizing above
+        // cannot happen within the sizing above (unless the host forced a
+        // smaller capacity)
+        if (cap < kRecChunk || at > cap - kRecChunk) {
             if (lane_id() == 0) atomicOr(ovf, 1u);
             return;
         }
@@ -905,15 +986,16 @@ __device__ __forceinline__ void chunk_append(bool take, const PairRec &v, PairRe
     ch.left -= h;
 }
 
-// Blocks of a worklist grid that take part at level k: one per
-// kPairsPerBlock pairs becoming eligible at the level, within
-// [kMinPersistent, launched grid] (more waves than the work needs only
-// contend on the union-find). The bucket starts stay on the device.
-__device__ __forceinline__ uint32_t level_blocks(const uint32_t *bstart, int k)
+// Blocks of a worklist grid that take part at level k: one per ppb pairs
+// becoming eligible at the level, within [minb, launched grid] (more waves
+// than the work needs only contend on the union-find). The bucket starts
+// stay on the device.
+__device__ __forceinline__ uint32_t level_blocks(const uint32_t *bstart, int k, uint32_t ppb,
+                                                 uint32_t minb)
 {
     uint32_t nfresh = bstart[k + 1] - bstart[k];
-    uint32_t want = (nfresh + kPairsPerBlock - 1) / kPairsPerBlock;
-    want = want > kMinPersistent ? want : kMinPersistent;
+    uint32_t want = (nfresh + ppb - 1) / ppb;
+    want = want > minb ? want : minb;
     return want < gridDim.x ? want : gridDim.x;
 }
 
@@ -926,15 +1008,19 @@ __device__ __forceinline__ uint32_t level_blocks(const uint32_t *bstart, int k)
 // next level, or handed to a wave test. Tests of grid-adjacent cells go to
 // `near`, the rest to `far`; the near tests run first so most far pairs are
 // already connected when theirs run.
-__global__ void __launch_bounds__(kBlock)
-    k_sweep(LevelArgs a, const PairRec *carried, const uint32_t *ncarried, const uint32_t *order,
-            const uint32_t *bstart, uint32_t *parent, PairRec *keep, uint32_t *nkeep,
-            PairRec *near, uint32_t *nnear, PairRec *far, uint32_t *nfar, uint32_t tcap,
-            uint32_t *ovf)
+__global__ void __launch_bounds__(kBlock) k_sweep(LevelArgs a, LevelLists L)
 {
+    a.j += (int)blockIdx.y; // global minpts index (kc); lists use the pass-relative one
+    LevelJ lj = level_j(L, blockIdx.y, a.k);
+    const PairRec *carried = lj.carried;
+    const uint32_t *ncarried = lj.nc, *bstart = lj.bstart, *order = L.order;
+    uint32_t *parent = lj.parent;
+    PairRec *keep = lj.keep, *near = lj.near, *far = lj.far;
+    uint32_t *nkeep = lj.nkeep, *nnear = lj.nnear, *nfar = lj.nfar, *ovf = L.ovf;
+    const uint32_t tcap = L.tcap;
     RecChunk chn, chf;
     bool chunked = bstart[a.k + 1] - bstart[a.k] >= kChunkedFrom;
-    uint32_t nb = level_blocks(bstart, a.k);
+    uint32_t nb = level_blocks(bstart, a.k, L.ppb, L.minb);
     if (blockIdx.x >= nb) return;
     const uint32_t *fresh = order + bstart[a.k];
     uint32_t nfresh = bstart[a.k + 1] - bstart[a.k];
@@ -1017,8 +1103,8 @@ __global__ void __launch_bounds__(kBlock)
             chunk_append(tnear, pr, near, nnear, tcap, ovf, chn);
             chunk_append(tfar, pr, far, nfar, tcap, ovf, chf);
         } else {
-            wave_append_rec(tnear, pr, near, nnear);
-            wave_append_rec(tfar, pr, far, nfar);
+            wave_append_cap(tnear, pr, near, nnear, tcap, ovf);
+            wave_append_cap(tfar, pr, far, nfar, tcap, ovf);
         }
     }
     chunk_close(chn, near);
@@ -1098,11 +1184,15 @@ __device__ void test_pair(const LevelArgs &a, const PairRec &pr, uint32_t *paren
 
 // the near tests, then the far ones, in one grid-stride pass (waves take
 // them in order, so most far tests start after the near ones)
-__global__ void __launch_bounds__(kBlock)
-    k_test(LevelArgs a, const uint32_t *bstart, const PairRec *near, const uint32_t *nnear,
-           const PairRec *far, const uint32_t *nfar, uint32_t *parent, PairRec *list_out,
-           uint32_t *nout, uint32_t tcap, const uint32_t *ovf)
+__global__ void __launch_bounds__(kBlock) k_test(LevelArgs a, LevelLists L)
 {
+    a.j += (int)blockIdx.y; // global minpts index (kc); lists use the pass-relative one
+    LevelJ lj = level_j(L, blockIdx.y, a.k);
+    const uint32_t *bstart = lj.bstart, *nnear = lj.nnear, *nfar = lj.nfar, *ovf = L.ovf;
+    const PairRec *near = lj.near, *far = lj.far;
+    uint32_t *parent = lj.parent, *nout = lj.nkeep;
+    PairRec *list_out = lj.keep;
+    const uint32_t tcap = L.tcap;
     // a test list that ran out of room (reported to the host) is not read
     if (*ovf) return;
     // tests are latency-bound waves: one block per kTestsPerBlock tests,
@@ -1111,8 +1201,8 @@ __global__ void __launch_bounds__(kBlock)
     nn = nn < tcap ? nn : tcap;
     nf = nf < tcap ? nf : tcap;
     uint32_t nt = nn + nf;
-    uint32_t nb = (nt + kTestsPerBlock - 1) / kTestsPerBlock;
-    nb = nb > kMinPersistent ? nb : kMinPersistent;
+    uint32_t nb = (nt + L.tpb - 1) / L.tpb;
+    nb = nb > L.minb ? nb : L.minb;
     nb = nb < gridDim.x ? nb : gridDim.x;
     if (blockIdx.x >= nb) return;
     if (bstart[a.k + 1] - bstart[a.k] < kChunkedFrom) {
@@ -1149,9 +1239,36 @@ __global__ void __launch_bounds__(kBlock)
 // giant component takes one atomic per wave rather than one per cell; after
 // the two most common roots of the wave the remaining lanes, spread over
 // small components, update theirs one atomic each.
-__global__ void k_roots_min(uint32_t *parent, const uint32_t *cmrjk, const uint8_t *ckj,
-                            uint32_t ncell, int k, uint32_t *compmin)
+// Snapshots of one level k for every minpts value j in jmask (blockIdx.y = j):
+// j's parents, compmin and per-cell tables sit at j*ncell, its core ranks at
+// cmr + (j*K + k)*ncell, and its snapshot takes slot sbase + (number of
+// minpts values of jmask below j).
+struct SnapArgs {
+    uint32_t *parent, *compmin;
+    const uint32_t *cmr;
+    const uint8_t *cellkc;
+    uint32_t *root, *lab, *bits, *pre; // slot s: root/lab + s*ncell, bits/pre + s*nw
+    uint64_t jmask;
+    uint32_t sbase, ncell, nw, K;
+    int k;
+};
+
+__device__ __forceinline__ bool snap_slot(const SnapArgs &S, uint32_t j, uint32_t *slot)
 {
+    if (!((S.jmask >> j) & 1ull)) return false;
+    *slot = S.sbase + (uint32_t)__popcll(S.jmask & ((1ull << j) - 1ull));
+    return true;
+}
+
+__global__ void k_roots_min(SnapArgs S)
+{
+    uint32_t j = blockIdx.y, slot;
+    if (!snap_slot(S, j, &slot)) return;
+    uint32_t *parent = S.parent + (size_t)j * S.ncell, *compmin = S.compmin + (size_t)j * S.ncell;
+    const uint32_t *cmrjk = S.cmr + ((size_t)j * S.K + S.k) * S.ncell;
+    const uint8_t *ckj = S.cellkc + (size_t)j * S.ncell;
+    const uint32_t ncell = S.ncell;
+    const int k = S.k;
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     uint32_t root = kNone;
     // a cell without a core point at level k is in no eligible pair yet, so
@@ -1187,11 +1304,17 @@ __global__ void k_roots_min(uint32_t *parent, const uint32_t *cmrjk, const uint8
 
 // Snapshot of level k: root rank of every cell (NONE if the cell holds no
 // core point), and bit r of bits set for every root rank r.
-__global__ void k_snapshot(const uint32_t *parent, const uint32_t *compmin, const uint8_t *ckj,
-                           uint32_t ncell, int k, uint32_t *rootrank, uint32_t *bits)
+__global__ void k_snapshot(SnapArgs S)
 {
+    uint32_t j = blockIdx.y, slot;
+    if (!snap_slot(S, j, &slot)) return;
+    const uint32_t *parent = S.parent + (size_t)j * S.ncell;
+    const uint32_t *compmin = S.compmin + (size_t)j * S.ncell;
+    const uint8_t *ckj = S.cellkc + (size_t)j * S.ncell;
+    uint32_t *rootrank = S.root + (size_t)slot * S.ncell, *bits = S.bits + (size_t)slot * S.nw;
+    const int k = S.k;
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= ncell) return;
+    if (c >= S.ncell) return;
     bool core = ckj[c] <= k;
     uint32_t r = core ? parent[c] : c;
     uint32_t m = core ? compmin[r] : kNone;
@@ -1233,12 +1356,17 @@ __device__ __forceinline__ uint32_t roots_below(const uint32_t *bits, const uint
 // root, counted from the first root of the cell's dataset (base 1: the
 // reference's numbering; 0: the standard rule's)
 // (also clears compmin for the next snapshot: it is no longer read)
-__global__ void k_cell_labels(const uint32_t *rootrank, const uint32_t *bits,
-                              const uint32_t *pre, const uint32_t *cell_ds, const uint32_t *seg,
-                              uint32_t ncell, uint32_t base, uint32_t *celllab, uint32_t *compmin)
+__global__ void k_cell_labels(SnapArgs S, const uint32_t *cell_ds, const uint32_t *seg,
+                              uint32_t base)
 {
+    uint32_t j = blockIdx.y, slot;
+    if (!snap_slot(S, j, &slot)) return;
+    const uint32_t *rootrank = S.root + (size_t)slot * S.ncell;
+    const uint32_t *bits = S.bits + (size_t)slot * S.nw, *pre = S.pre + (size_t)slot * S.nw;
+    uint32_t *celllab = S.lab + (size_t)slot * S.ncell;
+    uint32_t *compmin = S.compmin + (size_t)j * S.ncell;
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= ncell) return;
+    if (c >= S.ncell) return;
     compmin[c] = kNone;
     uint32_t rr = rootrank[c];
     celllab[c] = rr == kNone ? 0u
@@ -1427,6 +1555,14 @@ struct Ev {
     }
 };
 
+uint64_t splitmix(uint64_t z)
+{
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
 int read_u32(Context &c, const uint32_t *d, uint32_t *h)
 {
     void *hs;
@@ -1553,10 +1689,13 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_TRY(c.get("b.lst_s", lsz, &lst_s));
     VDB_TRY(c.get("b.lst_id", lsz, &lst_id));
     VDB_TRY(c.get("b.lst_n", nlist ? nlist : 1, &lst_n));
-    if (nlist)
+    if (nlist) {
+        VDB_TRY(c.kt_begin("k_knn"));
         k_knn<<<ceil_div(nlist, kWPB), kBlock, 0, c.stream()>>>(fview(G), list_pos, nlist,
                                                                 (int)L, Th[K - 1], Th[0], lst_s,
                                                                 lst_id, lst_n);
+        VDB_TRY(c.kt_end("k_knn"));
+    }
     VDB_HIP(hipGetLastError());
     uint8_t *kc, *cellkc;
     uint32_t *cmr;
@@ -1594,8 +1733,10 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     unsigned gc = ceil_div(G.ncell, kBlock);
     unsigned gcw = ceil_div(G.ncell, kWPB); // a wave per cell
     int W = 2 * R + 1;
+    VDB_TRY(c.kt_begin("k_pair_count"));
     k_pair_count<<<gc, kBlock, 0, c.stream()>>>(G.cell_box, G.row_lo, G.row_hi, W, G.ncell, dTk,
                                                 K, pcnt);
+    VDB_TRY(c.kt_end("k_pair_count"));
     VDB_HIP(hipGetLastError());
     VDB_TRY(scan_exclusive_u32_u64(c, pcnt, poff, G.ncell, poff + G.ncell));
     uint64_t npairs64 = 0;
@@ -1618,8 +1759,6 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_TRY(c.get("b.pa", psz, &pa));
     VDB_TRY(c.get("b.pb", psz, &pb));
     VDB_TRY(c.get("b.pk", psz, &pk));
-    VDB_TRY(c.get("b.la", psz, &la));
-    VDB_TRY(c.get("b.lb", psz, &lb));
     // test lists are filled in chunks (see kRecChunk): room for the records
     // plus the chunks' unused slots
     size_t tcap64 = psz + psz / 2 + (size_t)kMaxPersistent * kWPB * kRecChunk + kRecChunk;
@@ -1628,33 +1767,15 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         return kErrUnsupported;
     }
     uint32_t tcap = (uint32_t)tcap64;
-    VDB_TRY(c.get("b.tests", tcap, &tests));
-    VDB_TRY(c.get("b.tests2", tcap, &tests2));
-    uint32_t *tovf;
-    VDB_TRY(c.get("b.tovf", 1, &tovf));
-    k_fill_u32<<<1, kBlock, 0, c.stream()>>>(tovf, 0u, 1);
-    VDB_HIP(hipGetLastError());
-    // per level: carried-in list size, near and far test counts (level k
-    // reads carried[k], writes carried[k+1])
-    const uint32_t ncnt = 3 * ((uint32_t)K + 1);
-    VDB_TRY(c.get("b.cnts", ncnt, &cnts));
-    uint32_t *c_keep = cnts, *c_near = cnts + K + 1, *c_far = cnts + 2 * (K + 1);
-    uint32_t *ekey, *eval, *stk, *stv, *bstart;
-    // eligibility keys of up to kEligGroup minpts values at once
-    const int jgroup = std::min(J, kEligGroup);
-    VDB_TRY(c.get("b.ekey", psz * jgroup, &ekey));
-    VDB_TRY(c.get("b.eval", psz * jgroup, &eval));
-    VDB_TRY(c.get("b.stk", psz, &stk));
-    VDB_TRY(c.get("b.stv", psz, &stv));
-    VDB_TRY(c.get("b.bstart", (size_t)K + 2, &bstart));
-    k_pairs<<<gcw, kBlock, 0, c.stream()>>>(G.cell_box, G.row_lo, G.row_hi, W, G.ncell, dTk, K,
-                                            poff, pa, pb, pk);
-    VDB_HIP(hipGetLastError());
+    // VDB_BATCH_TEST_CAP (records, testing only) shrinks the test lists so
+    // that the overflow fallback can be exercised on small inputs
+    if (const char *ev = getenv("VDB_BATCH_TEST_CAP")) {
+        unsigned long long v = strtoull(ev, nullptr, 10);
+        if (v > 0 && v < tcap) tcap = (uint32_t)v;
+    }
     uint32_t *pos_of_caller, *labtmp;
     VDB_TRY(c.get("b.poc", n, &pos_of_caller));
     VDB_TRY(c.get("b.labtmp", (size_t)kLabelSlots * n, &labtmp));
-    k_pos_of_caller<<<gn, kBlock, 0, c.stream()>>>(caller_of_pos, n, pos_of_caller);
-    VDB_HIP(hipGetLastError());
     if (st) st->ms_prepare += tpairs.ms();
 
     // ---- variants of each (k, j)
@@ -1665,21 +1786,49 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         int j = (int)(std::lower_bound(mps.begin(), mps.end(), minpts[v]) - mps.begin());
         want[{k, j}].push_back(v);
     }
-    uint32_t *parent, *compmin;
-    VDB_TRY(c.get("b.parent", G.ncell, &parent));
-    VDB_TRY(c.get("b.compmin", G.ncell, &compmin));
-    // per-level snapshots of one j: cell root ranks, the root ranks as a
-    // bitset over ranks with its per-word prefix counts (cluster ids)
+    // Minpts values run side by side: every level kernel takes all minpts
+    // values of a pass at once (blockIdx.y), each with its own union-find,
+    // worklists and snapshots. A pass holds as many as the memory bound and
+    // the 32-bit sort size allow (all of them on the benchmark workloads).
     const uint32_t nw = (n + 31) / 32;
+    const uint32_t ncnt = 3 * ((uint32_t)K + 1);
+    const size_t per_j = 8 * (size_t)G.ncell + 2 * sizeof(PairRec) * (psz + (size_t)tcap) +
+                         16 * psz + 4 * (size_t)ncnt;
+    const size_t kPassBytes = (size_t)64 << 30;
+    int jc = J;
+    while (jc > 1 && ((uint64_t)jc * psz > 0x7fffffffull || (size_t)jc * per_j > kPassBytes))
+        jc = (jc + 1) / 2;
+    // snapshot slots: at most one per requested (k, j) of a pass
+    size_t max_slots = 1;
+    for (int j0 = 0; j0 < J; j0 += jc) {
+        size_t ns = 0;
+        for (auto &w : want)
+            if (w.first.second >= j0 && w.first.second < j0 + jc) ns++;
+        max_slots = std::max(max_slots, ns);
+    }
+    uint32_t *parent, *compmin, *ekey, *eval, *stk, *stv, *bstart, *tovf;
+    VDB_TRY(c.get("b.parent", (size_t)jc * G.ncell, &parent));
+    VDB_TRY(c.get("b.compmin", (size_t)jc * G.ncell, &compmin));
+    VDB_TRY(c.get("b.la", (size_t)jc * psz, &la));
+    VDB_TRY(c.get("b.lb", (size_t)jc * psz, &lb));
+    VDB_TRY(c.get("b.tests", (size_t)jc * tcap, &tests));
+    VDB_TRY(c.get("b.tests2", (size_t)jc * tcap, &tests2));
+    VDB_TRY(c.get("b.cnts", (size_t)jc * ncnt, &cnts));
+    VDB_TRY(c.get("b.ekey", (size_t)jc * psz, &ekey));
+    VDB_TRY(c.get("b.eval", (size_t)jc * psz, &eval));
+    VDB_TRY(c.get("b.stk", (size_t)jc * psz, &stk));
+    VDB_TRY(c.get("b.stv", (size_t)jc * psz, &stv));
+    VDB_TRY(c.get("b.bstart", (size_t)jc * (K + 1) + 2, &bstart));
+    VDB_TRY(c.get("b.tovf", 1, &tovf));
+    // per-slot snapshots: cell root ranks, cell labels, the root ranks as a
+    // bitset over ranks with its per-word prefix counts (cluster ids)
     uint32_t *snap_root, *snap_lab, *snap_bits, *snap_pre;
-    VDB_TRY(c.get("b.snap_root", (size_t)K * G.ncell, &snap_root));
-    VDB_TRY(c.get("b.snap_lab", (size_t)K * G.ncell, &snap_lab));
-    VDB_TRY(c.get("b.snap_bits", (size_t)K * nw, &snap_bits));
-    VDB_TRY(c.get("b.snap_pre", (size_t)K * nw, &snap_pre));
-    Ev tuf(c.stream());
-    double ms_labels = 0;
-    // label-pass events per j, read once the batch is done (reading them
-    // inside the loop would make the host wait once per minpts value)
+    VDB_TRY(c.get("b.snap_root", max_slots * G.ncell, &snap_root));
+    VDB_TRY(c.get("b.snap_lab", max_slots * G.ncell, &snap_lab));
+    VDB_TRY(c.get("b.snap_bits", max_slots * nw, &snap_bits));
+    VDB_TRY(c.get("b.snap_pre", max_slots * nw, &snap_pre));
+    // label-pass events, read once the batch is done (reading them inside
+    // the loop would make the host wait once per pass)
     struct EventList {
         std::vector<hipEvent_t> ev;
         ~EventList()
@@ -1688,143 +1837,239 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         }
     } lab;
     std::vector<hipEvent_t> &lab_ev = lab.ev;
-    int jg0 = -1, jgn = 0; // minpts values whose keys ekey/eval hold
-    for (int j = 0; j < J; j++) {
-        int kmax = -1;
-        for (auto &w : want)
-            if (w.first.second == j) kmax = std::max(kmax, w.first.first);
-        if (kmax < 0) continue;
-        uint32_t nbits = (uint32_t)K * nw;
-        k_init_parent<<<std::max({gc, ceil_div(ncnt, kBlock), ceil_div(nbits, kBlock)}), kBlock, 0,
-                        c.stream()>>>(parent, compmin, G.ncell, cnts, ncnt, snap_bits, nbits);
+    // Every launch from here on depends only on the call's shape and the
+    // pooled buffers: no host read, no allocation. With `events`, the label
+    // passes are bracketed by events (not inside a graph).
+    auto sequence = [&](bool events, const BatchShape &shape) -> int {
+        VDB_TRY(c.kt_begin("k_pairs"));
+        k_pairs<<<gcw, kBlock, 0, c.stream()>>>(G.cell_box, G.row_lo, G.row_hi, W, G.ncell, dTk, K,
+                                                poff, pa, pb, pk);
+        VDB_TRY(c.kt_end("k_pairs"));
         VDB_HIP(hipGetLastError());
-        // pair records bucketed by the level at which they become eligible
-        std::vector<uint32_t> bst(K + 2, 0);
-        const uint32_t *order = stv; // level-sorted pair order (unread without pairs)
-        if (npairs) {
-            if (j < jg0 || j >= jg0 + jgn) {
-                jg0 = j;
-                jgn = std::min(kEligGroup, J - j);
-                k_pair_elig<<<ceil_div(npairs, kBlock), kBlock, 0, c.stream()>>>(
-                    pa, pb, pk, cellkc_cm, G.cell_start, G.xs, G.ys, kc, dTk, J, jg0, jgn,
-                    K, npairs, ekey, eval);
-                VDB_HIP(hipGetLastError());
+        k_pos_of_caller<<<gn, kBlock, 0, c.stream()>>>(caller_of_pos, n, pos_of_caller);
+        VDB_HIP(hipGetLastError());
+        k_fill_u32<<<1, kBlock, 0, c.stream()>>>(tovf, 0u, 1);
+        VDB_HIP(hipGetLastError());
+        for (int j0 = 0; j0 < J; j0 += jc) {
+            const int nj = std::min(jc, J - j0);
+            // requested minpts values of the pass at each level, and the slots
+            // of their snapshots (level-major)
+            std::vector<uint64_t> jmask(K, 0);
+            int kmax = -1;
+            for (auto &w : want) {
+                int k = w.first.first, j = w.first.second;
+                if (j < j0 || j >= j0 + nj) continue;
+                jmask[k] |= 1ull << (j - j0);
+                kmax = std::max(kmax, k);
             }
-            size_t g = (size_t)(j - jg0) * npairs;
-            if (K + 1 <= 16) {
-                // levels 0..K fit one counting pass, which also gives the
-                // bucket starts
-                VDB_TRY(bucket_sort_values(c, ekey + g, eval + g, stv, npairs, K + 1, bstart));
-                order = stv;
+            if (kmax < 0) continue;
+            std::vector<uint32_t> sbase(K + 1, 0);
+            for (int k = 0; k < K; k++) sbase[k + 1] = sbase[k] + (uint32_t)__builtin_popcountll(jmask[k]);
+            size_t nslot = sbase[K];
+            k_init_parent<<<dim3(gc, nj), kBlock, 0, c.stream()>>>(
+                parent, compmin, G.ncell, cnts, (size_t)nj * ncnt, snap_bits, nslot * nw);
+            VDB_HIP(hipGetLastError());
+            // pairs bucketed by (minpts value, level at which they become
+            // eligible): bucket j*(K+1)+k of bstart
+            const uint32_t nb = (uint32_t)nj * (uint32_t)(K + 1);
+            const uint32_t *order = stv; // unread without pairs
+            if (npairs) {
+                for (int g0 = 0; g0 < nj; g0 += kEligGroup) {
+                    int gn = std::min(kEligGroup, nj - g0);
+                    VDB_TRY(c.kt_begin("k_pair_elig"));
+                    k_pair_elig<<<ceil_div(npairs, kBlock), kBlock, 0, c.stream()>>>(
+                        pa, pb, pk, cellkc_cm, G.cell_start, G.xs, G.ys, kc, dTk, J, j0 + g0, gn, K,
+                        npairs, (uint32_t)g0, ekey + (size_t)g0 * npairs, eval + (size_t)g0 * npairs);
+                    VDB_TRY(c.kt_end("k_pair_elig"));
+                    VDB_HIP(hipGetLastError());
+                }
+                uint32_t nkeys = (uint32_t)nj * npairs;
+                if (nb <= 16) {
+                    // one counting pass, which also gives the bucket starts
+                    VDB_TRY(bucket_sort_values(c, ekey, eval, stv, nkeys, (int)nb, bstart));
+                    order = stv;
+                } else {
+                    uint32_t *sk, *sv;
+                    VDB_TRY(radix_sort_pairs(c, ekey, eval, stk, stv, nkeys, bits_for(nb), &sk, &sv));
+                    k_bucket_starts<<<ceil_div((size_t)nkeys + 1, kBlock), kBlock, 0, c.stream()>>>(
+                        sk, nkeys, (int)nb - 1, bstart);
+                    VDB_HIP(hipGetLastError());
+                    order = sv;
+                }
             } else {
-                uint32_t *sk, *sv;
-                VDB_TRY(radix_sort_pairs(c, ekey + g, eval + g, stk, stv, npairs,
-                                         bits_for(K + 1), &sk, &sv));
-                k_bucket_starts<<<ceil_div(npairs + 1, kBlock), kBlock, 0, c.stream()>>>(
-                    sk, npairs, K, bstart);
+                k_fill_u32<<<ceil_div((size_t)nb + 1, kBlock), kBlock, 0, c.stream()>>>(bstart, 0u,
+                                                                                       nb + 1);
                 VDB_HIP(hipGetLastError());
-                order = sv;
             }
-            if (verbose) {
-                // bucket sizes (a sync per minpts value: verbose only)
-                VDB_HIP(hipMemcpyAsync(bst.data(), bstart, sizeof(uint32_t) * (K + 2),
-                                       hipMemcpyDeviceToHost, c.stream()));
-                VDB_HIP(hipStreamSynchronize(c.stream()));
+            PairRec *lin = la, *lout = lb;
+            for (int k = 0; k <= kmax; k++) {
+                LevelArgs a{G.xs, G.ys, G.cell_box, G.cell_start, G.cell_key, G.cell_ds, G.skbase,
+                            G.sncy, kc, pa, pb, pk, J, j0, k, Th[k]};
+                LevelLists LL{parent, lin, lout, tests, tests2, cnts, order, bstart,
+                              psz, G.ncell, tcap, (uint32_t)K, tovf, shape.ppb, shape.minb,
+                              shape.tpb};
+                // worklist sizes stay on the device: bounded grids stride over them
+                VDB_TRY(c.kt_begin("k_sweep"));
+                k_sweep<<<dim3(shape.maxb, nj), kBlock, 0, c.stream()>>>(a, LL);
+                VDB_TRY(c.kt_end("k_sweep"));
+                VDB_HIP(hipGetLastError());
+                VDB_TRY(c.kt_begin("k_test"));
+                k_test<<<dim3(shape.maxtb, nj), kBlock, 0, c.stream()>>>(a, LL);
+                VDB_TRY(c.kt_end("k_test"));
+                VDB_HIP(hipGetLastError());
+                if (verbose) {
+                    // worklist sizes of this level (a sync per level: verbose only)
+                    std::vector<uint32_t> hc((size_t)nj * ncnt), hb(nb + 1);
+                    VDB_HIP(hipMemcpyAsync(hc.data(), cnts, sizeof(uint32_t) * hc.size(),
+                                           hipMemcpyDeviceToHost, c.stream()));
+                    VDB_HIP(hipMemcpyAsync(hb.data(), bstart, sizeof(uint32_t) * hb.size(),
+                                           hipMemcpyDeviceToHost, c.stream()));
+                    VDB_HIP(hipStreamSynchronize(c.stream()));
+                    for (int jr = 0; jr < nj; jr++) {
+                        const uint32_t *h = hc.data() + (size_t)jr * ncnt;
+                        uint32_t b0 = (uint32_t)jr * (K + 1) + k;
+                        printf("\n[vdbscan] batch j=%d k=%d: %u fresh, %u near + %u far tests, "
+                               "%u carried on",
+                               j0 + jr, k, hb[b0 + 1] - hb[b0], h[K + 1 + k], h[2 * (K + 1) + k],
+                               h[k + 1]);
+                    }
+                }
+                std::swap(lin, lout);
+                if (!jmask[k]) continue;
+                // snapshots of level k: component roots, their smallest core
+                // rank, cluster ids in ascending root rank
+                SnapArgs S{parent, compmin, cmr + (size_t)j0 * K * G.ncell, cellkc + (size_t)j0 * G.ncell,
+                           snap_root, snap_lab, snap_bits, snap_pre, jmask[k], sbase[k], G.ncell, nw,
+                           (uint32_t)K, k};
+                VDB_TRY(c.kt_begin("k_roots_min"));
+                k_roots_min<<<dim3(gc, nj), kBlock, 0, c.stream()>>>(S);
+                VDB_TRY(c.kt_end("k_roots_min"));
+                VDB_HIP(hipGetLastError());
+                k_snapshot<<<dim3(gc, nj), kBlock, 0, c.stream()>>>(S);
+                VDB_HIP(hipGetLastError());
+                // one scan over the level's slots: a slot's counts start at the
+                // slots before it, an offset the label passes subtract away
+                size_t ns_k = sbase[k + 1] - sbase[k];
+                VDB_TRY(scan_popc_u32(c, snap_bits + (size_t)sbase[k] * nw,
+                                      snap_pre + (size_t)sbase[k] * nw, ns_k * nw));
+                k_cell_labels<<<dim3(gc, nj), kBlock, 0, c.stream()>>>(S, G.cell_ds, P.seg,
+                                                                       std_rule ? 0u : 1u);
+                VDB_HIP(hipGetLastError());
             }
-        } else {
-            k_fill_u32<<<1, kBlock, 0, c.stream()>>>(bstart, 0u, (uint32_t)K + 2);
-            VDB_HIP(hipGetLastError());
+            // labels of every requested variant of the pass, kLabelSlots rows
+            // (one minpts value) per launch
+            if (events) {
+                hipEvent_t le0, le1;
+                VDB_HIP(hipEventCreate(&le0));
+                lab_ev.push_back(le0);
+                VDB_HIP(hipEventCreate(&le1));
+                lab_ev.push_back(le1);
+                VDB_HIP(hipEventRecord(le0, c.stream()));
+            }
+            for (int jr = 0; jr < nj; jr++) {
+                int j = j0 + jr;
+                std::vector<std::pair<int, uint32_t>> rows_j; // (level, variant)
+                for (int k = 0; k < K; k++) {
+                    if (!((jmask[k] >> jr) & 1ull)) continue;
+                    for (uint32_t v : want[{k, j}]) rows_j.push_back({k, v});
+                }
+                for (size_t r0 = 0; r0 < rows_j.size(); r0 += kLabelSlots) {
+                    LabelSlots S;
+                    memset(&S, 0, sizeof S);
+                    S.n = (int)std::min<size_t>(kLabelSlots, rows_j.size() - r0);
+                    for (int t = 0; t < S.n; t++) {
+                        int k = rows_j[r0 + t].first;
+                        size_t sl = sbase[k] + (uint32_t)__builtin_popcountll(
+                                                   jmask[k] & ((1ull << jr) - 1ull));
+                        S.k[t] = k;
+                        S.T[t] = Th[k];
+                        S.rootrank[t] = snap_root + sl * G.ncell;
+                        S.cbits[t] = snap_bits + sl * nw;
+                        S.cpre[t] = snap_pre + sl * nw;
+                        S.celllab[t] = snap_lab + sl * G.ncell;
+                        S.row[t] = rows[rows_j[r0 + t].second];
+                    }
+                    S.out = out;
+                    S.nrows = nrows;
+                    S.seg = P.seg;
+                    S.tmp = labtmp;
+                    if (std_rule) {
+                        VDB_TRY(c.kt_begin("k_labels_j"));
+                        k_labels_j<true><<<gn, kBlock, 0, c.stream()>>>(
+                            S, G.cell_of_pos, G.cell_ds, G.pos_rank, kc, J, j, lst_idx, lst_s, lst_id,
+                            lst_n, (int)L, n);
+                        VDB_TRY(c.kt_end("k_labels_j"));
+                    } else {
+                        VDB_TRY(c.kt_begin("k_labels_j"));
+                        k_labels_j<false><<<gn, kBlock, 0, c.stream()>>>(
+                            S, G.cell_of_pos, G.cell_ds, G.pos_rank, kc, J, j, lst_idx, lst_s, lst_id,
+                            lst_n, (int)L, n);
+                        VDB_TRY(c.kt_end("k_labels_j"));
+                    }
+                    VDB_HIP(hipGetLastError());
+                    VDB_TRY(c.kt_begin("k_rows_from_pos"));
+                    k_rows_from_pos<<<gn, kBlock, 0, c.stream()>>>(S, pos_of_caller, P.nsets, n);
+                    VDB_TRY(c.kt_end("k_rows_from_pos"));
+                    VDB_HIP(hipGetLastError());
+                }
+            }
+            if (events) VDB_HIP(hipEventRecord(lab_ev.back(), c.stream()));
         }
-        PairRec *lin = la, *lout = lb;
-        std::vector<int> snap_k;
-        for (int k = 0; k <= kmax; k++) {
-            LevelArgs a{G.xs, G.ys, G.cell_box, G.cell_start, G.cell_key, G.cell_ds, G.skbase,
-                        G.sncy, kc, pa, pb, pk, J, j, k, Th[k]};
-            // worklist sizes stay on the device: bounded grids stride over them
-            uint32_t nfresh = bst[k + 1] - bst[k]; // verbose only
-            k_sweep<<<kMaxPersistent, kBlock, 0, c.stream()>>>(
-                a, lin, c_keep + k, order, bstart, parent, lout, c_keep + k + 1, tests, c_near + k,
-                tests2, c_far + k, tcap, tovf);
-            VDB_HIP(hipGetLastError());
-            k_test<<<kMaxTestBlocks, kBlock, 0, c.stream()>>>(a, bstart, tests, c_near + k, tests2,
-                                                              c_far + k, parent, lout,
-                                                              c_keep + k + 1, tcap, tovf);
-            VDB_HIP(hipGetLastError());
-            if (verbose) {
-                // worklist sizes of this level (a sync per level: verbose only)
-                uint32_t hc[3];
-                VDB_HIP(hipMemcpyAsync(hc, c_near + k, 4, hipMemcpyDeviceToHost, c.stream()));
-                VDB_HIP(hipMemcpyAsync(hc + 1, c_far + k, 4, hipMemcpyDeviceToHost, c.stream()));
-                VDB_HIP(hipMemcpyAsync(hc + 2, c_keep + k + 1, 4, hipMemcpyDeviceToHost,
-                                       c.stream()));
-                VDB_HIP(hipStreamSynchronize(c.stream()));
-                printf("\n[vdbscan] batch j=%d k=%d: %u fresh, %u near + %u far tests, %u carried on",
-                       j, k, nfresh, hc[0], hc[1], hc[2]);
+        return kOk;
+    };
+    // The sequence runs from a cached graph when the same call (shape,
+    // variants, buffers) comes again: the first call runs it directly, the
+    // second captures it, later ones replay it. Not while verbose output or
+    // per-kernel timing needs host work between launches.
+    // Autotuner: the first call of a workload (sizes, levels, minpts values)
+    // runs the sequence once per grid shape of kBatchShapes and keeps the
+    // fastest; the choice persists in the tuning cache.
+    char tkey[160];
+    snprintf(tkey, sizeof tkey, "batch:%u:%u:%d:%d:%u:%d:%d", n, P.nsets, K, J, L,
+             bits_for((uint64_t)G.ncell + 1), bits_for((uint64_t)npairs + 1));
+    int shape_id = 0;
+    bool tuned_now = false;
+    if (!c.tuned_get(tkey, &shape_id) || shape_id < 0 || shape_id >= kNumBatchShapes) {
+        shape_id = 0;
+        if (n >= (1u << 17) && !verbose && !c.kt_active()) {
+            float best_ms = 0;
+            for (int t = 0; t < kNumBatchShapes; t++) {
+                Ev tt(c.stream());
+                VDB_TRY(sequence(false, kBatchShapes[t]));
+                float ms = (float)tt.ms();
+                if (t == 0 || ms < best_ms) {
+                    best_ms = ms;
+                    shape_id = t;
+                }
             }
-            std::swap(lin, lout);
-            if (want.find({k, j}) == want.end()) continue;
-            // snapshot of level k: component roots, their smallest core rank,
-            // cluster ids in ascending root rank
-            int slot = (int)snap_k.size();
-            snap_k.push_back(k);
-            uint32_t *sroot = snap_root + (size_t)slot * G.ncell;
-            uint32_t *sbits = snap_bits + (size_t)slot * nw, *spre = snap_pre + (size_t)slot * nw;
-            const uint8_t *ckj = cellkc + (size_t)j * G.ncell;
-            k_roots_min<<<gc, kBlock, 0, c.stream()>>>(
-                parent, cmr + ((size_t)j * K + k) * G.ncell, ckj, G.ncell, k, compmin);
-            VDB_HIP(hipGetLastError());
-            k_snapshot<<<gc, kBlock, 0, c.stream()>>>(parent, compmin, ckj, G.ncell, k, sroot,
-                                                      sbits);
-            VDB_HIP(hipGetLastError());
-            VDB_TRY(scan_popc_u32(c, sbits, spre, nw));
-            k_cell_labels<<<gc, kBlock, 0, c.stream()>>>(sroot, sbits, spre, G.cell_ds, P.seg,
-                                                         G.ncell, std_rule ? 0u : 1u,
-                                                         snap_lab + (size_t)slot * G.ncell,
-                                                         compmin);
-            VDB_HIP(hipGetLastError());
+            tuned_now = true; // the last run's labels are in place
+            c.tuned_put(tkey, shape_id);
         }
-        // labels of every requested variant of this j, kLabelSlots rows per launch
-        hipEvent_t le0, le1;
-        VDB_HIP(hipEventCreate(&le0));
-        lab_ev.push_back(le0);
-        VDB_HIP(hipEventCreate(&le1));
-        lab_ev.push_back(le1);
-        VDB_HIP(hipEventRecord(le0, c.stream()));
-        std::vector<std::pair<int, uint32_t>> rows_j; // (slot, variant)
-        for (size_t sl = 0; sl < snap_k.size(); sl++)
-            for (uint32_t v : want[{snap_k[sl], j}]) rows_j.push_back({(int)sl, v});
-        for (size_t r0 = 0; r0 < rows_j.size(); r0 += kLabelSlots) {
-            LabelSlots S;
-            memset(&S, 0, sizeof S);
-            S.n = (int)std::min<size_t>(kLabelSlots, rows_j.size() - r0);
-            for (int t = 0; t < S.n; t++) {
-                int sl = rows_j[r0 + t].first;
-                int k = snap_k[sl];
-                S.k[t] = k;
-                S.T[t] = Th[k];
-                S.rootrank[t] = snap_root + (size_t)sl * G.ncell;
-                S.cbits[t] = snap_bits + (size_t)sl * nw;
-                S.cpre[t] = snap_pre + (size_t)sl * nw;
-                S.celllab[t] = snap_lab + (size_t)sl * G.ncell;
-                S.row[t] = rows[rows_j[r0 + t].second];
-            }
-            S.out = out;
-            S.nrows = nrows;
-            S.seg = P.seg;
-            S.tmp = labtmp;
-            if (std_rule)
-                k_labels_j<true><<<gn, kBlock, 0, c.stream()>>>(S, G.cell_of_pos, G.cell_ds,
-                                                                G.pos_rank, kc, J, j, lst_idx,
-                                                                lst_s, lst_id, lst_n, (int)L, n);
-            else
-                k_labels_j<false><<<gn, kBlock, 0, c.stream()>>>(S, G.cell_of_pos, G.cell_ds,
-                                                                 G.pos_rank, kc, J, j, lst_idx,
-                                                                 lst_s, lst_id, lst_n, (int)L, n);
-            VDB_HIP(hipGetLastError());
-            k_rows_from_pos<<<gn, kBlock, 0, c.stream()>>>(S, pos_of_caller, P.nsets, n);
-            VDB_HIP(hipGetLastError());
+    }
+    const BatchShape &shape = kBatchShapes[shape_id];
+    if (res) res->shape = shape_id;
+    Ev tuf(c.stream());
+    double ms_labels = 0;
+    const bool graphs = !verbose && !c.kt_active() && !getenv("VDB_NO_GRAPH");
+    if (graphs || tuned_now) {
+        uint64_t h = 0x9e3779b97f4a7c15ull;
+        auto mix = [&](uint64_t v) { h = splitmix(h ^ v); };
+        for (uint64_t v : {(uint64_t)n, (uint64_t)nvar, (uint64_t)nrows, (uint64_t)border,
+                           (uint64_t)(uintptr_t)out, (uint64_t)G.ncell, (uint64_t)nlist,
+                           (uint64_t)npairs, (uint64_t)tcap, (uint64_t)P.nsets, (uint64_t)L,
+                           (uint64_t)K, (uint64_t)J, (uint64_t)jc, (uint64_t)R,
+                           (uint64_t)shape_id})
+            mix(v);
+        for (uint32_t v = 0; v < nvar; v++) {
+            uint64_t eb;
+            memcpy(&eb, &eps[v], 8);
+            mix(eb);
+            mix(((uint64_t)minpts[v] << 32) | rows[v]);
         }
-        VDB_HIP(hipEventRecord(le1, c.stream()));
+        if (tuned_now) c.graph_mark(h); // the tuning runs wrote the labels
+        else VDB_TRY(c.graph_run(h, [&]() { return sequence(false, shape); }));
+    } else {
+        VDB_TRY(sequence(st != nullptr, shape));
     }
     double ms_uf_all = tuf.ms();
     for (size_t e = 0; e + 1 < lab_ev.size(); e += 2) {
@@ -1835,8 +2080,10 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     uint32_t htovf = 0;
     VDB_TRY(read_u32(c, tovf, &htovf));
     if (htovf) {
+        // labels of this group are incomplete: the caller re-runs the group
+        // on the per-variant engine, which rewrites every one of its rows
         set_error("cell-pair test list overflow");
-        return kErrDevice;
+        return kErrUnsupported;
     }
     if (st) {
         st->ms_uf += ms_uf_all - ms_labels;

@@ -4,10 +4,12 @@
 #include "vdb_batch.h"
 #include "vdb_cluster.h"
 #include "vdb_prims.h"
+#include "vdb_reuse.h"
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -257,6 +259,10 @@ int cluster_impl(const double *x, const double *y, uint32_t n, const double *eps
         return kErrInvalid;
     }
     VDB_TRY(check_options(opt));
+    if ((flags & VDB_REUSE_REFERENCE) && opt && opt->border != VDB_BORDER_REFERENCE) {
+        set_error("the reuse emulation gives the reference's labels only (border rule 0)");
+        return kErrInvalid;
+    }
     Context *cp = nullptr;
     VDB_TRY(get_context(&cp));
     Context &c = *cp;
@@ -267,8 +273,15 @@ int cluster_impl(const double *x, const double *y, uint32_t n, const double *eps
     uint32_t *out = labels;
     if (!(flags & VDB_DEVICE_PTRS)) VDB_TRY(c.get("out.labels", (size_t)n * nvar, &out));
     int used = 0;
-    VDB_TRY(cluster_points(c, P, "pts", eps, minpts, nvar, all_variants(nvar), out, opt, st,
-                           &used));
+    if (flags & VDB_REUSE_REFERENCE) {
+        // the reference's greedy reuse: per variant, fused-engine kernels
+        VDB_TRY(cluster_reuse(c, P, eps, minpts, nvar, out, opt ? opt->precision : VDB_PREC_FP64,
+                              cells_per_eps(opt), opt && opt->verbose));
+        used = VDB_PATH_FUSED;
+    } else {
+        VDB_TRY(cluster_points(c, P, "pts", eps, minpts, nvar, all_variants(nvar), out, opt, st,
+                               &used));
+    }
     if (!(flags & VDB_DEVICE_PTRS)) {
         VDB_HIP(hipMemcpyAsync(labels, out, sizeof(uint32_t) * (size_t)n * nvar,
                                hipMemcpyDeviceToHost, c.stream()));
@@ -304,6 +317,44 @@ int vdb_release(void)
     VDB_TRY(get_context(&c));
     std::lock_guard<std::mutex> lock(c->mu);
     c->release_all();
+    return kOk;
+}
+
+int vdb_time_kernel(const char *name)
+{
+    Context *c = nullptr;
+    VDB_TRY(get_context(&c));
+    std::lock_guard<std::mutex> lock(c->mu);
+    c->kt_select(name);
+    return kOk;
+}
+
+int vdb_kernel_time(double *ms, uint64_t *launches)
+{
+    if (!ms || !launches) {
+        set_error("NULL output argument");
+        return kErrInvalid;
+    }
+    Context *c = nullptr;
+    VDB_TRY(get_context(&c));
+    std::lock_guard<std::mutex> lock(c->mu);
+    return c->kt_read(ms, launches);
+}
+
+int vdb_tuning(char *buf, uint64_t cap, uint64_t *len)
+{
+    Context *c = nullptr;
+    VDB_TRY(get_context(&c));
+    std::lock_guard<std::mutex> lock(c->mu);
+    std::string js = "{\"cache\": \"" + c->tune_cache_path() +
+                     "\", \"tuned_in_process\": " + std::to_string(c->tuned_in_process) +
+                     ", \"entries\": " + c->tuned_json() + "}";
+    if (len) *len = js.size();
+    if (buf && cap) {
+        size_t m = std::min<size_t>(cap - 1, js.size());
+        memcpy(buf, js.data(), m);
+        buf[m] = 0;
+    }
     return kOk;
 }
 
@@ -561,8 +612,12 @@ int libVDBSCAN(double *inputx, double *inputy, unsigned int datasetSize, double 
     opt.precision = VDB_PREC_FP64;
     opt.verbose = verbose ? 1 : 0;
     vdb_stats st;
+    // VDBSCAN_REUSE=1 in the environment: the reference's greedy reuse with
+    // one search thread (VDB_REUSE_REFERENCE) instead of scratch labels
+    const char *ru = getenv("VDBSCAN_REUSE");
+    uint32_t flags = (ru && ru[0] == '1') ? VDB_REUSE_REFERENCE : 0u;
     int s = vdb_cluster(inputx, inputy, datasetSize, inputEpsilon, inputMinpts, numVariants,
-                        retArr, 0, &opt, &st);
+                        retArr, flags, &opt, &st);
     if (verbose) {
         if (s == 0)
             printf("\n[vdbscan] %u points x %u variants, path %d, %.3f ms device time\n",

@@ -68,6 +68,24 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
                     uint32_t minpts, const uint32_t *counts, int mode, int border,
                     size_t table_budget, uint32_t *out, PhaseTimes *pt);
 
+// Pieces of the fused engine for callers that label differently (the reuse
+// emulation). Over positions of G:
+//   core_forest: core[p] and the flat core forest (parent[p] = position of
+//     the smallest-rank point of p's component, for core p); best[] is
+//     cleared.
+//   border_min_key: best[p] = smallest key[root] over the core neighbours of
+//     non-core p (NONE where there is none, and for core p).
+//   border_pairs: for every non-core p with cl_of_pos[p] = C != 0, the
+//     pairs (C, root) for the distinct roots among p's core neighbours, as
+//     uint32 pairs; at most cap stored, *count_dev = all (repeats possible).
+int core_forest(Context &c, const Grid &G, const Items &I, double eps, uint32_t minpts,
+                const uint32_t *counts, uint8_t *core, uint32_t *parent, uint32_t *best);
+int border_min_key(Context &c, const Grid &G, const Items &I, double eps, const uint8_t *core,
+                   const uint32_t *parent, const uint32_t *key, uint32_t *best);
+int border_pairs(Context &c, const Grid &G, const Items &I, double eps, const uint8_t *core,
+                 const uint32_t *parent, const uint32_t *cl_of_pos, uint32_t *pairs,
+                 uint32_t cap, uint32_t *count_dev);
+
 // Scatter a per-position array to caller order.
 int pos_to_caller(Context &c, const Points &P, const Grid &G, const uint32_t *by_pos,
                   uint32_t *by_caller);

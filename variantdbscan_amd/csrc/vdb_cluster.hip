@@ -611,6 +611,12 @@ __global__ void k_core_init(const uint32_t *counts, uint32_t n, uint32_t minpts,
     best[p] = kNone;
 }
 
+__global__ void k_fill_u32_n(uint32_t *a, uint32_t v, uint32_t n)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
 __global__ void __launch_bounds__(kBlock)
     k_uf_fused(GridView g, const uint32_t *__restrict__ item_cell,
                const uint32_t *__restrict__ item_o0, uint32_t nitems, double T,
@@ -741,6 +747,66 @@ __global__ void __launch_bounds__(kBlock)
     for (int b = 0; b < kOrigins; b++)
         if (lane_id() == b) mine = bst[b];
     if (lane_id() < (int)nb && !core[o0 + lane_id()]) best[o0 + lane_id()] = mine;
+}
+
+// (C, root) pairs for the reuse emulation: for every non-core position p with
+// cl[p] = C != 0, each distinct root (position) among its core neighbours.
+// The wave drops a root already emitted for the same origin (the last four
+// are remembered); any repeat left is removed by the caller. At most cap
+// pairs are stored; *count counts them all.
+__global__ void __launch_bounds__(kBlock)
+    k_border_pairs(GridView g, const uint32_t *__restrict__ item_cell,
+                   const uint32_t *__restrict__ item_o0, uint32_t nitems, double T,
+                   const uint8_t *__restrict__ core, const uint32_t *__restrict__ parent,
+                   const uint32_t *__restrict__ cl, uint32_t *pairs, uint32_t cap, uint32_t *count)
+{
+    uint32_t it = blockIdx.x * kWavesPerBlock + wave_id_in_block();
+    if (it >= nitems) return;
+    uint32_t c = item_cell[it], o0 = item_o0[it];
+    uint32_t nb = g.cell_start[c + 1] - o0;
+    nb = nb < kOrigins ? nb : kOrigins;
+    double ox[kOrigins], oy[kOrigins];
+    uint32_t clb[kOrigins], seen[kOrigins][4];
+    uint32_t any = 0;
+#pragma unroll
+    for (int b = 0; b < kOrigins; b++) {
+        uint32_t ob = o0 + (b < (int)nb ? b : nb - 1);
+        uint32_t cb = b < (int)nb && !core[ob] ? cl[ob] : 0u;
+        clb[b] = cb;
+        ox[b] = sgpr_d(cb ? g.xs[ob] : __longlong_as_double(0x7ff8000000000000ll));
+        oy[b] = sgpr_d(g.ys[ob]);
+#pragma unroll
+        for (int s = 0; s < 4; s++) seen[b][s] = kNone;
+        any |= cb;
+    }
+    if (!any) return;
+    window_blocks(g, c, false, 0, [&](uint32_t q, bool v, double qx, double qy) {
+        bool qc = v && core[q];
+        if (!ballot(qc)) return;
+        uint32_t rr = qc ? parent[q] : kNone;
+#pragma unroll
+        for (int b = 0; b < kOrigins; b++) {
+            uint32_t cand = (qc && dist2_exact(ox[b], oy[b], qx, qy) <= T) ? rr : kNone;
+#pragma unroll
+            for (int s = 0; s < 4; s++) cand = cand == seen[b][s] ? kNone : cand;
+            uint64_t m;
+            while ((m = ballot(cand != kNone)) != 0) {
+                uint32_t r = (uint32_t)__shfl((int)cand, __builtin_ctzll(m));
+                if (cand == r) cand = kNone;
+                if (lane_id() == 0) {
+                    uint32_t at = atomicAdd(count, 1u);
+                    if (at < cap) {
+                        pairs[2 * (size_t)at] = clb[b];
+                        pairs[2 * (size_t)at + 1] = r;
+                    }
+                }
+                seen[b][3] = seen[b][2];
+                seen[b][2] = seen[b][1];
+                seen[b][1] = seen[b][0];
+                seen[b][0] = r;
+            }
+        }
+    });
 }
 
 // --------------------------------------------- union-find over a CSR table
@@ -918,31 +984,48 @@ Band band_for(const Grid &G, double T)
     double D = (G.R + 2) * G.w;
     double band = D * D * std::ldexp(1.0, -14) / 9.0;
     Band b;
+    if (T < 0) {
+        b.lo = -1.0f;
+        b.hi = -1.0f;
+        return b;
+    }
+    // The bound above is relative error only. It fails where float32 leaves
+    // its normal range: differences or squares below FLT_MIN carry an
+    // absolute error up to 2^-126 each (a pair 2 eps apart can round to 0
+    // at eps = 1e-24), and squares of differences above about 2^60
+    // overflow. Outside 2^-100 <= band and 8 D^2 <= 2^120 every candidate is
+    // decided in binary64 (accept nothing unchecked, recheck everything).
+    if (!(band >= std::ldexp(1.0, -100)) || !(8.0 * D * D <= std::ldexp(1.0, 120))) {
+        b.lo = -1.0f;
+        b.hi = INFINITY;
+        return b;
+    }
     double lo = T - band, hi = T + band;
     b.lo = lo > 0 ? std::nextafter((float)lo, 0.0f) : -1.0f;
     if (b.lo > lo) b.lo = std::nextafter(b.lo, -INFINITY);
     b.hi = std::nextafter((float)hi, INFINITY);
-    if (T < 0) {
-        b.lo = -1.0f;
-        b.hi = -1.0f;
-    }
     return b;
 }
 
+// events are created on the first start(): a pass that is not timed creates
+// none
 struct EvTimer {
     hipEvent_t a = nullptr, b = nullptr;
     hipStream_t s;
-    explicit EvTimer(hipStream_t st) : s(st)
-    {
-        hipEventCreate(&a);
-        hipEventCreate(&b);
-    }
+    explicit EvTimer(hipStream_t st) : s(st) {}
     ~EvTimer()
     {
         if (a) hipEventDestroy(a);
         if (b) hipEventDestroy(b);
     }
-    void start() { hipEventRecord(a, s); }
+    void start()
+    {
+        if (!a) {
+            hipEventCreate(&a);
+            hipEventCreate(&b);
+        }
+        hipEventRecord(a, s);
+    }
     float stop()
     {
         hipEventRecord(b, s);
@@ -1013,9 +1096,9 @@ int tune_cells_per_eps(Context &c, const Points &P, double eps, int precision, i
     char key[160];
     snprintf(key, sizeof key, "cells:%u:%d:%llx:%llx:%llx:%llx:%llx", P.n, precision, bits(eps),
              bits(P.ext[0]), bits(P.ext[1]), bits(P.ext[2]), bits(P.ext[3]));
-    auto hit = c.tuned.find(key);
-    if (hit != c.tuned.end()) {
-        *k_out = hit->second;
+    int cached = 0;
+    if (c.tuned_get(key, &cached) && cached >= 1 && cached <= kMaxCellsPerEps) {
+        *k_out = cached;
         return kOk;
     }
     uint32_t *counts;
@@ -1038,7 +1121,7 @@ int tune_cells_per_eps(Context &c, const Points &P, double eps, int precision, i
             best_ms = ms;
         }
     }
-    c.tuned[key] = best;
+    c.tuned_put(key, best);
     *k_out = best;
     return kOk;
 }
@@ -1050,12 +1133,16 @@ int count_neighbors(Context &c, const Grid &G, const Items &I, double eps, int p
     if (I.n == 0) return kOk;
     unsigned blocks = ceil_div(I.n, kWavesPerBlock);
     if (precision == 0) {
+        VDB_TRY(c.kt_begin("k_count"));
         k_count<0><<<blocks, kBlock, 0, c.stream()>>>(view(G, G.npts), I.cell, I.o0, I.n, T, Band{},
                                                       counts, nullptr);
+        VDB_TRY(c.kt_end("k_count"));
     } else {
+        VDB_TRY(c.kt_begin("k_count"));
         k_count<1><<<blocks, kBlock, 0, c.stream()>>>(view(G, G.npts), I.cell, I.o0, I.n, T,
                                                       band_for(G, T), counts,
                                                       (unsigned long long *)recheck_dev);
+        VDB_TRY(c.kt_end("k_count"));
     }
     VDB_HIP(hipGetLastError());
     return kOk;
@@ -1120,17 +1207,23 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
     unsigned blocks = ceil_div(I.n, kWavesPerBlock);
     if (mode == 0) {
         if (pt) tm.start();
-        if (I.n)
+        if (I.n) {
+            VDB_TRY(c.kt_begin("k_uf_fused"));
             k_uf_fused<<<blocks, kBlock, 0, c.stream()>>>(view(G, G.npts), I.cell, I.o0, I.n, T, core,
                                                           G.pos_rank, parent);
+            VDB_TRY(c.kt_end("k_uf_fused"));
+        }
         VDB_HIP(hipGetLastError());
         k_compress<<<g, kBlock, 0, c.stream()>>>(parent, core, n);
         VDB_HIP(hipGetLastError());
         VDB_TRY(root_keys());
         if (pt) pt->uf += tm.stop(), tm.start();
-        if (I.n)
+        if (I.n) {
+            VDB_TRY(c.kt_begin("k_border_fused"));
             k_border_fused<<<blocks, kBlock, 0, c.stream()>>>(view(G, G.npts), I.cell, I.o0, I.n, T,
                                                               core, bkey, parent, best);
+            VDB_TRY(c.kt_end("k_border_fused"));
+        }
         VDB_HIP(hipGetLastError());
         if (pt) pt->border += tm.stop();
     } else {
@@ -1183,14 +1276,18 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
                 return kErrNoMemory;
             }
             if (pt) tm.start();
+            VDB_TRY(c.kt_begin("k_fill"));
             k_fill<0, false><<<ceil_div(i1 - i0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
                 view(G, G.npts), I.cell, I.o0, i0, i1, T, Band{}, offsets, base, nullptr, nullptr, ids,
                 nullptr, cap, ovf);
+            VDB_TRY(c.kt_end("k_fill"));
             VDB_HIP(hipGetLastError());
             if (pt) pt->fill += tm.stop(), tm.start();
             uint32_t p0 = hio0[i0];
+            VDB_TRY(c.kt_begin("k_uf_table"));
             k_uf_table<<<ceil_div(pend - p0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
                 offsets, base, ids, p0, pend, core, G.pos_rank, parent);
+            VDB_TRY(c.kt_end("k_uf_table"));
             VDB_HIP(hipGetLastError());
             if (pt) pt->uf += tm.stop();
             ntiles++;
@@ -1215,9 +1312,11 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
             uint32_t p0 = hio0[i0];
             if (ntiles > 1) {
                 // the table no longer holds this tile: rebuild it
+                VDB_TRY(c.kt_begin("k_fill"));
                 k_fill<0, false><<<ceil_div(i1 - i0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
                     view(G, G.npts), I.cell, I.o0, i0, i1, T, Band{}, offsets, base, nullptr, nullptr,
                     ids, nullptr, cap, ovf);
+                VDB_TRY(c.kt_end("k_fill"));
                 VDB_HIP(hipGetLastError());
             }
             k_border_table<<<ceil_div(pend - p0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
@@ -1249,6 +1348,56 @@ __global__ void k_pos_to_caller(const uint32_t *v, const uint32_t *pos_rank,
     out[mapping[pos_rank[p]]] = v[p];
 }
 } // namespace
+
+int core_forest(Context &c, const Grid &G, const Items &I, double eps, uint32_t minpts,
+                const uint32_t *counts, uint8_t *core, uint32_t *parent, uint32_t *best)
+{
+    uint32_t n = G.npts;
+    double T = threshold_for_eps(eps);
+    unsigned g = ceil_div(n, kBlock);
+    k_core_init<<<g, kBlock, 0, c.stream()>>>(counts, n, minpts, core, parent, best);
+    VDB_HIP(hipGetLastError());
+    if (I.n) {
+        VDB_TRY(c.kt_begin("k_uf_fused"));
+        k_uf_fused<<<ceil_div(I.n, kWavesPerBlock), kBlock, 0, c.stream()>>>(
+            view(G, n), I.cell, I.o0, I.n, T, core, G.pos_rank, parent);
+        VDB_TRY(c.kt_end("k_uf_fused"));
+    }
+    VDB_HIP(hipGetLastError());
+    k_compress<<<g, kBlock, 0, c.stream()>>>(parent, core, n);
+    VDB_HIP(hipGetLastError());
+    return kOk;
+}
+
+int border_min_key(Context &c, const Grid &G, const Items &I, double eps, const uint8_t *core,
+                   const uint32_t *parent, const uint32_t *key, uint32_t *best)
+{
+    uint32_t n = G.npts;
+    k_fill_u32_n<<<ceil_div(n, kBlock), kBlock, 0, c.stream()>>>(best, kNone, n);
+    VDB_HIP(hipGetLastError());
+    if (I.n) {
+        VDB_TRY(c.kt_begin("k_border_fused"));
+        k_border_fused<<<ceil_div(I.n, kWavesPerBlock), kBlock, 0, c.stream()>>>(
+            view(G, n), I.cell, I.o0, I.n, threshold_for_eps(eps), core, key, parent, best);
+        VDB_TRY(c.kt_end("k_border_fused"));
+    }
+    VDB_HIP(hipGetLastError());
+    return kOk;
+}
+
+int border_pairs(Context &c, const Grid &G, const Items &I, double eps, const uint8_t *core,
+                 const uint32_t *parent, const uint32_t *cl_of_pos, uint32_t *pairs,
+                 uint32_t cap, uint32_t *count_dev)
+{
+    k_fill_u32_n<<<1, kBlock, 0, c.stream()>>>(count_dev, 0u, 1);
+    VDB_HIP(hipGetLastError());
+    if (I.n)
+        k_border_pairs<<<ceil_div(I.n, kWavesPerBlock), kBlock, 0, c.stream()>>>(
+            view(G, G.npts), I.cell, I.o0, I.n, threshold_for_eps(eps), core, parent, cl_of_pos,
+            pairs, cap, count_dev);
+    VDB_HIP(hipGetLastError());
+    return kOk;
+}
 
 int pos_to_caller(Context &c, const Points &P, const Grid &G, const uint32_t *by_pos,
                   uint32_t *by_caller)
@@ -1284,14 +1433,19 @@ int table_digest(Context &c, const Points &P, const Grid &G, const Items &I, dou
     if (I.n) {
         unsigned blocks = ceil_div(I.n, kWavesPerBlock);
         Band bd = precision ? band_for(G, T) : Band{};
-        if (precision == 0)
+        if (precision == 0) {
+            VDB_TRY(c.kt_begin("k_fill"));
             k_fill<0, false, true><<<blocks, kBlock, 0, c.stream()>>>(
                 view(G, G.npts), I.cell, I.o0, 0, I.n, T, bd, nullptr, 0, caller_of_pos, caller_of_pos,
                 (uint32_t *)acc, nullptr, 0, nullptr);
-        else
+            VDB_TRY(c.kt_end("k_fill"));
+        } else {
+            VDB_TRY(c.kt_begin("k_fill"));
             k_fill<1, false, true><<<blocks, kBlock, 0, c.stream()>>>(
                 view(G, G.npts), I.cell, I.o0, 0, I.n, T, bd, nullptr, 0, caller_of_pos, caller_of_pos,
                 (uint32_t *)acc, nullptr, 0, nullptr);
+            VDB_TRY(c.kt_end("k_fill"));
+        }
         VDB_HIP(hipGetLastError());
     }
     unsigned long long h[2];
@@ -1348,22 +1502,33 @@ int neighbor_table(Context &c, const Points &P, const Grid &G, const Items &I, d
     VDB_HIP(hipMemsetAsync(ovf, 0, sizeof(uint32_t), c.stream()));
     unsigned blocks = ceil_div(I.n, kWavesPerBlock);
     Band bd = precision ? band_for(G, T) : Band{};
-    if (precision == 0 && !d2)
+    if (precision == 0 && !d2) {
+        VDB_TRY(c.kt_begin("k_fill"));
         k_fill<0, false><<<blocks, kBlock, 0, c.stream()>>>(
             view(G, G.npts), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
             d2, capacity, ovf);
-    else if (precision == 0)
+        VDB_TRY(c.kt_end("k_fill"));
+    }
+    else if (precision == 0) {
+        VDB_TRY(c.kt_begin("k_fill"));
         k_fill<0, true><<<blocks, kBlock, 0, c.stream()>>>(
             view(G, G.npts), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
             d2, capacity, ovf);
-    else if (!d2)
+        VDB_TRY(c.kt_end("k_fill"));
+    }
+    else if (!d2) {
+        VDB_TRY(c.kt_begin("k_fill"));
         k_fill<1, false><<<blocks, kBlock, 0, c.stream()>>>(
             view(G, G.npts), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
             d2, capacity, ovf);
-    else
+        VDB_TRY(c.kt_end("k_fill"));
+    } else {
+        VDB_TRY(c.kt_begin("k_fill"));
         k_fill<1, true><<<blocks, kBlock, 0, c.stream()>>>(
             view(G, G.npts), I.cell, I.o0, 0, I.n, T, bd, offsets, 0, caller_of_pos, caller_of_pos, ids,
             d2, capacity, ovf);
+        VDB_TRY(c.kt_end("k_fill"));
+    }
     VDB_HIP(hipGetLastError());
     if (pt) pt->fill += tm.stop();
     uint32_t hov = 0;

@@ -1,6 +1,11 @@
 #include "vdb_context.h"
 
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <memory>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <vector>
 
 namespace vdb {
@@ -12,6 +17,12 @@ const char *last_error() { return g_last_error.c_str(); }
 
 Context::Context(int device) : device_(device)
 {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess)
+        dev_tag_ = std::string(prop.gcnArchName) + ":" + std::to_string(prop.multiProcessorCount) +
+                   ":";
+    else
+        dev_tag_ = "unknown:";
     // default priority on purpose: every stream of this library stays at the
     // default priority
     if (hipStreamCreateWithFlags(&own_stream_, hipStreamNonBlocking) != hipSuccess)
@@ -21,7 +32,8 @@ Context::Context(int device) : device_(device)
 
 Context::~Context()
 {
-    release_all();
+    release_all(); // also drops the graphs
+    for (hipEvent_t e : kt_ev_) hipEventDestroy(e);
     if (host_) hipHostFree(host_);
     if (own_stream_) hipStreamDestroy(own_stream_);
 }
@@ -37,6 +49,7 @@ int Context::get(const char *name, size_t bytes, void **out)
             b.bytes = 0;
         }
         size_t want = bytes < 256 ? 256 : bytes;
+        epoch_++;
         hipError_t e = hipMalloc(&b.ptr, want);
         if (e != hipSuccess) {
             (void)hipGetLastError();
@@ -55,6 +68,7 @@ void Context::release(const char *name)
 {
     auto it = pool_.find(name);
     if (it == pool_.end()) return;
+    epoch_++;
     if (it->second.ptr) {
         hipStreamSynchronize(stream_);
         hipFree(it->second.ptr);
@@ -65,6 +79,8 @@ void Context::release(const char *name)
 void Context::release_all()
 {
     if (stream_) hipStreamSynchronize(stream_);
+    drop_graphs();
+    epoch_++;
     for (auto &kv : pool_)
         if (kv.second.ptr) hipFree(kv.second.ptr);
     pool_.clear();
@@ -96,6 +112,195 @@ int Context::host_scratch(size_t bytes, void **out)
         host_bytes_ = want;
     }
     *out = host_;
+    return kOk;
+}
+
+void Context::kt_select(const char *name)
+{
+    kt_name_ = name ? name : "";
+    kt_used_ = 0;
+}
+
+int Context::kt_begin(const char *name)
+{
+    if (!kt_on(name)) return kOk;
+    if (kt_used_ + 2 > kt_ev_.size()) {
+        for (int i = 0; i < 2; i++) {
+            hipEvent_t e;
+            VDB_HIP(hipEventCreate(&e));
+            kt_ev_.push_back(e);
+        }
+    }
+    VDB_HIP(hipEventRecord(kt_ev_[kt_used_], stream_));
+    return kOk;
+}
+
+int Context::kt_end(const char *name)
+{
+    if (!kt_on(name)) return kOk;
+    VDB_HIP(hipEventRecord(kt_ev_[kt_used_ + 1], stream_));
+    kt_used_ += 2;
+    return kOk;
+}
+
+int Context::kt_read(double *ms, uint64_t *launches)
+{
+    double tot = 0;
+    if (kt_used_) VDB_HIP(hipEventSynchronize(kt_ev_[kt_used_ - 1]));
+    for (size_t i = 0; i + 1 < kt_used_; i += 2) {
+        float m = 0;
+        VDB_HIP(hipEventElapsedTime(&m, kt_ev_[i], kt_ev_[i + 1]));
+        tot += m;
+    }
+    *ms = tot;
+    *launches = kt_used_ / 2;
+    kt_used_ = 0;
+    return kOk;
+}
+
+std::string Context::tune_cache_path() const
+{
+    const char *e = getenv("VDB_TUNE_CACHE");
+    if (e && e[0]) return strcmp(e, "off") == 0 ? std::string() : std::string(e);
+    const char *x = getenv("XDG_CACHE_HOME");
+    if (x && x[0]) return std::string(x) + "/variantdbscan_amd/tune.json";
+    const char *h = getenv("HOME");
+    if (h && h[0]) return std::string(h) + "/.cache/variantdbscan_amd/tune.json";
+    return std::string();
+}
+
+// entries of a cache file: lines  "key": value  (the format tuned_put writes)
+static void read_tune_file(const std::string &path, std::unordered_map<std::string, int> &m)
+{
+    FILE *f = fopen(path.c_str(), "r");
+    if (!f) return;
+    char line[1024];
+    while (fgets(line, sizeof line, f)) {
+        char *a = strchr(line, '"');
+        char *b = a ? strchr(a + 1, '"') : nullptr;
+        char *colon = b ? strchr(b + 1, ':') : nullptr;
+        if (!colon) continue;
+        char *end = nullptr;
+        long v = strtol(colon + 1, &end, 10);
+        if (end == colon + 1) continue;
+        m[std::string(a + 1, b)] = (int)v;
+    }
+    fclose(f);
+}
+
+void Context::tuned_load()
+{
+    if (tuned_loaded_) return;
+    tuned_loaded_ = true;
+    std::string path = tune_cache_path();
+    if (!path.empty()) read_tune_file(path, tuned_);
+}
+
+bool Context::tuned_get(const std::string &key, int *value)
+{
+    tuned_load();
+    auto it = tuned_.find(dev_tag_ + key);
+    if (it == tuned_.end()) return false;
+    *value = it->second;
+    return true;
+}
+
+std::string Context::tuned_json()
+{
+    tuned_load();
+    std::string out = "{";
+    bool first = true;
+    for (auto &kv : tuned_) {
+        out += first ? "\n" : ",\n";
+        first = false;
+        out += " \"" + kv.first + "\": " + std::to_string(kv.second);
+    }
+    out += "\n}\n";
+    return out;
+}
+
+void Context::tuned_put(const std::string &key, int value)
+{
+    tuned_load();
+    tuned_in_process++;
+    tuned_[dev_tag_ + key] = value;
+    std::string path = tune_cache_path();
+    if (path.empty()) return;
+    // merge with what other processes wrote since, then replace the file
+    std::unordered_map<std::string, int> disk;
+    read_tune_file(path, disk);
+    for (auto &kv : tuned_) disk[kv.first] = kv.second;
+    tuned_ = disk;
+    for (size_t i = path.find('/', 1); i != std::string::npos; i = path.find('/', i + 1))
+        mkdir(path.substr(0, i).c_str(), 0755);
+    std::string tmp = path + ".tmp" + std::to_string((long)getpid());
+    FILE *f = fopen(tmp.c_str(), "w");
+    if (!f) return; // an unwritable cache only costs tuning again later
+    std::string js = tuned_json();
+    fwrite(js.data(), 1, js.size(), f);
+    if (fclose(f) == 0) rename(tmp.c_str(), path.c_str());
+    else remove(tmp.c_str());
+}
+
+void Context::drop_graphs()
+{
+    for (auto &g : graphs_) hipGraphExecDestroy(g.second);
+    graphs_.clear();
+    graph_pending_ = 0;
+}
+
+// key of a launch sequence over the pool as it stands
+static uint64_t graph_key(uint64_t shape, uint64_t epoch)
+{
+    uint64_t z = shape ^ (epoch * 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 31)) * 0xbf58476d1ce4e5b9ull;
+    return z ^ (z >> 29);
+}
+
+void Context::graph_mark(uint64_t shape) { graph_pending_ = graph_key(shape, epoch_); }
+
+int Context::graph_run(uint64_t shape, const std::function<int()> &seq)
+{
+    // the NULL stream (a framework's default stream) cannot be captured
+    if (!stream_) return seq();
+    auto key_now = [&]() { return graph_key(shape, epoch_); };
+    uint64_t key = key_now();
+    for (auto &g : graphs_)
+        if (g.first == key) {
+            VDB_HIP(hipGraphLaunch(g.second, stream_));
+            return kOk;
+        }
+    if (key != graph_pending_) {
+        VDB_TRY(seq());
+        graph_pending_ = key_now(); // the pool as this run left it
+        return kOk;
+    }
+    // the same sequence over the same buffers a second time: capture it
+    uint64_t e0 = epoch_;
+    if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        graph_pending_ = 0;
+        return seq();
+    }
+    int s = seq();
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(stream_, &g);
+    hipGraphExec_t ex = nullptr;
+    bool ok = s == kOk && ce == hipSuccess && g && epoch_ == e0;
+    if (ok && hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) ok = false;
+    if (g) hipGraphDestroy(g);
+    if (!ok) {
+        // run it directly instead (a failed capture leaves nothing queued)
+        (void)hipGetLastError();
+        graph_pending_ = 0;
+        return seq();
+    }
+    if (graphs_.size() >= 4) {
+        hipGraphExecDestroy(graphs_.front().second);
+        graphs_.erase(graphs_.begin());
+    }
+    graphs_.push_back({key, ex});
+    VDB_HIP(hipGraphLaunch(ex, stream_));
     return kOk;
 }
 

@@ -5,9 +5,11 @@
 
 #include "vdb_common.h"
 
+#include <functional>
 #include <mutex>
 #include <string>
 #include <unordered_map>
+#include <vector>
 
 namespace vdb {
 
@@ -49,8 +51,40 @@ public:
 
     std::mutex mu; // serialises library calls on this device
 
-    // autotuner choices, keyed by workload (see tune_cells_per_eps)
-    std::unordered_map<std::string, int> tuned;
+    // Autotuner choices, keyed by workload. They persist in a JSON file
+    // (one "key": value per line) so that later processes skip the tuning:
+    // $VDB_TUNE_CACHE, else $XDG_CACHE_HOME/variantdbscan_amd/tune.json,
+    // else $HOME/.cache/variantdbscan_amd/tune.json; VDB_TUNE_CACHE=off
+    // keeps them in memory only. Keys carry the device's architecture and
+    // CU count.
+    bool tuned_get(const std::string &key, int *value);
+    void tuned_put(const std::string &key, int value);
+    std::string tuned_json();
+    uint32_t tuned_in_process = 0; // tunings run by this process
+    std::string tune_cache_path() const;
+
+    // Per-kernel timing: while a kernel name is selected, every launch of
+    // that kernel is bracketed by two events on the launch stream, and
+    // kt_read() sums their durations. Events are created once and reused.
+    void kt_select(const char *name);
+    bool kt_active() const { return !kt_name_.empty(); }
+    bool kt_on(const char *name) const { return !kt_name_.empty() && kt_name_ == name; }
+    int kt_begin(const char *name);
+    int kt_end(const char *name);
+    int kt_read(double *ms, uint64_t *launches);
+
+    // Launch sequences replayed from cached HIP graphs. `seq` launches on
+    // stream() with no host read and no allocation; `shape` identifies
+    // everything its launches depend on besides the pooled buffers. The
+    // first call of a (shape, pool state) runs seq directly (the pool then
+    // holds every buffer seq uses), the second captures it into a graph,
+    // later ones replay the graph. Any allocation moves the pool epoch, so a
+    // graph never outlives the buffers it was captured with.
+    int graph_run(uint64_t shape, const std::function<int()> &seq);
+    // the sequence of `shape` has just run directly (as graph_run's first call)
+    void graph_mark(uint64_t shape);
+    uint64_t pool_epoch() const { return epoch_; }
+    void drop_graphs();
 
 private:
     int device_;
@@ -59,6 +93,16 @@ private:
     std::unordered_map<std::string, Buffer> pool_;
     void *host_ = nullptr;
     size_t host_bytes_ = 0;
+    std::unordered_map<std::string, int> tuned_;
+    bool tuned_loaded_ = false;
+    std::string dev_tag_; // "<arch>:<CUs>:" prefix of tuning keys
+    void tuned_load();
+    uint64_t epoch_ = 1;            // bumped by every pool allocation and release
+    uint64_t graph_pending_ = 0;    // key of the last sequence run directly
+    std::vector<std::pair<uint64_t, hipGraphExec_t>> graphs_; // oldest first
+    std::string kt_name_;
+    std::vector<hipEvent_t> kt_ev_; // pairs (begin, end) of recorded launches
+    size_t kt_used_ = 0;            // events recorded since the last read
 };
 
 // Context of the current HIP device (created on first use).

@@ -342,11 +342,12 @@ __global__ void k_row_ranges(const uint32_t *cell_key, uint32_t ncell, const uin
                              const uint32_t *sncy, const uint32_t *skbase, uint32_t nsets, int R,
                              uint32_t *row_lo, uint32_t *row_hi, uint32_t *cell_ds)
 {
-    uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    // 64-bit thread index: runs x window rows exceeds 2^32 on large grids
+    uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     uint32_t W = 2 * R + 1;
     uint32_t nrun = (ncell + kRowRun - 1) / kRowRun;
-    if (t >= nrun * W) return;
-    uint32_t run = t / W;
+    if (t >= (uint64_t)nrun * W) return;
+    uint32_t run = (uint32_t)(t / W);
     int dr = (int)(t % W) - R;
     uint32_t c0 = run * kRowRun;
     uint32_t c1 = c0 + kRowRun < ncell ? c0 + kRowRun : ncell;
@@ -502,9 +503,10 @@ __global__ void k_classify_rows(const double *box, uint32_t ncell, int W, double
                                 uint32_t *row_lo, uint32_t *row_hi, uint32_t *row_f0,
                                 uint32_t *row_f1)
 {
-    uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= ncell * (uint32_t)W) return;
-    uint32_t c = t / W;
+    // 64-bit thread index: cells x window rows exceeds 2^32 on large grids
+    uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= (uint64_t)ncell * (uint32_t)W) return;
+    uint32_t c = (uint32_t)(t / (uint32_t)W);
     const double *A = box + 4 * (size_t)c;
     uint32_t lo = row_lo[t], hi = row_hi[t];
     while (lo < hi && boxes_dist2_min(A, box + 4 * (size_t)lo) > T) lo++;

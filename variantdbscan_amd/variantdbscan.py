@@ -19,28 +19,31 @@ import numpy as np
 from . import _lib
 
 
-def VariantDBSCAN(eps_array, mp_array, x_array, y_array, mbbsize=70, verbose=False):
+def VariantDBSCAN(eps_array, mp_array, x_array, y_array, mbbsize=70, verbose=False,
+                  reuse=False):
+    """reuse=True (extension): a multi-variant call returns the labels of the
+    reference's greedy cluster reuse as it runs with one search thread (see
+    VDB_REUSE_REFERENCE in include/vdbscan.h); the default gives every
+    variant its one-variant labels."""
     libvdbscan = _lib.load()
-
-    def convert_type(in_array, new_dtype):          # src/variantdbscan.py:74-87
-        ret_array = in_array
-        if not isinstance(in_array, np.ndarray):
-            ret_array = np.array(in_array, dtype=new_dtype)
-        elif in_array.dtype != new_dtype:
-            ret_array = np.array(ret_array, dtype=new_dtype)
-        if not ret_array.flags["C_CONTIGUOUS"]:
-            ret_array = np.ascontiguousarray(ret_array)
-        return ret_array
-
-    c_eps_array = np.atleast_1d(convert_type(eps_array, np.float64))
-    c_mp_array = np.atleast_1d(convert_type(mp_array, np.uint32))
-    c_x_array = np.atleast_1d(convert_type(x_array, np.float64))
-    c_y_array = np.atleast_1d(convert_type(y_array, np.float64))
+    # float64 / uint32, C-contiguous (the conversions of src/variantdbscan.py:74-96)
+    c_eps_array = np.atleast_1d(np.ascontiguousarray(eps_array, dtype=np.float64))
+    c_mp_array = np.atleast_1d(np.ascontiguousarray(mp_array, dtype=np.uint32))
+    c_x_array = np.atleast_1d(np.ascontiguousarray(x_array, dtype=np.float64))
+    c_y_array = np.atleast_1d(np.ascontiguousarray(y_array, dtype=np.float64))
     if len(c_x_array) != len(c_y_array):
         raise ValueError("x_array and y_array differ in length")
     if len(c_eps_array) != len(c_mp_array):
         raise ValueError("eps_array and mp_array differ in length")
 
+    if reuse:
+        if int(mbbsize) <= 0:
+            raise _lib.VdbError("MBBsize must be positive")
+        results = _lib.cluster(c_x_array, c_y_array, c_eps_array, c_mp_array, reuse=True,
+                               verbose=verbose).reshape(-1)
+        if len(c_eps_array) > 1:
+            results = results.reshape([len(c_eps_array), len(c_x_array)]).T
+        return results
     results = np.zeros(len(c_x_array) * len(c_eps_array), dtype=np.uint32)  # :98
     res = libvdbscan.libVDBSCAN(
         ctypes.c_void_p(c_x_array.ctypes.data), ctypes.c_void_p(c_y_array.ctypes.data),
