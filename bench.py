@@ -62,6 +62,203 @@ def fill_bytes(n, pairs):
     return 24 * n + 4 * pairs
 
 
+# batched-engine kernels timed for the headline roofline
+BATCH_KERNELS = ("k_knn", "k_sweep", "k_test", "k_labels_all", "k_rows_all", "k_labels_j",
+                 "k_rows_from_pos", "k_roots_min", "k_pair_elig", "k_pairs", "k_pair_count")
+
+
+def batch_kernel_bytes(name, info, W):
+    """Algorithmic bytes per step of a batched-engine kernel (summed over its
+    launches; None for k_test and k_roots_min, whose traffic follows the
+    data). info: vdb_batch_info; W: fine-grid window width (2R+1).
+      k_sweep       per pair entering at its level: its order entry, cells
+                    and levels (14); per carried pair its record (12); per
+                    settled record the two cells' parents (8); per record
+                    written (carried on or a test slot) 12
+      k_knn         points read once (16 N), per listed point its position (4),
+                    its L sorted (s, id) entries (12 L) and count (4) written,
+                    per cell its start, key, dataset (12), box (32) and W
+                    window rows (8 W)
+      k_labels_all  per point: rank, cell, packed core levels, list index (16),
+                    per variant its cell label in and staged label out (8 V);
+                    the list walk of non-core points is not counted
+      k_rows_all    per point: its position (4), dataset bounds (8), its V
+                    staged labels in and V output labels out (8 V)
+      k_labels_j    per point: rank, cell, list index (12), its core level (1),
+                    the 8 slots' cell labels (32) and its 8 staged labels (32)
+      k_rows_from_pos per point: its position (4), dataset bounds (8), its
+                    8 staged labels (32) and 8 output labels (32)
+      k_pair_elig   per cell pair: cells and levels (10), both cells' core
+                    levels for each minpts value of the pass (2 g), key and
+                    value out (8 g), g = min(J, 8)
+      k_pairs       per cell: its window rows (8 W) and box (32); per stored
+                    pair 10 bytes
+      k_pair_count  per cell: window rows (8 W), box (32), count (4)"""
+    n, J, L = info["points"], info["minpts_values"], info["list_len"]
+    nl, nc, npairs = info["listed_points"], info["cells"], info["cell_pairs"]
+    g = min(J, 8)
+    V = info.get("snapshots") or info["levels"] * J  # one per requested (level, minpts)
+    fr, ca, te = info.get("fresh_pairs", 0), info.get("carried_pairs", 0), info.get("test_records", 0)
+    return {
+        "k_sweep": (14 * fr + 12 * ca + 8 * (fr + ca) + 12 * (ca + te)) if fr + ca else None,
+        "k_labels_all": (16 + 8 * V) * n,
+        "k_rows_all": (12 + 8 * V) * n,
+        "k_knn": 16 * n + nl * (8 + 12 * L) + nc * (44 + 8 * W),
+        "k_labels_j": 77 * n * -(-V // 8),
+        "k_rows_from_pos": 76 * n * -(-V // 8),
+        "k_pair_elig": npairs * (10 + 10 * g),
+        "k_pairs": nc * (8 * W + 32) + npairs * 10,
+        "k_pair_count": nc * (8 * W + 36),
+    }.get(name)
+
+
+def check_digests(labels, eps, mp):
+    """'64/64' when every variant's labels (device tensor, V x n) hash to the
+    committed digest of tests/golden/full_size_digests.json (aniso_2m:
+    the oracle's labels, matched to the compiled reference on the first
+    500,000 points), else 'k/64'; None without the file."""
+    import hashlib
+    path = os.path.join(ROOT, "tests", "golden", "full_size_digests.json")
+    if not os.path.exists(path):
+        return None
+    w = json.load(open(path))["workloads"]["aniso_2m"]["variants"]
+    host = labels.cpu().numpy().astype("<u4")
+    ok = 0
+    for v in range(len(eps)):
+        want = w["eps%.2f/mp%d" % (eps[v], mp[v])]["sha256"]
+        ok += hashlib.sha256(host[v].tobytes()).hexdigest() == want
+    return "%d/%d" % (ok, len(eps))
+
+
+def host_cpu():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return model, os.cpu_count(), usable
+
+
+def cpu_leg(x, y, eps, mp, dev, stream):
+    """The reference library (compiled from its sources into oracle/_ref) on
+    the host, per BASELINE's plan, on bounded samples of each workload (about
+    half a minute in all). Every sample's time is scaled to the full workload
+    by the ratio of exact pair counts (the reference's work is one candidate
+    test per pair per query), counted by the library on the GPU. A
+    single-variant reference call runs on one core (it parallelises across
+    variants only, src/main.cpp:215-216)."""
+    import torch
+    from concurrent.futures import ThreadPoolExecutor
+    import variantdbscan_amd as vd
+    from variantdbscan_amd import datasets
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = 16 if oracle.ref_available(16) else 8 if oracle.ref_available(8) else 1
+    if not oracle.ref_available(1) or not oracle.ref_available(threads):
+        return None
+    model, nproc, usable = host_cpu()
+
+    def pairs(px, py, e):
+        return int(vd.neighbor_counts(px, py, e).sum(dtype=np.uint64))
+
+    n = len(x)
+    ns = 200_000
+    xs, ys = x[:ns], y[:ns]
+    sx, sy = datasets.skew_8m()
+    s1x, s1y = sx[:1_000_000], sy[:1_000_000]
+    px, py = datasets.plain_10k(seed=1)
+    # pair counts of full workloads and samples (GPU)
+    E = sorted(set(eps.tolist()))
+    p_full = {e: pairs(x, y, e) for e in E}
+    p_samp = {e: pairs(xs, ys, e) for e in E}
+    p4_full, p4_samp = pairs(sx, sy, 0.3), pairs(s1x, s1y, 0.3)
+    sets = datasets.multi_256()
+    # the 16 datasets around the median size (the largest, up to 500k points,
+    # would hold tens of GB of reference expansion stacks across 16 threads)
+    order = np.argsort([len(s[0]) for s in sets], kind="stable")
+    pick = sorted(order[len(sets) // 2 - 8: len(sets) // 2 + 8].tolist())
+    E5, M5 = datasets.MULTI_EPS, datasets.MULTI_MINPTS
+    eps5 = np.array([e for e in E5 for m in M5])
+    mp5 = np.array([m for e in E5 for m in M5], dtype=np.uint32)
+    p5 = [sum(pairs(s[0], s[1], e) for e in E5) for s in sets]
+    del sx, sy
+    log("cpu: reference (%s, %d usable cores): configs 1, 2, 4 in parallel, single-threaded" %
+        (model, usable))
+    jobs = {
+        "config1": (px, py, [1.0], [4], 1),
+        "config2": (xs, ys, [0.5], [8], 1),
+        "config4": (s1x, s1y, [0.3], [10], 1),
+    }
+    with ThreadPoolExecutor(len(jobs)) as ex:
+        fut = {k: ex.submit(oracle.run_reference, *v[:4], threads=v[4], timeout=600)
+               for k, v in jobs.items()}
+        sec = {k: f.result()[1] for k, f in fut.items()}
+    log("cpu: reference config 3 sample, %d threads" % threads)
+    _, sec3 = oracle.run_reference(xs, ys, eps, mp, threads=threads, timeout=600)
+    log("cpu: reference config 5 sample (%d datasets), %d threads" % (len(pick), threads))
+    sec5 = sum(oracle.run_reference(sets[d][0], sets[d][1], eps5, mp5, threads=threads,
+                                    timeout=600)[1] for d in pick)
+    V = len(eps)
+    r3 = sum(p_full[e] for e in eps) / sum(p_samp[e] for e in eps)
+    r2 = p_full[0.5] / p_samp[0.5]
+    r4 = p4_full / p4_samp
+    r5 = sum(p5) / sum(p5[d] for d in pick)
+    # the GPU batched engine on the config-3 sample, for a like-for-like line
+    gx = torch.from_numpy(xs).to(dev)
+    gy = torch.from_numpy(ys).to(dev)
+    gl = torch.empty(V * ns, dtype=torch.int32, device=dev)
+    for _ in range(3):
+        vd.cluster_device(gx.data_ptr(), gy.data_ptr(), ns, eps, mp, gl.data_ptr(),
+                          path=vd.PATH_BATCH)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(3):
+        vd.cluster_device(gx.data_ptr(), gy.data_ptr(), ns, eps, mp, gl.data_ptr(),
+                          path=vd.PATH_BATCH)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    gms = e0.elapsed_time(e1) / 3
+    return {
+        "value": V / (sec3 * r3), "unit": "variant-clusterings/s", "cores": threads,
+        "kind": "reference",
+        "sample": "config 3 (2M points x 64 variants) timed on its first 200,000 points, all 64 "
+                  "variants in one libVDBSCAN call (NSEARCHTHREADS=%d, parallel across "
+                  "variants, greedy cluster reuse), %.2f s; scaled to 2M points by the pair-count "
+                  "ratio %.1f (sum over the 64 variants' eps)" % (threads, sec3, r3),
+        "cpu_model": model, "nproc": nproc, "usable_cores": usable,
+        "sample_variants_per_s": V / sec3, "gpu_same_sample_variants_per_s": V / (gms / 1e3),
+        "configs": {
+            "config1": {"workload": "10,000 points, eps 1.0, minpts 4, one call (full size)",
+                        "seconds": sec["config1"], "cores": 1},
+            "config2": {"workload": "2M points, eps 0.5, minpts 8, one single-variant call",
+                        "sample": "first 200,000 points", "sample_seconds": sec["config2"],
+                        "pair_ratio": r2, "seconds_full_estimate": sec["config2"] * r2,
+                        "variants_per_s": 1.0 / (sec["config2"] * r2), "cores": 1},
+            "config4": {"workload": "8M points (skewed), eps 0.3, minpts 10, one call",
+                        "sample": "first 1,000,000 points", "sample_seconds": sec["config4"],
+                        "pair_ratio": r4, "seconds_full_estimate": sec["config4"] * r4,
+                        "cores": 1},
+            "config5": {"workload": "256 datasets x 16 variants, one call per dataset",
+                        "sample": "the 16 datasets around the median size (%d to %d "
+                                  "points)" % (min(len(sets[d][0]) for d in pick),
+                                               max(len(sets[d][0]) for d in pick)),
+                        "sample_seconds": sec5, "pair_ratio": r5,
+                        "seconds_full_estimate": sec5 * r5,
+                        "datasets_x_variants_per_s": len(sets) * len(eps5) / (sec5 * r5),
+                        "cores": threads},
+        },
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,8 +273,11 @@ def main():
     ap.add_argument("--no-standard", action="store_true",
                     help="skip the standard-border-rule timing of the batch leg")
     ap.add_argument("--table-steps", type=int, default=2)
+    ap.add_argument("--kernel-steps", type=int, default=3,
+                    help="steps per kernel of the headline roofline's per-kernel timing")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="run a 1 GiB device copy in the table leg (HBM counter calibration)")
+                    help="run a 1 GiB device copy first in the batch and table legs (HBM "
+                         "counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -91,7 +291,10 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     L = vd._lib.load()
-    stream = torch.cuda.current_stream()
+    # a stream of its own (default priority): the library replays its launch
+    # sequences from HIP graphs, and the NULL stream cannot be captured
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
     L.vdb_set_stream(stream.cuda_stream)
 
     x, y = datasets.aniso_2m()
@@ -106,6 +309,14 @@ def main():
 
     # ------------------------------------------------------------ batch
     if args.only in (None, "batch"):
+        if args.calibrate_copy:
+            # plain copy of known size (1 GiB read + 1 GiB written): the scale
+            # of the HBM counters of the batched kernels' profile
+            ca = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+            cb = torch.empty_like(ca)
+            cb.copy_(ca)
+            torch.cuda.synchronize()
+            del ca, cb
         labels = torch.empty(V * n, dtype=torch.int32, device=dev)
         log("batch: warmup %d" % args.warmup)
         for _ in range(args.warmup):
@@ -133,6 +344,56 @@ def main():
                ("ms_total", "ms_prepare", "ms_count", "ms_uf", "ms_labels")}
         out.update(value=value, ms_per_step=ms_step, wall_s=wall, batch_phases_ms=avg)
         log("batch: %.3f ms/step -> %.1f variants/s" % (ms_step, value))
+
+        # Roofline of the headline step: every launch of each batched kernel
+        # timed with HIP events on the library's stream over further steps
+        # of the same call (graph replay is off while a kernel is timed);
+        # the dominant kernel's algorithmic bytes per launch over its average
+        # launch time.
+        info = vd._lib.batch_info()
+        d0 = min(E) / np.sqrt(2.0) * (1.0 - 2.0 ** -20)
+        W = 2 * int(np.ceil(max(E) * (1.0 + 2.0 ** -30) / d0)) + 1
+        per = {}
+        for kname in BATCH_KERNELS:
+            vd._lib.time_kernel(kname)
+            for _ in range(args.kernel_steps):
+                vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, eps, mp, labels.data_ptr(),
+                                  path=vd.PATH_BATCH)
+            ms, launches = vd._lib.kernel_time()
+            vd._lib.time_kernel(None)
+            if launches:
+                lps = launches / args.kernel_steps
+                bstep = batch_kernel_bytes(kname, info, W)
+                per[kname] = {"ms_per_step": ms / args.kernel_steps,
+                              "launches_per_step": lps,
+                              "ms_per_launch": ms / launches,
+                              # average over the step's launches
+                              "algorithmic_bytes": None if bstep is None else bstep / lps}
+        dom = max(per, key=lambda k: per[k]["ms_per_step"])
+        out["batch_kernels"] = per
+        out["batch_info"] = info
+        rec = per[dom]
+        if rec["algorithmic_bytes"] is not None:
+            ach = rec["algorithmic_bytes"] / (rec["ms_per_launch"] / 1e3) / 1e9
+            # HBM bytes per launch from the calibrated FETCH_SIZE/WRITE_SIZE
+            # passes over this workload, recorded under profiles/
+            traffic, tsrc = None, None
+            tf = os.path.join(ROOT, "profiles", "hbm_counters_%s_aniso2m_batch64.json" % dom)
+            if os.path.exists(tf):
+                traffic = json.load(open(tf)).get("traffic_bytes")
+                tsrc = os.path.relpath(tf, ROOT)
+            out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                               "traffic_source": tsrc,
+                               "kernel": dom, "algorithmic_bytes": rec["algorithmic_bytes"],
+                               "ms_per_launch": rec["ms_per_launch"],
+                               "ms_per_step": rec["ms_per_step"],
+                               "workload": "the headline step (2M points x 64 variants)"}
+        log("batch: dominant kernel %s, %.3f ms per step (%.1f launches)" %
+            (dom, rec["ms_per_step"], rec["launches_per_step"]))
+        labels_ok = check_digests(labels.view(V, n), eps, mp)
+        if labels_ok is not None:
+            out["labels_match_reference"] = labels_ok
 
         # spot check against the per-variant engine on two variants
         chk = [0, V - 1]
@@ -204,7 +465,7 @@ def main():
         tf = os.path.join(ROOT, "profiles", "hbm_counters_k_fill.json")
         if os.path.exists(tf):
             traffic = json.load(open(tf)).get("traffic_bytes")
-        out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        out["roofline_table_fill"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                            "traffic_source": "profiles/hbm_counters_k_fill.md",
                            "kernel": "k_fill<0, false, false> (CSR neighbour-table fill, FP64, "
@@ -335,40 +596,7 @@ def main():
 
     # ----------------------------------------------------------- CPU leg
     if args.only is None and not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle
-        threads = 16 if oracle.ref_available(16) else 8 if oracle.ref_available(8) else 1
-        if oracle.ref_available(threads):
-            rng = np.random.default_rng(0)
-            idx = np.sort(rng.choice(n, 100_000, replace=False))
-            xs, ys = x[idx], y[idx]
-            log("cpu: reference on %d points x %d variants, %d threads" % (len(xs), V, threads))
-            _, sec = oracle.run_reference(xs, ys, eps, mp, threads=threads, timeout=600)
-            # the GPU engine on the same sample, for a like-for-like line
-            gx = torch.from_numpy(xs).to(dev)
-            gy = torch.from_numpy(ys).to(dev)
-            gl = torch.empty(V * len(xs), dtype=torch.int32, device=dev)
-            vd.cluster_device(gx.data_ptr(), gy.data_ptr(), len(xs), eps, mp, gl.data_ptr(),
-                              path=vd.PATH_BATCH)
-            torch.cuda.synchronize()
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            for _ in range(3):
-                vd.cluster_device(gx.data_ptr(), gy.data_ptr(), len(xs), eps, mp, gl.data_ptr(),
-                                  path=vd.PATH_BATCH)
-            e1.record(stream)
-            torch.cuda.synchronize()
-            gms = e0.elapsed_time(e1) / 3
-            out["cpu_baseline"] = {
-                "value": V / sec, "unit": "variant-clusterings/s", "cores": threads,
-                "kind": "reference",
-                "sample": "100,000-point random subsample of the 2M points, all 64 variants "
-                          "in one libVDBSCAN call (reference OpenMP R-tree build, "
-                          "NSEARCHTHREADS=%d, parallel across variants)" % threads,
-                "seconds": sec, "gpu_same_sample_variants_per_s": V / (gms / 1e3)}
-        else:
-            out["cpu_baseline"] = None
+        out["cpu_baseline"] = cpu_leg(x, y, eps, mp, dev, stream)
 
     line = {
         "metric": METRIC,
@@ -390,7 +618,8 @@ def main():
         "roofline": out.get("roofline"),
         "cpu_baseline": out.get("cpu_baseline"),
     }
-    for k in ("ratio_vs_plain", "ratio_vs_plain_table", "plain_fp64", "table_build",
+    for k in ("labels_match_reference", "batch_kernels", "batch_info", "roofline_table_fill",
+              "ratio_vs_plain", "ratio_vs_plain_table", "plain_fp64", "table_build",
               "batch_standard_border", "end_to_end_host",
               "skew_8m", "multi_256", "batch_phases_ms",
               "labels_match_per_variant_engine", "wall_s"):

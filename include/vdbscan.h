@@ -121,6 +121,18 @@ int vdb_release(void);
 int vdb_time_kernel(const char *name);
 int vdb_kernel_time(double *ms, uint64_t *launches);
 
+/* Shape of the last batched run on this device, for byte models of its
+ * kernels: info[0] points, [1] eps levels, [2] minpts values, [3] list length
+ * (largest minpts), [4] points with a truncated list, [5] fine-grid cells,
+ * [6] cell pairs, [7] grid shape chosen by the autotuner, [8] requested
+ * (eps level, minpts) snapshots, and the union-find worklists summed over
+ * levels and minpts values: [9] pairs entering at their level, [10] pairs
+ * carried from an earlier level, [11] test records (with unused chunk
+ * slots); [9]-[11] are 0 when the minpts values ran in several passes.
+ * count entries are written (0 beyond these). No counterpart in the
+ * reference. */
+int vdb_batch_info(uint64_t *info, uint32_t count);
+
 /* Autotuner state as JSON: {"cache": "<file>", "tuned_in_process": N,
  * "entries": {"<device>:<workload key>": choice, ...}}, the choices read from
  * the tuning cache file and those made in this process (the per-variant

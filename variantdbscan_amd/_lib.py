@@ -24,7 +24,7 @@ EXPORTED = ("libVDBSCAN", "vdb_last_error", "vdb_version", "vdb_set_stream", "vd
             "vdb_cluster", "vdb_rank_order", "vdb_neighbor_counts", "vdb_neighbor_table",
             "vdb_neighbor_table_digest", "vdb_cluster_multi", "vdb_read_xy_csv",
             "vdb_read_lonlat_csv", "vdb_read_variants_csv", "vdb_write_labels_csv", "vdb_free",
-            "vdb_time_kernel", "vdb_kernel_time", "vdb_tuning")
+            "vdb_time_kernel", "vdb_kernel_time", "vdb_tuning", "vdb_batch_info")
 
 
 class VdbOptions(ctypes.Structure):
@@ -114,6 +114,8 @@ def load():
     L.vdb_kernel_time.restype = ctypes.c_int
     L.vdb_tuning.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
     L.vdb_tuning.restype = ctypes.c_int
+    L.vdb_batch_info.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32]
+    L.vdb_batch_info.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -201,6 +203,17 @@ def tuning():
     buf = ctypes.create_string_buffer(n.value + 1)
     _check(load().vdb_tuning(buf, n.value + 1, ctypes.byref(n)))
     return json.loads(buf.value.decode())
+
+
+def batch_info():
+    """Shape of the last batched run: dict of points, levels, minpts_values,
+    list_len, listed_points, cells, cell_pairs, shape, snapshots, and the
+    union-find worklist totals fresh_pairs, carried_pairs, test_records."""
+    keys = ("points", "levels", "minpts_values", "list_len", "listed_points", "cells",
+            "cell_pairs", "shape", "snapshots", "fresh_pairs", "carried_pairs", "test_records")
+    a = (ctypes.c_uint64 * len(keys))()
+    _check(load().vdb_batch_info(a, len(keys)))
+    return {k: int(a[i]) for i, k in enumerate(keys)}
 
 
 def rank_order(x, y):

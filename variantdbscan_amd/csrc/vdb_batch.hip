@@ -458,15 +458,19 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 
 // point's list holds exact binary64 s in ascending order, so count_k >= m
 // exactly when entry m-1 exists and has s <= T_k; an unlisted point has L
 // (>= every m_j) entries within eps_0.
+// With J <= 8 and K <= 15 every point's levels also go out packed, 4 bits
+// per minpts value (kc4[p] bits 4j..4j+3 = kc_j(p)), for the label pass that
+// takes all variants of a call at once.
 __global__ void k_core_levels(const uint32_t *lst_idx, const double *lst_s,
                               const uint32_t *lst_n, int L, const double *Tk, int K,
-                              const uint32_t *mj, int J, uint32_t n, uint8_t *kc)
+                              const uint32_t *mj, int J, uint32_t n, uint8_t *kc, uint32_t *kc4)
 {
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= n) return;
     uint32_t i = lst_idx[p];
     uint32_t m = i != kNone ? lst_n[i] : 0;
     uint32_t il = i != kNone ? i : 0;
+    uint32_t packed = 0;
     for (int j = 0; j < J; j++) {
         uint32_t need = mj[j];
         int lev = K;
@@ -482,7 +486,9 @@ __global__ void k_core_levels(const uint32_t *lst_idx, const double *lst_s,
             }
         }
         kc[(size_t)p * J + j] = (uint8_t)lev;
+        packed |= (uint32_t)(lev & 15) << (4 * (j & 7));
     }
+    if (kc4) kc4[p] = packed;
 }
 
 // per cell, per j, per k: smallest rank of a point with kc_j <= k (cells list
@@ -1493,6 +1499,192 @@ __global__ void k_rows_from_pos(LabelSlots S, const uint32_t *pos_of_caller, uin
     }
 }
 
+// Labels of every variant of a pass in one kernel, for passes of at most
+// kAllJ minpts values and kAllK levels. The minpts values are taken JG at a
+// time (a group); for each group a point's list is walked once, an entry
+// (neighbour q at s = d^2) updating, for every minpts value j of the group,
+// the border slots at levels max(level of s, kc_j(q)) .. kc_j(p)-1: the
+// levels at which q is a core point within eps and p is not core. The
+// running minima sit in registers, one per (j of the group, level < KB),
+// unrolled, so the snapshot loads of an entry are all in flight together.
+// Staging: per position, word j*KB + k holds the label of (j, k) (zero when
+// not requested), JG*KB words per group, written in 16-byte stores.
+constexpr int kAllJ = 8, kAllK = 15, kAllRows = 256, kAllStage = 128;
+struct AllSlots {
+    int J, K;
+    uint32_t stride;              // staged words per position
+    int8_t slot[kAllJ][16];       // (j, k) -> snapshot; -1: not requested
+    uint16_t jreq[kAllJ];         // bit k: (j, k) requested
+    double T[16];                 // level thresholds (s <= T_k: within eps_k)
+    const uint32_t *root, *lab, *bits, *pre; // snapshot t at + t*ncell (root, lab), + t*nw
+    uint32_t ncell, nw;
+    const uint32_t *seg;
+    uint32_t *tmp;
+};
+
+template <bool STD, int KB, int JG>
+__global__ void __launch_bounds__(kBlock)
+    k_labels_all(AllSlots S, const uint32_t *cell_of_pos, const uint32_t *cell_ds,
+                 const uint32_t *pos_rank, const uint32_t *kc4, const uint32_t *lst_idx,
+                 const double *lst_s, const uint32_t *lst_id, const uint32_t *lst_n, int L,
+                 uint32_t n)
+{
+    static_assert((JG * KB) % 4 == 0 && KB <= 16 && JG <= kAllJ, "whole 16-byte groups");
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t r = pos_rank[p];
+    const uint32_t C = cell_of_pos[p];
+    const uint32_t kp4 = kc4[p];
+    const uint32_t li = lst_idx[p];
+    const uint32_t s0 = S.seg[cell_ds[C]];
+    const int K = S.K;
+    for (int j0 = 0; j0 < S.J; j0 += JG) {
+        // per minpts value of the group: p's core level, and the walk bound
+        // (largest threshold of a requested level at which p is not core)
+        int kp[JG];
+        double Tmax = -1.0;
+#pragma unroll
+        for (int jj = 0; jj < JG; jj++) {
+            int j = j0 + jj;
+            kp[jj] = j < S.J ? (int)((kp4 >> (4 * j)) & 15u) : 0;
+            uint32_t below = S.jreq[j < S.J ? j : 0] & ((1u << kp[jj]) - 1u);
+            if (j < S.J && below) {
+                double t = S.T[31 - __builtin_clz(below)];
+                Tmax = t > Tmax ? t : Tmax;
+            }
+        }
+        uint32_t best[JG][KB];
+#pragma unroll
+        for (int jj = 0; jj < JG; jj++)
+#pragma unroll
+            for (int k = 0; k < KB; k++) best[jj][k] = kNone;
+        if (Tmax >= 0.0 && li != kNone) {
+            // non-core at level k: count_k(p) < m_j <= L, so its list holds
+            // every eps_k neighbour as the prefix with s <= T_k
+            uint32_t m = lst_n[li];
+            const double *sv = lst_s + lst_at(li, 0, L); // entry e at sv[64 e]
+            const uint32_t *iv = lst_id + lst_at(li, 0, L);
+            int ke = 0; // level of the current entry (entries ascend in s)
+            constexpr int kU = 2;
+            for (uint32_t e0 = 0; e0 < m; e0 += kU) {
+                double se[kU];
+                uint32_t q[kU];
+#pragma unroll
+                for (int u = 0; u < kU; u++) {
+                    bool ok = e0 + u < m;
+                    se[u] = ok ? sv[(size_t)(e0 + u) * 64u] : INFINITY;
+                    q[u] = ok ? iv[(size_t)(e0 + u) * 64u] : 0u;
+                }
+                uint32_t kq[kU], cq[kU];
+#pragma unroll
+                for (int u = 0; u < kU; u++) {
+                    bool ok = se[u] <= Tmax;
+                    kq[u] = ok ? kc4[q[u]] : 0xffffffffu;
+                    cq[u] = ok ? cell_of_pos[q[u]] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < kU; u++) {
+                    if (!(se[u] <= Tmax)) break;
+                    while (ke < K && se[u] > S.T[ke]) ke++;
+                    uint32_t v[JG][KB];
+#pragma unroll
+                    for (int jj = 0; jj < JG; jj++) {
+                        int j = j0 + jj;
+                        int kqj = (int)((kq[u] >> (4 * (j & 7))) & 15u);
+                        int lo = kqj > ke ? kqj : ke;
+                        int hi = kp[jj] < K ? kp[jj] : K;
+#pragma unroll
+                        for (int k = 0; k < KB; k++) {
+                            v[jj][k] = kNone;
+                            int t = S.slot[j & 7][k];
+                            if (j < S.J && t >= 0 && k >= lo && k < hi)
+                                v[jj][k] = S.root[(size_t)t * S.ncell + cq[u]];
+                        }
+                    }
+#pragma unroll
+                    for (int jj = 0; jj < JG; jj++)
+#pragma unroll
+                        for (int k = 0; k < KB; k++)
+                            best[jj][k] = v[jj][k] < best[jj][k] ? v[jj][k] : best[jj][k];
+                }
+                if (!(se[kU - 1] <= Tmax)) break;
+            }
+        }
+        uint32_t lbl[JG * KB];
+#pragma unroll
+        for (int jj = 0; jj < JG; jj++) {
+            int j = j0 + jj;
+#pragma unroll
+            for (int k = 0; k < KB; k++) {
+                uint32_t x = 0u;
+                int t = S.slot[j & 7][k];
+                if (j < S.J && t >= 0) {
+                    if (kp[jj] <= k) {
+                        x = S.lab[(size_t)t * S.ncell + C];
+                    } else {
+                        const uint32_t *bits = S.bits + (size_t)t * S.nw;
+                        const uint32_t *pre = S.pre + (size_t)t * S.nw;
+                        uint32_t b = best[jj][k];
+                        if (STD)
+                            x = b != kNone ? roots_below(bits, pre, b) - roots_below(bits, pre, s0)
+                                           : kNone;
+                        else
+                            x = (b != kNone && b < r)
+                                    ? roots_below(bits, pre, b) - roots_below(bits, pre, s0) + 1
+                                    : 0u;
+                    }
+                }
+                lbl[jj * KB + k] = x;
+            }
+        }
+        uint4 *dst = (uint4 *)(S.tmp + (size_t)p * S.stride + (size_t)j0 * KB);
+#pragma unroll
+        for (int w = 0; w < JG * KB; w += 4)
+            dst[w / 4] = make_uint4(lbl[w], lbl[w + 1], lbl[w + 2], lbl[w + 3]);
+    }
+}
+
+// Rows from the staged labels, thread per caller index (every row is written
+// contiguously). A point's WMAX staged words are loaded at once (read while
+// their cache lines are in cache), then up to kAllRows rows are written, the
+// rows first[w] .. first[w+1]-1 taking staged word w.
+struct AllRows {
+    uint32_t stride;
+    uint16_t first[kAllStage + 1];
+    uint32_t row[kAllRows];
+    uint32_t *out;
+    uint32_t nrows;
+    const uint32_t *seg;
+    const uint32_t *tmp;
+};
+
+template <int WMAX>
+__global__ void __launch_bounds__(kBlock)
+    k_rows_all(AllRows R, const uint32_t *pos_of_caller, uint32_t nsets, uint32_t n)
+{
+    static_assert(WMAX % 4 == 0 && WMAX <= kAllStage, "whole 16-byte groups");
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t p = pos_of_caller[i];
+    uint32_t ds = seg_of(R.seg, nsets, i);
+    uint32_t s0 = R.seg[ds], nd = R.seg[ds + 1] - s0;
+    uint32_t *base = R.out + (size_t)s0 * R.nrows + (i - s0);
+    const uint4 *src = (const uint4 *)(R.tmp + (size_t)p * R.stride);
+    uint32_t v[WMAX];
+#pragma unroll
+    for (int g = 0; g < WMAX / 4; g++) {
+        uint4 x = make_uint4(0u, 0u, 0u, 0u);
+        if (4u * g < R.stride) x = src[g];
+        v[4 * g] = x.x;
+        v[4 * g + 1] = x.y;
+        v[4 * g + 2] = x.z;
+        v[4 * g + 3] = x.w;
+    }
+#pragma unroll
+    for (int w = 0; w < WMAX; w++)
+        for (int e = R.first[w]; e < R.first[w + 1]; e++) base[(size_t)R.row[e] * nd] = v[w];
+}
+
 __global__ void k_pos_of_caller(const uint32_t *caller_of_pos, uint32_t n, uint32_t *pos_of_caller)
 {
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
@@ -1700,8 +1892,11 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     uint8_t *kc, *cellkc;
     uint32_t *cmr;
     VDB_TRY(c.get("b.kc", (size_t)n * J, &kc));
+    // packed levels for the one-sweep label pass (J <= kAllJ, K <= kAllK)
+    uint32_t *kc4 = nullptr;
+    if (J <= kAllJ && K <= kAllK) VDB_TRY(c.get("b.kc4", n, &kc4));
     k_core_levels<<<gn, kBlock, 0, c.stream()>>>(lst_idx, lst_s, lst_n, (int)L, dTk, K, dmj, J,
-                                                 n, kc);
+                                                 n, kc, kc4);
     VDB_HIP(hipGetLastError());
     VDB_TRY(c.get("b.cmr", (size_t)G.ncell * J * K, &cmr));
     VDB_TRY(c.get("b.cellkc", (size_t)G.ncell * J, &cellkc));
@@ -1773,9 +1968,8 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         unsigned long long v = strtoull(ev, nullptr, 10);
         if (v > 0 && v < tcap) tcap = (uint32_t)v;
     }
-    uint32_t *pos_of_caller, *labtmp;
+    uint32_t *pos_of_caller;
     VDB_TRY(c.get("b.poc", n, &pos_of_caller));
-    VDB_TRY(c.get("b.labtmp", (size_t)kLabelSlots * n, &labtmp));
     if (st) st->ms_prepare += tpairs.ms();
 
     // ---- variants of each (k, j)
@@ -1827,6 +2021,54 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_TRY(c.get("b.snap_lab", max_slots * G.ncell, &snap_lab));
     VDB_TRY(c.get("b.snap_bits", max_slots * nw, &snap_bits));
     VDB_TRY(c.get("b.snap_pre", max_slots * nw, &snap_pre));
+    // one label kernel for the whole call when every minpts value runs in
+    // one pass and the core levels are packed; else a kernel per minpts
+    // value, kLabelSlots snapshots at a time
+    const bool one_sweep = kc4 && jc == J;
+    AllSlots A;
+    std::vector<AllRows> all_rows;
+    int all_kb = 0, all_jg = 0, all_wmax = 0;
+    if (one_sweep) {
+        memset(&A, 0, sizeof A);
+        memset(A.slot, -1, sizeof A.slot);
+        A.J = J;
+        A.K = K;
+        for (int k = 0; k < K; k++) A.T[k] = Th[k];
+        // snapshot of (k, j): level-major over the requested pairs, as the
+        // level passes number them
+        int nsnap = 0;
+        for (int k = 0; k < K; k++)
+            for (int j = 0; j < J; j++)
+                if (want.count({k, j})) {
+                    A.slot[j][k] = (int8_t)nsnap++;
+                    A.jreq[j] |= (uint16_t)(1u << k);
+                }
+        // register tiles of 32 minima: 4 minpts values x 8 levels, or 2 x 16
+        all_kb = K <= 8 ? 8 : 16;
+        all_jg = all_kb == 8 ? 4 : 2;
+        A.stride = (uint32_t)(((J + all_jg - 1) / all_jg) * all_jg * all_kb);
+        all_wmax = A.stride <= 16 ? 16 : A.stride <= 32 ? 32 : A.stride <= 64 ? 64 : 128;
+        std::vector<std::pair<uint32_t, uint32_t>> word_row; // (staged word, output row)
+        for (auto &w : want)
+            for (uint32_t v : w.second)
+                word_row.push_back({(uint32_t)(w.first.second * all_kb + w.first.first), rows[v]});
+        std::sort(word_row.begin(), word_row.end());
+        for (size_t r0 = 0; r0 < word_row.size(); r0 += kAllRows) {
+            AllRows R;
+            memset(&R, 0, sizeof R);
+            size_t r1 = std::min(word_row.size(), r0 + kAllRows);
+            R.stride = A.stride;
+            size_t e = r0;
+            for (int t = 0; t <= kAllStage; t++) {
+                while (e < r1 && word_row[e].first < (uint32_t)t) e++;
+                R.first[t] = (uint16_t)(e - r0);
+            }
+            for (size_t q = r0; q < r1; q++) R.row[q - r0] = word_row[q].second;
+            all_rows.push_back(R);
+        }
+    }
+    uint32_t *labtmp;
+    VDB_TRY(c.get("b.labtmp", (size_t)(one_sweep ? A.stride : kLabelSlots) * n, &labtmp));
     // label-pass events, read once the batch is done (reading them inside
     // the loop would make the host wait once per pass)
     struct EventList {
@@ -1884,7 +2126,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
                     VDB_HIP(hipGetLastError());
                 }
                 uint32_t nkeys = (uint32_t)nj * npairs;
-                if (nb <= 16) {
+                if (nb <= 128) {
                     // one counting pass, which also gives the bucket starts
                     VDB_TRY(bucket_sort_values(c, ekey, eval, stv, nkeys, (int)nb, bstart));
                     order = stv;
@@ -1956,8 +2198,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
                                                                        std_rule ? 0u : 1u);
                 VDB_HIP(hipGetLastError());
             }
-            // labels of every requested variant of the pass, kLabelSlots rows
-            // (one minpts value) per launch
+            // labels of every requested variant of the pass
             if (events) {
                 hipEvent_t le0, le1;
                 VDB_HIP(hipEventCreate(&le0));
@@ -1966,7 +2207,49 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
                 lab_ev.push_back(le1);
                 VDB_HIP(hipEventRecord(le0, c.stream()));
             }
-            for (int jr = 0; jr < nj; jr++) {
+            if (one_sweep) {
+                A.root = snap_root;
+                A.lab = snap_lab;
+                A.bits = snap_bits;
+                A.pre = snap_pre;
+                A.ncell = G.ncell;
+                A.nw = nw;
+                A.seg = P.seg;
+                A.tmp = labtmp;
+                VDB_TRY(c.kt_begin("k_labels_all"));
+#define VDB_LABELS_ALL(STD, KB, JG)                                                              \
+    k_labels_all<STD, KB, JG><<<gn, kBlock, 0, c.stream()>>>(A, G.cell_of_pos, G.cell_ds,        \
+                                                             G.pos_rank, kc4, lst_idx, lst_s,     \
+                                                             lst_id, lst_n, (int)L, n)
+                if (all_kb == 8) {
+                    if (std_rule) VDB_LABELS_ALL(true, 8, 4);
+                    else VDB_LABELS_ALL(false, 8, 4);
+                } else {
+                    if (std_rule) VDB_LABELS_ALL(true, 16, 2);
+                    else VDB_LABELS_ALL(false, 16, 2);
+                }
+#undef VDB_LABELS_ALL
+                VDB_TRY(c.kt_end("k_labels_all"));
+                VDB_HIP(hipGetLastError());
+                for (AllRows R : all_rows) {
+                    R.out = out;
+                    R.nrows = nrows;
+                    R.seg = P.seg;
+                    R.tmp = labtmp;
+                    VDB_TRY(c.kt_begin("k_rows_all"));
+                    if (all_wmax == 16)
+                        k_rows_all<16><<<gn, kBlock, 0, c.stream()>>>(R, pos_of_caller, P.nsets, n);
+                    else if (all_wmax == 32)
+                        k_rows_all<32><<<gn, kBlock, 0, c.stream()>>>(R, pos_of_caller, P.nsets, n);
+                    else if (all_wmax == 64)
+                        k_rows_all<64><<<gn, kBlock, 0, c.stream()>>>(R, pos_of_caller, P.nsets, n);
+                    else
+                        k_rows_all<128><<<gn, kBlock, 0, c.stream()>>>(R, pos_of_caller, P.nsets, n);
+                    VDB_TRY(c.kt_end("k_rows_all"));
+                    VDB_HIP(hipGetLastError());
+                }
+            }
+            for (int jr = 0; jr < nj && !one_sweep; jr++) {
                 int j = j0 + jr;
                 std::vector<std::pair<int, uint32_t>> rows_j; // (level, variant)
                 for (int k = 0; k < K; k++) {
@@ -2089,6 +2372,32 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         st->ms_uf += ms_uf_all - ms_labels;
         st->ms_labels += ms_labels;
     }
+    // worklist totals of the call (when all minpts values ran in one pass,
+    // whose counters are still in place): the byte model of the level kernels
+    uint64_t nsnap = want.size(), nfresh_all = 0, ncarried_all = 0, ntests_all = 0;
+    if (jc == J && npairs) {
+        std::vector<uint32_t> hc((size_t)J * ncnt), hb((size_t)J * (K + 1) + 1);
+        VDB_HIP(hipMemcpyAsync(hc.data(), cnts, sizeof(uint32_t) * hc.size(),
+                               hipMemcpyDeviceToHost, c.stream()));
+        VDB_HIP(hipMemcpyAsync(hb.data(), bstart, sizeof(uint32_t) * hb.size(),
+                               hipMemcpyDeviceToHost, c.stream()));
+        VDB_HIP(hipStreamSynchronize(c.stream()));
+        int kmax = -1;
+        for (auto &w : want) kmax = std::max(kmax, w.first.first);
+        for (int j = 0; j < J; j++) {
+            const uint32_t *h = hc.data() + (size_t)j * ncnt;
+            for (int k = 0; k <= kmax; k++) {
+                size_t b0 = (size_t)j * (K + 1) + k;
+                nfresh_all += hb[b0 + 1] - hb[b0];
+                ncarried_all += h[k];
+                ntests_all += (uint64_t)h[K + 1 + k] + h[2 * (K + 1) + k];
+            }
+        }
+    }
+    uint64_t info[kBatchInfo] = {n,       (uint64_t)K, (uint64_t)J, L,           nlist,
+                                 G.ncell, npairs,      (uint64_t)shape_id,        nsnap,
+                                 nfresh_all, ncarried_all, ntests_all};
+    memcpy(c.last_batch, info, sizeof info);
     if (res) {
         res->levels = (uint32_t)K;
         res->minpts_values = (uint32_t)J;

@@ -341,6 +341,19 @@ int vdb_kernel_time(double *ms, uint64_t *launches)
     return c->kt_read(ms, launches);
 }
 
+int vdb_batch_info(uint64_t *info, uint32_t count)
+{
+    if (!info && count) {
+        set_error("NULL output argument");
+        return kErrInvalid;
+    }
+    Context *c = nullptr;
+    VDB_TRY(get_context(&c));
+    std::lock_guard<std::mutex> lock(c->mu);
+    for (uint32_t i = 0; i < count; i++) info[i] = i < (uint32_t)kBatchInfo ? c->last_batch[i] : 0;
+    return kOk;
+}
+
 int vdb_tuning(char *buf, uint64_t cap, uint64_t *len)
 {
     Context *c = nullptr;

@@ -13,6 +13,9 @@
 
 namespace vdb {
 
+// entries of vdb_batch_info
+constexpr int kBatchInfo = 12;
+
 struct Buffer {
     void *ptr = nullptr;
     size_t bytes = 0;
@@ -61,6 +64,7 @@ public:
     void tuned_put(const std::string &key, int value);
     std::string tuned_json();
     uint32_t tuned_in_process = 0; // tunings run by this process
+    uint64_t last_batch[kBatchInfo] = {0}; // vdb_batch_info of the last batched run
     std::string tune_cache_path() const;
 
     // Per-kernel timing: while a kernel name is selected, every launch of

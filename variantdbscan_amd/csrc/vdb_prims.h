@@ -22,7 +22,7 @@ int radix_sort_pairs(Context &c, uint32_t *keys, uint32_t *vals, uint32_t *tmp_k
                      uint32_t *tmp_vals, size_t n, int key_bits, uint32_t **keys_out,
                      uint32_t **vals_out);
 
-// Stable sort of n >= 1 values by keys in [0, nbuckets), nbuckets <= 16, in
+// Stable sort of n >= 1 values by keys in [0, nbuckets), nbuckets <= 128, in
 // one counting pass: out_vals gets the values in key order (the keys are not
 // written), starts[d] (device, nbuckets + 1 entries) the first position of
 // key d and starts[nbuckets] = n.

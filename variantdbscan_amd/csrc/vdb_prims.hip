@@ -197,20 +197,19 @@ __global__ void __launch_bounds__(kSortBlock)
     }
 }
 
-// ------------------------------------------------- bucket sort (<= 16 keys)
-// One stable counting pass for keys below kBuckets: a tile histogram made
-// with ballots (lanes sharing a key are counted by one lane, no LDS atomics),
-// a digit-major scan whose entries at tile 0 are the bucket starts, and a
-// scatter of the values only.
-constexpr int kBucketBits = 4;
-constexpr int kBuckets = 1 << kBucketBits;
+// ---------------------------------------- bucket sort (<= 16 or 128 keys)
+// One stable counting pass for keys below kBuckets = 2^BITS: a tile histogram
+// made with ballots (lanes sharing a key are counted by one lane, no LDS
+// atomics), a digit-major scan whose entries at tile 0 are the bucket starts,
+// and a scatter of the values only.
+constexpr int kMaxBucketBits = 7;
 
-// lanes of the wave holding the same key d (d == kBuckets: no key)
-__device__ __forceinline__ uint64_t bucket_peers(uint32_t d)
+// lanes of the wave holding the same key d (d == 2^BITS: no key)
+template <int BITS> __device__ __forceinline__ uint64_t bucket_peers(uint32_t d)
 {
     uint64_t peers = ~0ull;
 #pragma unroll
-    for (int b = 0; b <= kBucketBits; b++) {
+    for (int b = 0; b <= BITS; b++) {
         bool bit = (d >> b) & 1u;
         uint64_t bal = ballot(bit);
         peers &= bit ? bal : ~bal;
@@ -218,9 +217,11 @@ __device__ __forceinline__ uint64_t bucket_peers(uint32_t d)
     return peers;
 }
 
+template <int BITS>
 __global__ void __launch_bounds__(kSortBlock)
     k_bucket_hist(const uint32_t *keys, size_t n, uint32_t *hist, unsigned tiles)
 {
+    constexpr int kBuckets = 1 << BITS;
     __shared__ uint32_t h[kSortWaves][kBuckets];
     for (int i = threadIdx.x; i < kSortWaves * kBuckets; i += kSortBlock) (&h[0][0])[i] = 0;
     __syncthreads();
@@ -229,7 +230,7 @@ __global__ void __launch_bounds__(kSortBlock)
     for (int i = 0; i < kSortItems; i++) {
         size_t idx = base + (size_t)i * kSortBlock + threadIdx.x;
         uint32_t d = idx < n ? keys[idx] : (uint32_t)kBuckets;
-        uint64_t peers = bucket_peers(d);
+        uint64_t peers = bucket_peers<BITS>(d);
         // one lane per key: the wave's own row, no other writer
         if (d < (uint32_t)kBuckets && mask_rank(peers) == 0) h[w][d] += (uint32_t)popc64(peers);
     }
@@ -241,10 +242,12 @@ __global__ void __launch_bounds__(kSortBlock)
     }
 }
 
+template <int BITS>
 __global__ void __launch_bounds__(kSortBlock)
     k_bucket_scatter(const uint32_t *keys, const uint32_t *vals, uint32_t *ovals, size_t n,
                      const uint32_t *offs, unsigned tiles)
 {
+    constexpr int kBuckets = 1 << BITS;
     __shared__ uint32_t base_d[kBuckets];
     __shared__ uint32_t wcnt[kSortWaves][kBuckets];
     for (int d = threadIdx.x; d < kBuckets; d += kSortBlock)
@@ -258,7 +261,7 @@ __global__ void __launch_bounds__(kSortBlock)
         bool valid = idx < n;
         uint32_t d = valid ? keys[idx] : (uint32_t)kBuckets;
         uint32_t v = valid ? vals[idx] : 0u;
-        uint64_t peers = bucket_peers(d);
+        uint64_t peers = bucket_peers<BITS>(d);
         int rank_w = mask_rank(peers);
         if (valid && rank_w == 0) wcnt[w][d] = (uint32_t)popc64(peers);
         __syncthreads();
@@ -410,23 +413,38 @@ int radix_sort_pairs(Context &c, uint32_t *keys, uint32_t *vals, uint32_t *tmp_k
     return kOk;
 }
 
-int bucket_sort_values(Context &c, const uint32_t *keys, const uint32_t *vals,
-                       uint32_t *out_vals, uint32_t n, int nbuckets, uint32_t *starts)
+template <int BITS>
+int bucket_sort_bits(Context &c, const uint32_t *keys, const uint32_t *vals, uint32_t *out_vals,
+                     uint32_t n, uint32_t *offs_out[1])
 {
-    if (nbuckets < 1 || nbuckets > kBuckets || n == 0) {
-        set_error("bucket sort: 1..16 buckets and at least one value");
-        return kErrInvalid;
-    }
+    constexpr int kBuckets = 1 << BITS;
     unsigned tiles = ceil_div(n, kSortTile);
     uint32_t *hist = nullptr, *offs = nullptr;
     VDB_TRY(c.get("bucket.hist", (size_t)tiles * kBuckets, &hist));
     VDB_TRY(c.get("bucket.offs", (size_t)tiles * kBuckets, &offs));
-    k_bucket_hist<<<tiles, kSortBlock, 0, c.stream()>>>(keys, n, hist, tiles);
+    k_bucket_hist<BITS><<<tiles, kSortBlock, 0, c.stream()>>>(keys, n, hist, tiles);
     VDB_HIP(hipGetLastError());
     VDB_TRY(scan_exclusive_u32(c, hist, offs, (size_t)tiles * kBuckets, nullptr));
-    k_bucket_scatter<<<tiles, kSortBlock, 0, c.stream()>>>(keys, vals, out_vals, n, offs, tiles);
+    k_bucket_scatter<BITS><<<tiles, kSortBlock, 0, c.stream()>>>(keys, vals, out_vals, n, offs,
+                                                                   tiles);
     VDB_HIP(hipGetLastError());
-    k_bucket_bounds<<<1, kWave, 0, c.stream()>>>(offs, tiles, nbuckets, n, starts);
+    *offs_out = offs;
+    return kOk;
+}
+
+int bucket_sort_values(Context &c, const uint32_t *keys, const uint32_t *vals,
+                       uint32_t *out_vals, uint32_t n, int nbuckets, uint32_t *starts)
+{
+    if (nbuckets < 1 || nbuckets > (1 << kMaxBucketBits) || n == 0) {
+        set_error("bucket sort: 1..128 buckets and at least one value");
+        return kErrInvalid;
+    }
+    unsigned tiles = ceil_div(n, kSortTile);
+    uint32_t *offs = nullptr;
+    if (nbuckets <= 16) VDB_TRY(bucket_sort_bits<4>(c, keys, vals, out_vals, n, &offs));
+    else VDB_TRY(bucket_sort_bits<kMaxBucketBits>(c, keys, vals, out_vals, n, &offs));
+    // one thread per bucket start (up to 129)
+    k_bucket_bounds<<<1, kSortBlock, 0, c.stream()>>>(offs, tiles, nbuckets, n, starts);
     VDB_HIP(hipGetLastError());
     return kOk;
 }
