@@ -809,52 +809,7 @@ __global__ void __launch_bounds__(kBlock)
     });
 }
 
-// --------------------------------------------- union-find over a CSR table
-// One wave per row; rows [p0, p1) of position space, ids are positions.
-// Each unordered core pair is united from its larger end only. A row lists
-// its window rows in ascending cell key and each run in position order, so
-// its ids ascend: the scan stops at the first block that reaches p.
-__global__ void __launch_bounds__(kBlock)
-    k_uf_table(const uint64_t *__restrict__ offsets, uint64_t off_base,
-               const uint32_t *__restrict__ ids, uint32_t p0, uint32_t p1,
-               const uint8_t *__restrict__ core, const uint32_t *__restrict__ rank,
-               uint32_t *parent)
-{
-    uint32_t p = p0 + blockIdx.x * kWavesPerBlock + wave_id_in_block();
-    if (p >= p1 || !core[p]) return;
-    uint64_t a = offsets[p] - off_base, e = offsets[p + 1] - off_base;
-    uint32_t rp = uf_find(parent, p);
-    // eight blocks per step: their id, core and parent loads overlap
-    constexpr int U = 8;
-    for (uint64_t j0 = a; j0 < e; j0 += U * kWave) {
-        uint32_t q[U], pq[U];
-        bool qc[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            uint64_t j = j0 + u * kWave + lane_id();
-            q[u] = j < e ? ids[j] : kNone;
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) qc[u] = q[u] < p && core[q[u]];
-        // A cached parent may be stale: a stale "same root" is still right
-        // (a node once under rp stays in rp's component), a stale "different"
-        // only costs a union attempt, which rereads at agent scope.
-#pragma unroll
-        for (int u = 0; u < U; u++) pq[u] = qc[u] ? parent[q[u]] : 0u;
-        bool past = false;
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            bool need = qc[u] && pq[u] != rp;
-            if (ballot(need)) {
-                if (need) uf_unite(parent, rank, q[u], p);
-                rp = uf_find(parent, p);
-            }
-            past |= q[u] != kNone && q[u] >= p;
-        }
-        if (ballot(past)) break;
-    }
-}
-
+// --------------------------------------------- border pass over a CSR table
 __global__ void __launch_bounds__(kBlock)
     k_border_table(const uint64_t *__restrict__ offsets, uint64_t off_base,
                    const uint32_t *__restrict__ ids, uint32_t p0, uint32_t p1,
@@ -1259,7 +1214,25 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
         VDB_TRY(c.get("cv.ids", want ? want : 1, &ids));
         // tiles never exceed the buffer actually held
         cap = want;
-        // tiles are runs of items (items are ordered by cell, origins ascending)
+        // Core components: the fused half-window union-find (distances
+        // recomputed, each unordered pair once) rather than a pass over the
+        // table, which would read every row's ids again (95 GB at 2M points,
+        // eps 0.5) at gather speed.
+        if (pt) tm.start();
+        if (I.n) {
+            VDB_TRY(c.kt_begin("k_uf_fused"));
+            k_uf_fused<<<blocks, kBlock, 0, c.stream()>>>(view(G, G.npts), I.cell, I.o0, I.n, T,
+                                                          core, G.pos_rank, parent);
+            VDB_TRY(c.kt_end("k_uf_fused"));
+        }
+        VDB_HIP(hipGetLastError());
+        k_compress<<<g, kBlock, 0, c.stream()>>>(parent, core, n);
+        VDB_HIP(hipGetLastError());
+        VDB_TRY(root_keys());
+        if (pt) pt->uf += tm.stop();
+        // The table, tile by tile (tiles are runs of items: items are ordered
+        // by cell, origins ascending), each tile filled once and its border
+        // rows (fewer than minpts entries each) read back for the border pass.
         uint32_t ntiles = 0;
         uint32_t i0 = 0;
         while (i0 < I.n) {
@@ -1271,6 +1244,7 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
                 i1++;
             }
             uint32_t pend = i1 < I.n ? hio0[i1] : n;
+            uint32_t p0 = hio0[i0];
             if (hoff[pend] - base > cap) {
                 set_error("neighbour table tile does not fit the table budget");
                 return kErrNoMemory;
@@ -1278,53 +1252,19 @@ int cluster_variant(Context &c, const Points &P, const Grid &G, const Items &I, 
             if (pt) tm.start();
             VDB_TRY(c.kt_begin("k_fill"));
             k_fill<0, false><<<ceil_div(i1 - i0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
-                view(G, G.npts), I.cell, I.o0, i0, i1, T, Band{}, offsets, base, nullptr, nullptr, ids,
-                nullptr, cap, ovf);
+                view(G, G.npts), I.cell, I.o0, i0, i1, T, Band{}, offsets, base, nullptr, nullptr,
+                ids, nullptr, cap, ovf);
             VDB_TRY(c.kt_end("k_fill"));
             VDB_HIP(hipGetLastError());
             if (pt) pt->fill += tm.stop(), tm.start();
-            uint32_t p0 = hio0[i0];
-            VDB_TRY(c.kt_begin("k_uf_table"));
-            k_uf_table<<<ceil_div(pend - p0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
-                offsets, base, ids, p0, pend, core, G.pos_rank, parent);
-            VDB_TRY(c.kt_end("k_uf_table"));
-            VDB_HIP(hipGetLastError());
-            if (pt) pt->uf += tm.stop();
-            ntiles++;
-            i0 = i1;
-        }
-        if (pt) tm.start();
-        k_compress<<<g, kBlock, 0, c.stream()>>>(parent, core, n);
-        VDB_HIP(hipGetLastError());
-        VDB_TRY(root_keys());
-        if (pt) pt->uf += tm.stop(), tm.start();
-        // border rows are short (fewer than minpts entries): re-read them tile by tile
-        i0 = 0;
-        while (i0 < I.n) {
-            uint32_t i1 = i0;
-            uint64_t base = hoff[hio0[i0]];
-            while (i1 < I.n) {
-                uint32_t oend = (i1 + 1 < I.n) ? hio0[i1 + 1] : n;
-                if (hoff[oend] - base > cap && i1 > i0) break;
-                i1++;
-            }
-            uint32_t pend = i1 < I.n ? hio0[i1] : n;
-            uint32_t p0 = hio0[i0];
-            if (ntiles > 1) {
-                // the table no longer holds this tile: rebuild it
-                VDB_TRY(c.kt_begin("k_fill"));
-                k_fill<0, false><<<ceil_div(i1 - i0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
-                    view(G, G.npts), I.cell, I.o0, i0, i1, T, Band{}, offsets, base, nullptr, nullptr,
-                    ids, nullptr, cap, ovf);
-                VDB_TRY(c.kt_end("k_fill"));
-                VDB_HIP(hipGetLastError());
-            }
             k_border_table<<<ceil_div(pend - p0, kWavesPerBlock), kBlock, 0, c.stream()>>>(
                 offsets, base, ids, p0, pend, core, bkey, parent, best);
             VDB_HIP(hipGetLastError());
+            if (pt) pt->border += tm.stop();
+            ntiles++;
             i0 = i1;
         }
-        if (pt) pt->border += tm.stop(), pt->tiles += ntiles;
+        if (pt) pt->tiles += ntiles;
         uint32_t hov = 0;
         VDB_HIP(hipMemcpyAsync(&hov, ovf, sizeof hov, hipMemcpyDeviceToHost, c.stream()));
         VDB_HIP(hipStreamSynchronize(c.stream()));
