@@ -5,18 +5,21 @@
 // line "v <id> <clusters> <sum of labels> <noise>" and writes all labels to
 // the file given as argv[2] for comparison with vdb_cluster.
 //
-// usage: api_check <points.bin> <labels.out>
+// usage: api_check <points.bin> <labels.out> [scratch]
 //   points.bin: u32 n, u32 nvar, f64 x[n], f64 y[n], f64 eps[nvar], u32 minpts[nvar]
+//   scratch: every variant through algDBScanParallel (no reuse)
 #include "vdbscan.hpp"
 
 #include <algorithm>
 #include <cstdio>
 #include <cstdint>
+#include <string>
 #include <vector>
 
 int main(int argc, char **argv)
 {
-    if (argc != 3) return 2;
+    if (argc != 3 && argc != 4) return 2;
+    bool scratch = argc == 4 && std::string(argv[3]) == "scratch";
     FILE *f = fopen(argv[1], "rb");
     if (!f) return 3;
     uint32_t n = 0, nv = 0;
@@ -41,7 +44,7 @@ int main(int argc, char **argv)
     for (size_t i = 0; i < 2 * ex.size(); i++) {
         int reuse = 0, clus = 0;
         if (!sched.determineReuse((int)i, &reuse, &clus)) continue;
-        if (reuse == clus) inst[clus]->algDBScanParallel();
+        if (reuse == clus || scratch) inst[clus]->algDBScanParallel();
         else inst[clus]->assignPointsToPredefinedCluster(reuse);
         sched.finishedList[clus] = true;
     }
