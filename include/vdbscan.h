@@ -148,6 +148,25 @@ int vdb_cluster(const double *x, const double *y, uint32_t n, const double *eps,
                 const uint32_t *minpts, uint32_t nvar, uint32_t *labels,
                 uint32_t flags, const vdb_options *opt, vdb_stats *stats);
 
+/* Allocation-free variant of vdb_cluster with VDB_DEVICE_PTRS. No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation). x, y and labels are device pointers, eps and minpts
+ * host arrays, labels in retArr layout. Every variant runs on the per-variant
+ * fused engine (opt->path VDB_PATH_AUTO or VDB_PATH_FUSED; either border rule
+ * and precision; cells_per_eps 0 means 2, no autotuner). All other device
+ * memory is a static array of 128 MiB per GPU inside the library, held until
+ * the process ends: at most 262,144 points per call (more returns
+ * VDB_EUNSUPPORTED, as does a call whose grids outgrow the array). The call
+ * allocates no device or pinned memory, creates no events or graphs, and
+ * waits for its work before returning. Calls on one GPU share the array
+ * (they are serialised; overlapping calls would race on it). stats gets
+ * path_used only. Make any other library call once on the device first (it
+ * creates the library's stream). */
+int vdb_cluster_noalloc(const double *x, const double *y, uint32_t n, const double *eps,
+                        const uint32_t *minpts, uint32_t nvar, uint32_t *labels,
+                        const vdb_options *opt, vdb_stats *stats);
+
 /* Many independent datasets in one call: dataset d is points
  * [offsets[d], offsets[d+1]) of x, y (offsets has nsets+1 entries, offsets[0]
  * = 0), every dataset clustered under the same nvar variants. Labels of

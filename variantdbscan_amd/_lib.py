@@ -24,7 +24,8 @@ EXPORTED = ("libVDBSCAN", "vdb_last_error", "vdb_version", "vdb_set_stream", "vd
             "vdb_cluster", "vdb_rank_order", "vdb_neighbor_counts", "vdb_neighbor_table",
             "vdb_neighbor_table_digest", "vdb_cluster_multi", "vdb_read_xy_csv",
             "vdb_read_lonlat_csv", "vdb_read_variants_csv", "vdb_write_labels_csv", "vdb_free",
-            "vdb_time_kernel", "vdb_kernel_time", "vdb_tuning", "vdb_batch_info")
+            "vdb_time_kernel", "vdb_kernel_time", "vdb_tuning", "vdb_cluster_noalloc",
+            "vdb_batch_info")
 
 
 class VdbOptions(ctypes.Structure):

@@ -38,8 +38,42 @@ Context::~Context()
     if (own_stream_) hipStreamDestroy(own_stream_);
 }
 
+void Context::arena_begin(void *base, size_t bytes)
+{
+    arena_ = (char *)base;
+    arena_cap_ = bytes;
+    arena_used_ = 0;
+    arena_map_.clear();
+}
+
+size_t Context::arena_end()
+{
+    size_t used = arena_used_;
+    arena_ = nullptr;
+    arena_cap_ = arena_used_ = 0;
+    arena_map_.clear();
+    return used;
+}
+
 int Context::get(const char *name, size_t bytes, void **out)
 {
+    if (arena_) {
+        Buffer &a = arena_map_[name];
+        if (a.bytes < bytes || !a.ptr) {
+            size_t want = ((bytes < 256 ? 256 : bytes) + 255) & ~(size_t)255;
+            if (want > arena_cap_ - arena_used_) {
+                set_error(std::string("the allocation-free entry point's scratch (") +
+                          std::to_string(arena_cap_) + " bytes) is too small for this call ('" +
+                          name + "' needs " + std::to_string(want) + " more bytes)");
+                return kErrUnsupported;
+            }
+            a.ptr = arena_ + arena_used_;
+            a.bytes = want;
+            arena_used_ += want;
+        }
+        *out = a.ptr;
+        return kOk;
+    }
     Buffer &b = pool_[name];
     if (b.bytes < bytes) {
         if (b.ptr) {
@@ -101,6 +135,14 @@ size_t Context::slot_bytes(const char *name) const
 
 int Context::host_scratch(size_t bytes, void **out)
 {
+    if (arena_) {
+        if (bytes > sizeof host_static_) {
+            set_error("host staging beyond the allocation-free entry point's static buffer");
+            return kErrUnsupported;
+        }
+        *out = host_static_;
+        return kOk;
+    }
     if (host_bytes_ < bytes) {
         if (host_) {
             VDB_HIP(hipStreamSynchronize(stream_));

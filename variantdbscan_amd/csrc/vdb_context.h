@@ -50,6 +50,17 @@ public:
     // Pinned host staging for small device->host reads.
     int host_scratch(size_t bytes, void **out);
 
+    // Allocation-free mode (vdb_cluster_noalloc): between arena_begin and
+    // arena_end, get() carves named slices of the given device region instead
+    // of allocating (same name, same slice while it is large enough; a larger
+    // request takes a new slice), host_scratch() returns a static host
+    // buffer, and per-kernel timing records nothing. A request beyond the
+    // region fails with kErrUnsupported. arena_end returns the bytes used.
+    void arena_begin(void *base, size_t bytes);
+    size_t arena_end();
+    bool in_arena() const { return arena_ != nullptr; }
+    void *noalloc_scratch = nullptr; // this device's static scratch array
+
     int sync();
 
     std::mutex mu; // serialises library calls on this device
@@ -72,7 +83,10 @@ public:
     // kt_read() sums their durations. Events are created once and reused.
     void kt_select(const char *name);
     bool kt_active() const { return !kt_name_.empty(); }
-    bool kt_on(const char *name) const { return !kt_name_.empty() && kt_name_ == name; }
+    bool kt_on(const char *name) const
+    {
+        return !arena_ && !kt_name_.empty() && kt_name_ == name;
+    }
     int kt_begin(const char *name);
     int kt_end(const char *name);
     int kt_read(double *ms, uint64_t *launches);
@@ -101,6 +115,10 @@ private:
     bool tuned_loaded_ = false;
     std::string dev_tag_; // "<arch>:<CUs>:" prefix of tuning keys
     void tuned_load();
+    char *arena_ = nullptr;
+    size_t arena_cap_ = 0, arena_used_ = 0;
+    std::unordered_map<std::string, Buffer> arena_map_;
+    alignas(64) unsigned char host_static_[256];
     uint64_t epoch_ = 1;            // bumped by every pool allocation and release
     uint64_t graph_pending_ = 0;    // key of the last sequence run directly
     std::vector<std::pair<uint64_t, hipGraphExec_t>> graphs_; // oldest first
