@@ -71,10 +71,10 @@ def batch_kernel_bytes(name, info, W):
     """Algorithmic bytes per step of a batched-engine kernel (summed over its
     launches; None for k_test and k_roots_min, whose traffic follows the
     data). info: vdb_batch_info; W: fine-grid window width (2R+1).
-      k_sweep       per pair entering at its level: its order entry, cells
-                    and levels (14); per carried pair its record (12); per
-                    settled record the two cells' parents (8); per record
-                    written (carried on or a test slot) 12
+      k_sweep       per record read (a pair entering at its level or carried
+                    from an earlier one) 12; per record the two cells'
+                    parents (8); per record written (carried on or a test
+                    slot) 12
       k_knn         points read once (16 N), per listed point its position (4),
                     its L sorted (s, id) entries (12 L) and count (4) written,
                     per cell its start, key, dataset (12), box (32) and W
@@ -100,7 +100,7 @@ def batch_kernel_bytes(name, info, W):
     V = info.get("snapshots") or info["levels"] * J  # one per requested (level, minpts)
     fr, ca, te = info.get("fresh_pairs", 0), info.get("carried_pairs", 0), info.get("test_records", 0)
     return {
-        "k_sweep": (14 * fr + 12 * ca + 8 * (fr + ca) + 12 * (ca + te)) if fr + ca else None,
+        "k_sweep": (12 * (fr + ca) + 8 * (fr + ca) + 12 * (ca + te)) if fr + ca else None,
         "k_labels_all": (16 + 8 * V) * n,
         "k_rows_all": (12 + 8 * V) * n,
         "k_knn": 16 * n + nl * (8 + 12 * L) + nc * (44 + 8 * W),

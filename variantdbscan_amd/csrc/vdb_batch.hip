@@ -776,8 +776,8 @@ struct LevelLists {
     PairRec *lout;           // carried out of it
     PairRec *near, *far;     // test lists: j at + j*tcap
     uint32_t *cnt;           // j at cnt + j*3*(K+1): keep[K+1], near[K+1], far[K+1]
-    const uint32_t *order;   // sorted pair order of all minpts values
-    const uint32_t *bstart;  // J*(K+1)+1 bucket starts into order
+    const PairRec *frec;     // pair records of all minpts values, sorted by (j, level)
+    const uint32_t *bstart;  // J*(K+1)+1 bucket starts into frec
     size_t lstride;
     uint32_t ncell, tcap, K;
     uint32_t *ovf;
@@ -902,6 +902,16 @@ __global__ void k_pair_elig(const uint32_t *pa, const uint32_t *pb, const uint16
 }
 
 // bstart[k] = first index of sorted keys with key >= k, for k = 0..K+1
+// records of the radix-sorted pair order (the bucket sort writes them itself)
+__global__ void k_pack_records(const uint32_t *order, const uint32_t *pa, const uint32_t *pb,
+                               const uint16_t *pk, uint32_t n, PairRec *rec)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t o = order[i], v = o & 0x7fffffffu;
+    rec[i] = PairRec{pa[v], pb[v], (uint32_t)pk[v] | ((o >> 31) << 16)};
+}
+
 __global__ void k_bucket_starts(const uint32_t *key, uint32_t n, int K, uint32_t *bstart)
 {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -1006,10 +1016,9 @@ __device__ __forceinline__ uint32_t level_blocks(const uint32_t *bstart, int k, 
 }
 
 // Grid-stride over the carried pairs (count on the device) followed by the
-// pairs that become eligible at this level, read through the level-sorted
-// pair order (order[i] = pair index | exact-level flag in bit 31; a stable
-// sort keeps the indices of one level ascending, so the gathers of a wave
-// stay close together). Each eligible pair is settled
+// pairs that become eligible at this level, whose records (cells, levels,
+// exact-level flag in bit 16 of k) the sort has laid out contiguously per
+// (minpts value, level). Each eligible pair is settled
 // (already connected; fully inside eps; small and tested here), kept for the
 // next level, or handed to a wave test. Tests of grid-adjacent cells go to
 // `near`, the rest to `far`; the near tests run first so most far pairs are
@@ -1019,7 +1028,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(LevelArgs a, LevelLists L)
     a.j += (int)blockIdx.y; // global minpts index (kc); lists use the pass-relative one
     LevelJ lj = level_j(L, blockIdx.y, a.k);
     const PairRec *carried = lj.carried;
-    const uint32_t *ncarried = lj.nc, *bstart = lj.bstart, *order = L.order;
+    const uint32_t *ncarried = lj.nc, *bstart = lj.bstart;
     uint32_t *parent = lj.parent;
     PairRec *keep = lj.keep, *near = lj.near, *far = lj.far;
     uint32_t *nkeep = lj.nkeep, *nnear = lj.nnear, *nfar = lj.nfar, *ovf = L.ovf;
@@ -1028,25 +1037,20 @@ __global__ void __launch_bounds__(kBlock) k_sweep(LevelArgs a, LevelLists L)
     bool chunked = bstart[a.k + 1] - bstart[a.k] >= kChunkedFrom;
     uint32_t nb = level_blocks(bstart, a.k, L.ppb, L.minb);
     if (blockIdx.x >= nb) return;
-    const uint32_t *fresh = order + bstart[a.k];
+    const PairRec *fresh = L.frec + bstart[a.k];
     uint32_t nfresh = bstart[a.k + 1] - bstart[a.k];
     uint32_t nc = *ncarried;
     uint32_t total = nc + nfresh;
     auto load_rec = [&](uint32_t i) {
         PairRec r{0, 0, 0};
         if (i < total) {
-            if (i < nc) {
-                r = carried[i];
-            } else {
-                uint32_t o = fresh[i - nc], v = o & 0x7fffffffu;
-                r = PairRec{a.pa[v], a.pb[v], (uint32_t)a.pk[v] | ((o >> 31) << 16)};
-            }
+            r = i < nc ? carried[i] : fresh[i - nc];
         }
         return r;
     };
     const uint32_t step = nb * kBlock;
-    // the next step's record is loaded while this one is settled: its loads
-    // (order, then the pair arrays) do not wait behind this step's parent loads
+    // the next step's record is loaded while this one is settled: its load
+    // does not wait behind this step's parent loads
     PairRec next = load_rec(blockIdx.x * kBlock + threadIdx.x);
     for (uint32_t base = blockIdx.x * kBlock; base < total; base += step) {
         uint32_t i = base + threadIdx.x;
@@ -1987,7 +1991,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     const uint32_t nw = (n + 31) / 32;
     const uint32_t ncnt = 3 * ((uint32_t)K + 1);
     const size_t per_j = 8 * (size_t)G.ncell + 2 * sizeof(PairRec) * (psz + (size_t)tcap) +
-                         16 * psz + 4 * (size_t)ncnt;
+                         (16 + sizeof(PairRec)) * psz + 4 * (size_t)ncnt;
     const size_t kPassBytes = (size_t)64 << 30;
     int jc = J;
     while (jc > 1 && ((uint64_t)jc * psz > 0x7fffffffull || (size_t)jc * per_j > kPassBytes))
@@ -2010,8 +2014,14 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_TRY(c.get("b.cnts", (size_t)jc * ncnt, &cnts));
     VDB_TRY(c.get("b.ekey", (size_t)jc * psz, &ekey));
     VDB_TRY(c.get("b.eval", (size_t)jc * psz, &eval));
-    VDB_TRY(c.get("b.stk", (size_t)jc * psz, &stk));
-    VDB_TRY(c.get("b.stv", (size_t)jc * psz, &stv));
+    // radix sort buffers: only passes of more than 128 (minpts, level) buckets
+    stk = stv = nullptr;
+    if ((size_t)jc * (K + 1) > 128) {
+        VDB_TRY(c.get("b.stk", (size_t)jc * psz, &stk));
+        VDB_TRY(c.get("b.stv", (size_t)jc * psz, &stv));
+    }
+    PairRec *frec; // pair records sorted by (minpts value, level)
+    VDB_TRY(c.get("b.frec", (size_t)jc * psz, &frec));
     VDB_TRY(c.get("b.bstart", (size_t)jc * (K + 1) + 2, &bstart));
     VDB_TRY(c.get("b.tovf", 1, &tovf));
     // per-slot snapshots: cell root ranks, cell labels, the root ranks as a
@@ -2114,7 +2124,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             // pairs bucketed by (minpts value, level at which they become
             // eligible): bucket j*(K+1)+k of bstart
             const uint32_t nb = (uint32_t)nj * (uint32_t)(K + 1);
-            const uint32_t *order = stv; // unread without pairs
+
             if (npairs) {
                 for (int g0 = 0; g0 < nj; g0 += kEligGroup) {
                     int gn = std::min(kEligGroup, nj - g0);
@@ -2127,16 +2137,19 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
                 }
                 uint32_t nkeys = (uint32_t)nj * npairs;
                 if (nb <= 128) {
-                    // one counting pass, which also gives the bucket starts
-                    VDB_TRY(bucket_sort_values(c, ekey, eval, stv, nkeys, (int)nb, bstart));
-                    order = stv;
+                    // one counting pass, which also gives the bucket starts,
+                    // writing the pair records in key order
+                    VDB_TRY(bucket_sort_records(c, ekey, eval, pa, pb, pk, (uint32_t *)frec, nkeys,
+                                                (int)nb, bstart));
                 } else {
                     uint32_t *sk, *sv;
                     VDB_TRY(radix_sort_pairs(c, ekey, eval, stk, stv, nkeys, bits_for(nb), &sk, &sv));
                     k_bucket_starts<<<ceil_div((size_t)nkeys + 1, kBlock), kBlock, 0, c.stream()>>>(
                         sk, nkeys, (int)nb - 1, bstart);
                     VDB_HIP(hipGetLastError());
-                    order = sv;
+                    k_pack_records<<<ceil_div(nkeys, kBlock), kBlock, 0, c.stream()>>>(
+                        sv, pa, pb, pk, nkeys, frec);
+                    VDB_HIP(hipGetLastError());
                 }
             } else {
                 k_fill_u32<<<ceil_div((size_t)nb + 1, kBlock), kBlock, 0, c.stream()>>>(bstart, 0u,
@@ -2147,7 +2160,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             for (int k = 0; k <= kmax; k++) {
                 LevelArgs a{G.xs, G.ys, G.cell_box, G.cell_start, G.cell_key, G.cell_ds, G.skbase,
                             G.sncy, kc, pa, pb, pk, J, j0, k, Th[k]};
-                LevelLists LL{parent, lin, lout, tests, tests2, cnts, order, bstart,
+                LevelLists LL{parent, lin, lout, tests, tests2, cnts, frec, bstart,
                               psz, G.ncell, tcap, (uint32_t)K, tovf, shape.ppb, shape.minb,
                               shape.tpb};
                 // worklist sizes stay on the device: bounded grids stride over them

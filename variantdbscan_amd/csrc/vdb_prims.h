@@ -29,6 +29,14 @@ int radix_sort_pairs(Context &c, uint32_t *keys, uint32_t *vals, uint32_t *tmp_k
 int bucket_sort_values(Context &c, const uint32_t *keys, const uint32_t *vals,
                        uint32_t *out_vals, uint32_t n, int nbuckets, uint32_t *starts);
 
+// The same sort, writing in place of each value v a three-word record
+// (a[i], b[i], c[i] | (v >> 31) << 16) with i = v & 0x7fffffff: the
+// records of three parallel arrays, gathered in input order (coalesced) and
+// stored in key order, so that a consumer streams them.
+int bucket_sort_records(Context &c, const uint32_t *keys, const uint32_t *vals,
+                        const uint32_t *a, const uint32_t *b, const uint16_t *cc,
+                        uint32_t *out_rec, uint32_t n, int nbuckets, uint32_t *starts);
+
 // ext_dev[0..3] = (min x, max x, min y, max y) over n points (n >= 1).
 int extent_f64(Context &c, const double *x, const double *y, size_t n, double *ext_dev);
 

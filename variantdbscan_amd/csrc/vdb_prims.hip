@@ -242,10 +242,31 @@ __global__ void __launch_bounds__(kSortBlock)
     }
 }
 
-template <int BITS>
+// what the scatter moves: the value itself, or the record it indexes
+struct MoveValue {
+    const uint32_t *vals;
+    uint32_t *out;
+    using T = uint32_t;
+    __device__ T load(size_t i) const { return vals[i]; }
+    __device__ void store(uint32_t pos, const T &v) const { out[pos] = v; }
+};
+struct MoveRecord {
+    const uint32_t *vals, *a, *b;
+    const uint16_t *c;
+    uint3 *out;
+    using T = uint3;
+    __device__ T load(size_t i) const
+    {
+        uint32_t v = vals[i], q = v & 0x7fffffffu;
+        return make_uint3(a[q], b[q], (uint32_t)c[q] | ((v >> 31) << 16));
+    }
+    __device__ void store(uint32_t pos, const T &v) const { out[pos] = v; }
+};
+
+template <int BITS, class Move>
 __global__ void __launch_bounds__(kSortBlock)
-    k_bucket_scatter(const uint32_t *keys, const uint32_t *vals, uint32_t *ovals, size_t n,
-                     const uint32_t *offs, unsigned tiles)
+    k_bucket_scatter(const uint32_t *keys, Move mv, size_t n, const uint32_t *offs,
+                     unsigned tiles)
 {
     constexpr int kBuckets = 1 << BITS;
     __shared__ uint32_t base_d[kBuckets];
@@ -260,7 +281,8 @@ __global__ void __launch_bounds__(kSortBlock)
         size_t idx = tbase + (size_t)it * kSortBlock + threadIdx.x;
         bool valid = idx < n;
         uint32_t d = valid ? keys[idx] : (uint32_t)kBuckets;
-        uint32_t v = valid ? vals[idx] : 0u;
+        typename Move::T v{};
+        if (valid) v = mv.load(idx);
         uint64_t peers = bucket_peers<BITS>(d);
         int rank_w = mask_rank(peers);
         if (valid && rank_w == 0) wcnt[w][d] = (uint32_t)popc64(peers);
@@ -268,7 +290,7 @@ __global__ void __launch_bounds__(kSortBlock)
         if (valid) {
             uint32_t pre = 0;
             for (int q = 0; q < w; q++) pre += wcnt[q][d];
-            ovals[base_d[d] + pre + (uint32_t)rank_w] = v;
+            mv.store(base_d[d] + pre + (uint32_t)rank_w, v);
         }
         __syncthreads();
         for (int dd = threadIdx.x; dd < kBuckets; dd += kSortBlock) {
@@ -413,9 +435,9 @@ int radix_sort_pairs(Context &c, uint32_t *keys, uint32_t *vals, uint32_t *tmp_k
     return kOk;
 }
 
-template <int BITS>
-int bucket_sort_bits(Context &c, const uint32_t *keys, const uint32_t *vals, uint32_t *out_vals,
-                     uint32_t n, uint32_t *offs_out[1])
+template <int BITS, class Move>
+int bucket_sort_bits(Context &c, const uint32_t *keys, const Move &mv, uint32_t n,
+                     uint32_t *offs_out[1])
 {
     constexpr int kBuckets = 1 << BITS;
     unsigned tiles = ceil_div(n, kSortTile);
@@ -425,15 +447,15 @@ int bucket_sort_bits(Context &c, const uint32_t *keys, const uint32_t *vals, uin
     k_bucket_hist<BITS><<<tiles, kSortBlock, 0, c.stream()>>>(keys, n, hist, tiles);
     VDB_HIP(hipGetLastError());
     VDB_TRY(scan_exclusive_u32(c, hist, offs, (size_t)tiles * kBuckets, nullptr));
-    k_bucket_scatter<BITS><<<tiles, kSortBlock, 0, c.stream()>>>(keys, vals, out_vals, n, offs,
-                                                                   tiles);
+    k_bucket_scatter<BITS, Move><<<tiles, kSortBlock, 0, c.stream()>>>(keys, mv, n, offs, tiles);
     VDB_HIP(hipGetLastError());
     *offs_out = offs;
     return kOk;
 }
 
-int bucket_sort_values(Context &c, const uint32_t *keys, const uint32_t *vals,
-                       uint32_t *out_vals, uint32_t n, int nbuckets, uint32_t *starts)
+template <class Move>
+int bucket_sort_move(Context &c, const uint32_t *keys, const Move &mv, uint32_t n, int nbuckets,
+                     uint32_t *starts)
 {
     if (nbuckets < 1 || nbuckets > (1 << kMaxBucketBits) || n == 0) {
         set_error("bucket sort: 1..128 buckets and at least one value");
@@ -441,12 +463,26 @@ int bucket_sort_values(Context &c, const uint32_t *keys, const uint32_t *vals,
     }
     unsigned tiles = ceil_div(n, kSortTile);
     uint32_t *offs = nullptr;
-    if (nbuckets <= 16) VDB_TRY(bucket_sort_bits<4>(c, keys, vals, out_vals, n, &offs));
-    else VDB_TRY(bucket_sort_bits<kMaxBucketBits>(c, keys, vals, out_vals, n, &offs));
+    if (nbuckets <= 16) VDB_TRY(bucket_sort_bits<4>(c, keys, mv, n, &offs));
+    else VDB_TRY(bucket_sort_bits<kMaxBucketBits>(c, keys, mv, n, &offs));
     // one thread per bucket start (up to 129)
     k_bucket_bounds<<<1, kSortBlock, 0, c.stream()>>>(offs, tiles, nbuckets, n, starts);
     VDB_HIP(hipGetLastError());
     return kOk;
+}
+
+int bucket_sort_values(Context &c, const uint32_t *keys, const uint32_t *vals,
+                       uint32_t *out_vals, uint32_t n, int nbuckets, uint32_t *starts)
+{
+    return bucket_sort_move(c, keys, MoveValue{vals, out_vals}, n, nbuckets, starts);
+}
+
+int bucket_sort_records(Context &c, const uint32_t *keys, const uint32_t *vals,
+                        const uint32_t *a, const uint32_t *b, const uint16_t *cc,
+                        uint32_t *out_rec, uint32_t n, int nbuckets, uint32_t *starts)
+{
+    return bucket_sort_move(c, keys, MoveRecord{vals, a, b, cc, (uint3 *)out_rec}, n, nbuckets,
+                            starts);
 }
 
 int extent_f64(Context &c, const double *x, const double *y, size_t n, double *ext_dev)
