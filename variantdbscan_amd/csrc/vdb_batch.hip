@@ -1534,6 +1534,15 @@ __global__ void __launch_bounds__(kBlock)
                  uint32_t n)
 {
     static_assert((JG * KB) % 4 == 0 && KB <= 16 && JG <= kAllJ, "whole 16-byte groups");
+    // the call's thresholds, slot table and requests in LDS: the walk reads
+    // them at run-time indices without scalar memory round trips
+    __shared__ double sT[16];
+    __shared__ int8_t sslot[kAllJ][16];
+    __shared__ uint16_t sjreq[kAllJ];
+    if (threadIdx.x < 16) sT[threadIdx.x] = S.T[threadIdx.x];
+    if (threadIdx.x < kAllJ * 16) (&sslot[0][0])[threadIdx.x] = (&S.slot[0][0])[threadIdx.x];
+    if (threadIdx.x < kAllJ) sjreq[threadIdx.x] = S.jreq[threadIdx.x];
+    __syncthreads();
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= n) return;
     const uint32_t r = pos_rank[p];
@@ -1551,9 +1560,9 @@ __global__ void __launch_bounds__(kBlock)
         for (int jj = 0; jj < JG; jj++) {
             int j = j0 + jj;
             kp[jj] = j < S.J ? (int)((kp4 >> (4 * j)) & 15u) : 0;
-            uint32_t below = S.jreq[j < S.J ? j : 0] & ((1u << kp[jj]) - 1u);
+            uint32_t below = sjreq[j < S.J ? j : 0] & ((1u << kp[jj]) - 1u);
             if (j < S.J && below) {
-                double t = S.T[31 - __builtin_clz(below)];
+                double t = sT[31 - __builtin_clz(below)];
                 Tmax = t > Tmax ? t : Tmax;
             }
         }
@@ -1589,7 +1598,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
                 for (int u = 0; u < kU; u++) {
                     if (!(se[u] <= Tmax)) break;
-                    while (ke < K && se[u] > S.T[ke]) ke++;
+                    while (ke < K && se[u] > sT[ke]) ke++;
                     uint32_t v[JG][KB];
 #pragma unroll
                     for (int jj = 0; jj < JG; jj++) {
@@ -1600,7 +1609,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
                         for (int k = 0; k < KB; k++) {
                             v[jj][k] = kNone;
-                            int t = S.slot[j & 7][k];
+                            int t = sslot[j & 7][k];
                             if (j < S.J && t >= 0 && k >= lo && k < hi)
                                 v[jj][k] = S.root[(size_t)t * S.ncell + cq[u]];
                         }
@@ -1621,7 +1630,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
             for (int k = 0; k < KB; k++) {
                 uint32_t x = 0u;
-                int t = S.slot[j & 7][k];
+                int t = sslot[j & 7][k];
                 if (j < S.J && t >= 0) {
                     if (kp[jj] <= k) {
                         x = S.lab[(size_t)t * S.ncell + C];
