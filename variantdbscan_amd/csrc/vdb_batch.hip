@@ -264,7 +264,8 @@ __device__ __forceinline__ void wave_sort(double &es, uint32_t &eid, int cnt)
 // 8 waves per SIMD: the search is latency-bound, so occupancy pays more
 // than the scalar registers it costs (their spills go to vector lanes)
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
-    k_knn(FG g, const uint32_t *__restrict__ list_pos, uint32_t nlist, int L, double Tmax,
+    k_knn(FG g, const uint32_t *__restrict__ list_pos, const uint32_t *__restrict__ nlist_dev,
+          int L, double Tmax,
           double T0, double *__restrict__ lst_s, uint32_t *__restrict__ lst_id,
           uint32_t *__restrict__ lst_n)
 {
@@ -276,7 +277,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 
     __shared__ double abuf_s[kWPB][kWave];
     __shared__ uint32_t abuf_id[kWPB][kWave];
     uint32_t i = blockIdx.x * kWPB + wave_id_in_block();
-    if (i >= nlist) return;
+    if (i >= *nlist_dev) return; // launched for an upper bound of the count
     double *bs = abuf_s[wave_id_in_block()];
     uint32_t *bid = abuf_id[wave_id_in_block()];
     int lane = lane_id();
@@ -1736,23 +1737,42 @@ __global__ void k_isolated_labels(const uint32_t *mapping, const uint32_t *seg, 
     out[(size_t)s0 * nrows + (size_t)row * nd + (i - s0)] = lbl;
 }
 
+// Launches on another stream of the library while in scope (c.stream() is
+// that stream), then back on the call's stream.
+struct StreamScope {
+    Context &c;
+    hipStream_t prev;
+    StreamScope(Context &cx, hipStream_t s) : c(cx), prev(cx.stream()) { c.set_stream(s); }
+    ~StreamScope() { c.set_stream(prev); }
+};
+
+// Phase timer on a stream: events only when asked for (stats), the end
+// recorded by stop() without waiting; ms() waits for it (call it once the
+// call's work is done).
 struct Ev {
-    hipEvent_t a, b;
+    hipEvent_t a = nullptr, b = nullptr;
     hipStream_t s;
-    explicit Ev(hipStream_t st) : s(st)
+    bool stopped = false;
+    Ev(hipStream_t st, bool on) : s(st)
     {
-        hipEventCreate(&a);
-        hipEventCreate(&b);
+        if (!on) return;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
         hipEventRecord(a, s);
     }
     ~Ev()
     {
-        hipEventDestroy(a);
-        hipEventDestroy(b);
+        if (a) hipEventDestroy(a);
+        if (b) hipEventDestroy(b);
+    }
+    void stop()
+    {
+        if (a && b && !stopped) hipEventRecord(b, s);
+        stopped = true;
     }
     double ms()
     {
-        hipEventRecord(b, s);
+        if (!a || !b) return 0.0;
+        stop();
         hipEventSynchronize(b);
         float m = 0;
         hipEventElapsedTime(&m, a, b);
@@ -1823,7 +1843,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         set_error("eps range too wide for one fine grid");
         return kErrUnsupported;
     }
-    Ev total(c.stream());
+    Ev total(c.stream(), st != nullptr || verbose);
     // isolated-point variants
     for (uint32_t v = 0; v < nvar; v++) {
         if (threshold_for_eps(eps[v]) >= 0) continue;
@@ -1837,7 +1857,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     for (int k = 0; k < K; k++) Th[k] = threshold_for_eps(lev[k]);
 
     // ---- fine grid with clique cells at eps_0
-    Ev tprep(c.stream());
+    Ev tprep(c.stream(), st != nullptr);
     double d = lev[0] / std::sqrt(2.0) * (1.0 - std::ldexp(1.0, -20));
     int R = (int)std::ceil(lev[K - 1] * (1.0 + std::ldexp(1.0, -30)) / d);
     // a window row per lane in k_knn (eps ratio 16 gives R = 23)
@@ -1858,12 +1878,6 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     k_clique_check<<<ceil_div(G.ncell, kBlock), kBlock, 0, c.stream()>>>(G.cell_box, G.ncell,
                                                                          Th[0], bad);
     VDB_HIP(hipGetLastError());
-    uint32_t hbad = 0;
-    VDB_TRY(read_u32(c, bad, &hbad));
-    if (hbad) {
-        set_error("fine-grid cells are not cliques at the smallest eps");
-        return kErrUnsupported;
-    }
     double *dTk;
     uint32_t *dmj;
     VDB_TRY(c.get("b.Tk", K, &dTk));
@@ -1872,33 +1886,80 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
                            c.stream()));
     VDB_HIP(hipMemcpyAsync(dmj, mps.data(), sizeof(uint32_t) * J, hipMemcpyHostToDevice,
                            c.stream()));
-    if (st) st->ms_prepare += tprep.ms();
+    tprep.stop();
+    // pinned read-back area of the call: [0, 8) pair count, [8, 12) clique
+    // check, [16, 20) list count, [32, ...) the final overflow flag, list
+    // count and worklist counters
+    const size_t ncnt_all = (size_t)J * 3 * ((size_t)K + 1);
+    char *hs = nullptr;
+    VDB_TRY(c.host_scratch(32 + 4 * (4 + ncnt_all + (size_t)J * (K + 1) + 1), (void **)&hs));
+
+    // ---- cell pairs within eps_max: counted on the library's second
+    // stream beside the lists (both read only the grid); the pair total and
+    // the clique check come back in one read
+    hipStream_t aux;
+    VDB_TRY(c.aux_stream(&aux)); // (also creates the ordering events)
+    const hipStream_t mainst = c.stream();
+    // while a kernel is being timed everything stays on the call's stream,
+    // so that each launch is timed alone
+    const hipStream_t ps = c.kt_active() ? mainst : aux;
+    VDB_HIP(hipEventRecord(c.aux_event(0), mainst));
+    VDB_HIP(hipStreamWaitEvent(ps, c.aux_event(0), 0));
+    Ev tpairs(ps, st != nullptr);
+    uint32_t *pcnt;
+    uint64_t *poff;
+    VDB_TRY(c.get("b.pcnt", G.ncell, &pcnt));
+    VDB_TRY(c.get("b.poff", (size_t)G.ncell + 1, &poff));
+    unsigned gc = ceil_div(G.ncell, kBlock);
+    unsigned gcw = ceil_div(G.ncell, kWPB); // a wave per cell
+    int W = 2 * R + 1;
+    {
+        StreamScope on(c, ps);
+        VDB_TRY(c.kt_begin("k_pair_count"));
+        k_pair_count<<<gc, kBlock, 0, c.stream()>>>(G.cell_box, G.row_lo, G.row_hi, W, G.ncell,
+                                                    dTk, K, pcnt);
+        VDB_TRY(c.kt_end("k_pair_count"));
+        VDB_HIP(hipGetLastError());
+        VDB_TRY(scan_exclusive_u32_u64(c, pcnt, poff, G.ncell, poff + G.ncell));
+        VDB_HIP(hipMemcpyAsync(hs, poff + G.ncell, 8, hipMemcpyDeviceToHost, ps));
+        VDB_HIP(hipMemcpyAsync(hs + 8, bad, 4, hipMemcpyDeviceToHost, ps));
+        VDB_HIP(hipEventRecord(c.aux_event(1), ps));
+    }
 
     // ---- truncated distance-sorted table
-    Ev tknn(c.stream());
-    uint32_t *flag, *excl, *list_pos, *lst_idx;
+    Ev tknn(c.stream(), st != nullptr);
+    uint32_t *flag, *excl, *list_pos, *lst_idx, *nlist_dev;
     VDB_TRY(c.get("b.flag", n, &flag));
     VDB_TRY(c.get("b.excl", n, &excl));
     VDB_TRY(c.get("b.lst_idx", n, &lst_idx));
+    VDB_TRY(c.get("b.nlist", 1, &nlist_dev));
     unsigned gn = ceil_div(n, kBlock);
     k_list_flags<<<gn, kBlock, 0, c.stream()>>>(G.cell_start, G.cell_of_pos, n, L, flag);
     VDB_HIP(hipGetLastError());
-    uint32_t nlist = 0;
-    VDB_TRY(compact_count(c, flag, excl, n, &nlist));
-    VDB_TRY(c.get("b.list_pos", nlist ? nlist : 1, &list_pos));
+    VDB_TRY(scan_exclusive_u32(c, flag, excl, n, nlist_dev));
+    // Lists are sized for every point, so that nothing waits for their count
+    // (the truncated-list kernel reads it on the device), unless that takes
+    // more than 8 GB: then the count is read first.
+    const bool exact_lists = (size_t)n * L * 12 > ((size_t)8 << 30);
+    uint32_t nlist_cap = n;
+    if (exact_lists) {
+        VDB_HIP(hipMemcpyAsync(hs + 16, nlist_dev, 4, hipMemcpyDeviceToHost, c.stream()));
+        VDB_HIP(hipStreamSynchronize(c.stream()));
+        memcpy(&nlist_cap, hs + 16, 4);
+    }
+    VDB_TRY(c.get("b.list_pos", nlist_cap ? nlist_cap : 1, &list_pos));
     k_list_index<<<gn, kBlock, 0, c.stream()>>>(flag, excl, n, list_pos, lst_idx);
     VDB_HIP(hipGetLastError());
     double *lst_s;
     uint32_t *lst_id, *lst_n;
-    size_t lsz = (size_t)(nlist ? (nlist + 63) / 64 * 64 : 64) * L; // whole blocks of 64 lists
+    size_t lsz = (size_t)(nlist_cap ? (nlist_cap + 63) / 64 * 64 : 64) * L; // blocks of 64 lists
     VDB_TRY(c.get("b.lst_s", lsz, &lst_s));
     VDB_TRY(c.get("b.lst_id", lsz, &lst_id));
-    VDB_TRY(c.get("b.lst_n", nlist ? nlist : 1, &lst_n));
-    if (nlist) {
+    VDB_TRY(c.get("b.lst_n", nlist_cap ? nlist_cap : 1, &lst_n));
+    if (nlist_cap) {
         VDB_TRY(c.kt_begin("k_knn"));
-        k_knn<<<ceil_div(nlist, kWPB), kBlock, 0, c.stream()>>>(fview(G), list_pos, nlist,
-                                                                (int)L, Th[K - 1], Th[0], lst_s,
-                                                                lst_id, lst_n);
+        k_knn<<<ceil_div(nlist_cap, kWPB), kBlock, 0, c.stream()>>>(
+            fview(G), list_pos, nlist_dev, (int)L, Th[K - 1], Th[0], lst_s, lst_id, lst_n);
         VDB_TRY(c.kt_end("k_knn"));
     }
     VDB_HIP(hipGetLastError());
@@ -1930,30 +1991,17 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         k_cell_core<<<gc0, kBlock, 0, c.stream()>>>(G.cell_start, G.ncell, kc, J, K, G.pos_rank,
                                                     cmr, cellkc, cellkc_cm);
     VDB_HIP(hipGetLastError());
-    if (st) st->ms_count += tknn.ms();
+    tknn.stop();
 
-    // ---- cell pairs within eps_max
-    Ev tpairs(c.stream());
-    uint32_t *pcnt;
-    uint64_t *poff;
-    VDB_TRY(c.get("b.pcnt", G.ncell, &pcnt));
-    VDB_TRY(c.get("b.poff", (size_t)G.ncell + 1, &poff));
-    unsigned gc = ceil_div(G.ncell, kBlock);
-    unsigned gcw = ceil_div(G.ncell, kWPB); // a wave per cell
-    int W = 2 * R + 1;
-    VDB_TRY(c.kt_begin("k_pair_count"));
-    k_pair_count<<<gc, kBlock, 0, c.stream()>>>(G.cell_box, G.row_lo, G.row_hi, W, G.ncell, dTk,
-                                                K, pcnt);
-    VDB_TRY(c.kt_end("k_pair_count"));
-    VDB_HIP(hipGetLastError());
-    VDB_TRY(scan_exclusive_u32_u64(c, pcnt, poff, G.ncell, poff + G.ncell));
+    // ---- the pair count (and the clique check) read back
+    VDB_HIP(hipEventSynchronize(c.aux_event(1)));
     uint64_t npairs64 = 0;
-    {
-        void *hs;
-        VDB_TRY(c.host_scratch(8, &hs));
-        VDB_HIP(hipMemcpyAsync(hs, poff + G.ncell, 8, hipMemcpyDeviceToHost, c.stream()));
-        VDB_HIP(hipStreamSynchronize(c.stream()));
-        npairs64 = *(uint64_t *)hs;
+    uint32_t hbad = 0;
+    memcpy(&npairs64, hs, 8);
+    memcpy(&hbad, hs + 8, 4);
+    if (hbad) {
+        set_error("fine-grid cells are not cliques at the smallest eps");
+        return kErrUnsupported;
     }
     if (npairs64 >= 0xffffffffull) {
         set_error("too many cell pairs");
@@ -1983,7 +2031,19 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     }
     uint32_t *pos_of_caller;
     VDB_TRY(c.get("b.poc", n, &pos_of_caller));
-    if (st) st->ms_prepare += tpairs.ms();
+    // the cell pairs stored on the second stream too, beside the lists; the
+    // main stream waits for them before the level passes
+    {
+        StreamScope on(c, ps);
+        VDB_TRY(c.kt_begin("k_pairs"));
+        k_pairs<<<gcw, kBlock, 0, c.stream()>>>(G.cell_box, G.row_lo, G.row_hi, W, G.ncell, dTk, K,
+                                                poff, pa, pb, pk);
+        VDB_TRY(c.kt_end("k_pairs"));
+        VDB_HIP(hipGetLastError());
+        VDB_HIP(hipEventRecord(c.aux_event(2), ps));
+    }
+    tpairs.stop();
+    VDB_HIP(hipStreamWaitEvent(mainst, c.aux_event(2), 0));
 
     // ---- variants of each (k, j)
     std::map<std::pair<int, int>, std::vector<uint32_t>> want;
@@ -2102,11 +2162,6 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     // pooled buffers: no host read, no allocation. With `events`, the label
     // passes are bracketed by events (not inside a graph).
     auto sequence = [&](bool events, const BatchShape &shape) -> int {
-        VDB_TRY(c.kt_begin("k_pairs"));
-        k_pairs<<<gcw, kBlock, 0, c.stream()>>>(G.cell_box, G.row_lo, G.row_hi, W, G.ncell, dTk, K,
-                                                poff, pa, pb, pk);
-        VDB_TRY(c.kt_end("k_pairs"));
-        VDB_HIP(hipGetLastError());
         k_pos_of_caller<<<gn, kBlock, 0, c.stream()>>>(caller_of_pos, n, pos_of_caller);
         VDB_HIP(hipGetLastError());
         k_fill_u32<<<1, kBlock, 0, c.stream()>>>(tovf, 0u, 1);
@@ -2339,7 +2394,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         if (n >= (1u << 17) && !verbose && !c.kt_active()) {
             float best_ms = 0;
             for (int t = 0; t < kNumBatchShapes; t++) {
-                Ev tt(c.stream());
+                Ev tt(c.stream(), true);
                 VDB_TRY(sequence(false, kBatchShapes[t]));
                 float ms = (float)tt.ms();
                 if (t == 0 || ms < best_ms) {
@@ -2353,14 +2408,14 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     }
     const BatchShape &shape = kBatchShapes[shape_id];
     if (res) res->shape = shape_id;
-    Ev tuf(c.stream());
+    Ev tuf(c.stream(), st != nullptr);
     double ms_labels = 0;
     const bool graphs = !verbose && !c.kt_active() && !getenv("VDB_NO_GRAPH");
     if (graphs || tuned_now) {
         uint64_t h = 0x9e3779b97f4a7c15ull;
         auto mix = [&](uint64_t v) { h = splitmix(h ^ v); };
         for (uint64_t v : {(uint64_t)n, (uint64_t)nvar, (uint64_t)nrows, (uint64_t)border,
-                           (uint64_t)(uintptr_t)out, (uint64_t)G.ncell, (uint64_t)nlist,
+                           (uint64_t)(uintptr_t)out, (uint64_t)G.ncell, (uint64_t)nlist_cap,
                            (uint64_t)npairs, (uint64_t)tcap, (uint64_t)P.nsets, (uint64_t)L,
                            (uint64_t)K, (uint64_t)J, (uint64_t)jc, (uint64_t)R,
                            (uint64_t)shape_id})
@@ -2376,14 +2431,25 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     } else {
         VDB_TRY(sequence(st != nullptr, shape));
     }
-    double ms_uf_all = tuf.ms();
-    for (size_t e = 0; e + 1 < lab_ev.size(); e += 2) {
-        float m = 0;
-        VDB_HIP(hipEventElapsedTime(&m, lab_ev[e], lab_ev[e + 1]));
-        ms_labels += m;
+    tuf.stop();
+    // one read back at the end: the overflow flag, the list count and (when
+    // all minpts values ran in one pass, whose counters are still in place)
+    // the worklist counters, for the byte model of the level kernels
+    const bool counters = jc == J && npairs;
+    uint32_t *hc = (uint32_t *)(hs + 32 + 8);
+    uint32_t *hb = hc + ncnt_all;
+    VDB_HIP(hipMemcpyAsync(hs + 32, tovf, 4, hipMemcpyDeviceToHost, c.stream()));
+    VDB_HIP(hipMemcpyAsync(hs + 36, nlist_dev, 4, hipMemcpyDeviceToHost, c.stream()));
+    if (counters) {
+        VDB_HIP(hipMemcpyAsync(hc, cnts, sizeof(uint32_t) * ncnt_all, hipMemcpyDeviceToHost,
+                               c.stream()));
+        VDB_HIP(hipMemcpyAsync(hb, bstart, sizeof(uint32_t) * ((size_t)J * (K + 1) + 1),
+                               hipMemcpyDeviceToHost, c.stream()));
     }
-    uint32_t htovf = 0;
-    VDB_TRY(read_u32(c, tovf, &htovf));
+    VDB_HIP(hipStreamSynchronize(c.stream()));
+    uint32_t htovf = 0, nlist = 0;
+    memcpy(&htovf, hs + 32, 4);
+    memcpy(&nlist, hs + 36, 4);
     if (htovf) {
         // labels of this group are incomplete: the caller re-runs the group
         // on the per-variant engine, which rewrites every one of its rows
@@ -2391,23 +2457,23 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         return kErrUnsupported;
     }
     if (st) {
+        double ms_uf_all = tuf.ms();
+        for (size_t e = 0; e + 1 < lab_ev.size(); e += 2) {
+            float m = 0;
+            VDB_HIP(hipEventElapsedTime(&m, lab_ev[e], lab_ev[e + 1]));
+            ms_labels += m;
+        }
+        st->ms_prepare += tprep.ms() + tpairs.ms();
+        st->ms_count += tknn.ms();
         st->ms_uf += ms_uf_all - ms_labels;
         st->ms_labels += ms_labels;
     }
-    // worklist totals of the call (when all minpts values ran in one pass,
-    // whose counters are still in place): the byte model of the level kernels
     uint64_t nsnap = want.size(), nfresh_all = 0, ncarried_all = 0, ntests_all = 0;
-    if (jc == J && npairs) {
-        std::vector<uint32_t> hc((size_t)J * ncnt), hb((size_t)J * (K + 1) + 1);
-        VDB_HIP(hipMemcpyAsync(hc.data(), cnts, sizeof(uint32_t) * hc.size(),
-                               hipMemcpyDeviceToHost, c.stream()));
-        VDB_HIP(hipMemcpyAsync(hb.data(), bstart, sizeof(uint32_t) * hb.size(),
-                               hipMemcpyDeviceToHost, c.stream()));
-        VDB_HIP(hipStreamSynchronize(c.stream()));
+    if (counters) {
         int kmax = -1;
         for (auto &w : want) kmax = std::max(kmax, w.first.first);
         for (int j = 0; j < J; j++) {
-            const uint32_t *h = hc.data() + (size_t)j * ncnt;
+            const uint32_t *h = hc + (size_t)j * ncnt;
             for (int k = 0; k <= kmax; k++) {
                 size_t b0 = (size_t)j * (K + 1) + k;
                 nfresh_all += hb[b0 + 1] - hb[b0];

@@ -35,7 +35,21 @@ Context::~Context()
     release_all(); // also drops the graphs
     for (hipEvent_t e : kt_ev_) hipEventDestroy(e);
     if (host_) hipHostFree(host_);
+    for (hipEvent_t e : aux_ev_)
+        if (e) hipEventDestroy(e);
+    if (aux_) hipStreamDestroy(aux_);
     if (own_stream_) hipStreamDestroy(own_stream_);
+}
+
+int Context::aux_stream(hipStream_t *out)
+{
+    if (!aux_) {
+        // default priority, like the library's own stream
+        VDB_HIP(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
+        for (hipEvent_t &e : aux_ev_) VDB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    *out = aux_;
+    return kOk;
 }
 
 void Context::arena_begin(void *base, size_t bytes)

@@ -50,6 +50,13 @@ public:
     // Pinned host staging for small device->host reads.
     int host_scratch(size_t bytes, void **out);
 
+    // A second stream of the library (default priority, created on first
+    // use) for work that may run beside the current stream's, with events
+    // (no timing) to order the two: aux_event(i), i < kAuxEvents.
+    static constexpr int kAuxEvents = 3;
+    int aux_stream(hipStream_t *out);
+    hipEvent_t aux_event(int i) const { return aux_ev_[i]; }
+
     // Allocation-free mode (vdb_cluster_noalloc): between arena_begin and
     // arena_end, get() carves named slices of the given device region instead
     // of allocating (same name, same slice while it is large enough; a larger
@@ -108,6 +115,8 @@ private:
     int device_;
     hipStream_t own_stream_ = nullptr;
     hipStream_t stream_ = nullptr;
+    hipStream_t aux_ = nullptr;
+    hipEvent_t aux_ev_[kAuxEvents] = {};
     std::unordered_map<std::string, Buffer> pool_;
     void *host_ = nullptr;
     size_t host_bytes_ = 0;
