@@ -92,6 +92,43 @@ __global__ void __launch_bounds__(kScanBlock)
     if (!tile_off && total && threadIdx.x == 0) *total = tot;
 }
 
+// Up to kScanFewTiles tiles, the second pass needs no scan of the tile sums:
+// each block adds up the sums of the tiles before it (at most that many
+// loads), and the last block writes the total.
+constexpr unsigned kScanFewTiles = 1024;
+template <typename Src, typename Out>
+__global__ void __launch_bounds__(kScanBlock)
+    k_tile_scan_few(Src in, Out *out, size_t n, const Out *sums, Out *total)
+{
+    __shared__ Out sm[kScanBlock / kWave + 1];
+    __shared__ Out red[kScanBlock / kWave];
+    Out part = 0;
+    for (unsigned t = threadIdx.x; t < blockIdx.x; t += kScanBlock) part += sums[t];
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, kWave);
+    if (lane_id() == 0) red[threadIdx.x / kWave] = part;
+    __syncthreads();
+    Out off = 0;
+#pragma unroll
+    for (int w = 0; w < kScanBlock / kWave; w++) off += red[w];
+    size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    Out v[kScanItems];
+    Out s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        v[i] = (base + i < n) ? (Out)in[base + i] : (Out)0;
+        s += v[i];
+    }
+    Out tot;
+    Out run = block_excl_scan<Out, kScanBlock / kWave>(s, sm, &tot) + off;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        if (base + i < n) out[base + i] = run;
+        run += v[i];
+    }
+    if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = off + tot;
+}
+
 template <typename Src, typename Out>
 int scan_impl(Context &c, Src in, Out *out, size_t n, Out *total_dev, int level)
 {
@@ -110,9 +147,15 @@ int scan_impl(Context &c, Src in, Out *out, size_t n, Out *total_dev, int level)
     snprintf(nb, sizeof nb, "scan.L%d.b.%d", level, (int)sizeof(Out));
     Out *sums = nullptr, *offs = nullptr;
     VDB_TRY(c.get(na, tiles, &sums));
-    VDB_TRY(c.get(nb, tiles, &offs));
     k_tile_sums<Src, Out><<<(unsigned)tiles, kScanBlock, 0, c.stream()>>>(in, n, sums);
     VDB_HIP(hipGetLastError());
+    if (tiles <= kScanFewTiles) {
+        k_tile_scan_few<Src, Out><<<(unsigned)tiles, kScanBlock, 0, c.stream()>>>(in, out, n, sums,
+                                                                                  total_dev);
+        VDB_HIP(hipGetLastError());
+        return kOk;
+    }
+    VDB_TRY(c.get(nb, tiles, &offs));
     VDB_TRY((scan_impl<const Out *, Out>(c, sums, offs, tiles, total_dev, level + 1)));
     k_tile_scan<Src, Out><<<(unsigned)tiles, kScanBlock, 0, c.stream()>>>(in, out, n, offs, nullptr);
     VDB_HIP(hipGetLastError());
