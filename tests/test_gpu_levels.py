@@ -62,3 +62,52 @@ def test_batch_levels_standard_rule(aniso20k, levels, minpts):
     for v in range(0, len(eps), 3):
         want = DBSCAN(eps=eps[v], min_samples=int(mp[v]), algorithm="kd_tree").fit(X).labels_
         assert (got[v] == want).all(), (levels, v)
+
+
+# Shapes of the one-kernel label pass: minpts values in groups of four (here
+# 3, 5 and 6 values: partial groups), 8 or 16 level tiles (K <= 8, 9..15),
+# requests that leave (level, minpts) pairs out, and repeated variants (one
+# staged label, several output rows).
+LAYOUTS = {
+    "k6-j6-full": ([0.2, 0.3, 0.4, 0.5, 0.6, 0.7], [3, 4, 6, 9, 12, 20], None),
+    "k9-j3-full": ([0.2, 0.25, 0.3, 0.35, 0.4, 0.5, 0.6, 0.7, 0.8], [4, 9, 15], None),
+    "k8-j5-sparse": ([0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9], [3, 5, 8, 13, 21], 3),
+    "k4-j2-repeats": ([0.25, 0.4, 0.55, 0.7], [4, 10], "repeat"),
+}
+
+
+def _layout(name):
+    E, M, mode = LAYOUTS[name]
+    eps = np.array([e for e in E for m in M])
+    mp = np.array([m for e in E for m in M], dtype=np.uint32)
+    if mode == "repeat":
+        # every variant twice, the copies in reverse order
+        eps = np.concatenate([eps, eps[::-1]])
+        mp = np.concatenate([mp, mp[::-1]])
+    elif mode:
+        keep = np.arange(len(eps)) % mode != 1  # drop every third pair
+        eps, mp = eps[keep], mp[keep]
+    return eps, mp
+
+
+@pytest.mark.parametrize("name", sorted(LAYOUTS))
+@pytest.mark.parametrize("border", [vd.BORDER_REFERENCE, vd.BORDER_STANDARD], ids=["ref", "std"])
+def test_batch_label_layouts(aniso20k, name, border):
+    x, y = aniso20k
+    eps, mp = _layout(name)
+    say("aniso 20k, label layout %s, %d variants, border %d" % (name, len(eps), border))
+    got, st = vd.cluster(x, y, eps, mp, path=vd.PATH_BATCH, border=border, return_stats=True)
+    assert st["path_used"] == vd.PATH_BATCH
+    if border == vd.BORDER_REFERENCE:
+        want = oracle.libvdbscan(x, y, eps, mp)
+        for v in range(len(eps)):
+            assert (got[v] == want[v]).all(), (name, v, float(eps[v]), int(mp[v]))
+        return
+    # standard rule: the oracle is pure Python, so every third variant (and
+    # each distinct one once)
+    seen = {}
+    for v in range(0, len(eps), 3):
+        key = (float(eps[v]), int(mp[v]))
+        if key not in seen:
+            seen[key] = oracle.standard_labels(x, y, key[0], key[1])
+        assert (got[v] == seen[key]).all(), (name, v) + key
