@@ -105,10 +105,13 @@ int run_per_variant(Context &c, Points &P, const double *eps, const uint32_t *mi
         Timer t1(c.stream());
         VDB_TRY(count_neighbors(c, G, I, e, precision, counts, (uint64_t *)rc));
         if (st) st->ms_count += t1.ms();
-        if (verbose)
+        if (verbose) {
+            uint32_t largest = 0;
+            VDB_TRY(grid_max_cell(c, G, &largest));
             printf("\n[vdbscan] eps=%g grid %ux%u (%d cells per eps), %u non-empty cells, "
                    "largest %u points",
-                   e, G.ncx, G.ncy, G.R, G.ncell, G.max_cell);
+                   e, G.ncx, G.ncy, G.R, G.ncell, largest);
+        }
         for (uint32_t v : kv.second) {
             VDB_TRY(cluster_variant(c, P, G, I, e, minpts[v], counts, mode, border, budget,
                                     labels_dev + (size_t)v * P.n, st ? &pt : nullptr));

@@ -61,7 +61,6 @@ struct Grid {
     // that can hold a neighbour at all. Null until classified.
     uint32_t *row_f0 = nullptr;
     uint32_t *row_f1 = nullptr;
-    uint32_t max_cell = 0;          // largest cell population
     uint32_t npts = 0;              // points in the grid
     // per dataset: grid origin, dimensions and first key (keys of dataset d
     // are kbase[d] + cx*ncy[d] + cy)
@@ -88,6 +87,9 @@ __device__ __forceinline__ uint32_t seg_of(const uint32_t *seg, uint32_t nsets, 
 // tag prefixes the pool slot names so several grids can coexist.
 int build_grid(Context &c, const Points &P, double w_min, int R, const char *tag,
                Grid &G, bool *enlarged);
+
+// The largest cell population of G (waits for the stream; verbose output).
+int grid_max_cell(Context &c, const Grid &G, uint32_t *out);
 
 // Classify the window rows of G for the threshold T (see Grid::row_f0).
 int classify_rows(Context &c, Grid &G, double T, const char *tag);

@@ -646,15 +646,9 @@ int build_grid(Context &c, const Points &P, double w_min, int R, const char *tag
     k_cell_tail<<<1, 64, 0, c.stream()>>>(G.cell_start, ncell, n);
     VDB_HIP(hipGetLastError());
 
-    uint32_t *maxpop;
-    VDB_TRY(c.get(slot(tag, "maxpop").c_str(), 1, &maxpop));
-    VDB_HIP(hipMemsetAsync(maxpop, 0, sizeof(uint32_t), c.stream()));
     VDB_TRY(c.get(slot(tag, "cell_box").c_str(), (size_t)ncell * 4, &G.cell_box));
     k_cell_box<<<ceil_div(ncell, kBlock), kBlock, 0, c.stream()>>>(
         G.xs, G.ys, G.cell_start, ncell, G.cell_box);
-    VDB_HIP(hipGetLastError());
-    k_max_pop<<<std::min<unsigned>(ceil_div(ncell, kBlock), 512), kBlock, 0, c.stream()>>>(
-        G.cell_start, ncell, maxpop);
     VDB_HIP(hipGetLastError());
 
     size_t W = 2 * (size_t)R + 1;
@@ -664,12 +658,23 @@ int build_grid(Context &c, const Points &P, double w_min, int R, const char *tag
     k_row_ranges<<<ceil_div(ceil_div(ncell, kRowRun) * W, kBlock), kBlock, 0, c.stream()>>>(
         G.cell_key, ncell, G.sncx, G.sncy, G.skbase, ns, R, G.row_lo, G.row_hi, G.cell_ds);
     VDB_HIP(hipGetLastError());
+    return kOk;
+}
 
+int grid_max_cell(Context &c, const Grid &G, uint32_t *out)
+{
+    uint32_t *maxpop;
+    VDB_TRY(c.get("grid.maxpop", 1, &maxpop));
+    VDB_HIP(hipMemsetAsync(maxpop, 0, sizeof(uint32_t), c.stream()));
+    if (G.ncell)
+        k_max_pop<<<std::min<unsigned>(ceil_div(G.ncell, kBlock), 512), kBlock, 0, c.stream()>>>(
+            G.cell_start, G.ncell, maxpop);
+    VDB_HIP(hipGetLastError());
     void *h = nullptr;
     VDB_TRY(c.host_scratch(sizeof(uint32_t), &h));
     VDB_HIP(hipMemcpyAsync(h, maxpop, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream()));
     VDB_HIP(hipStreamSynchronize(c.stream()));
-    memcpy(&G.max_cell, h, sizeof(uint32_t));
+    memcpy(out, h, sizeof(uint32_t));
     return kOk;
 }
 
