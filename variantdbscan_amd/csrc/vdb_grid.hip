@@ -110,6 +110,31 @@ __global__ void k_seg_extent_final(const double *part, const uint32_t *bbase, ui
         for (int q = 0; q < 4; q++) ext[4 * (size_t)d + q] = m[q];
 }
 
+// Bin parameters of every dataset from its extent (thread per dataset), as
+// src/main.cpp:465-527 computes them: min from DBL_MAX, max from DBL_MIN
+// (the smallest positive double), width (max - min) * 1.01, offset -min for
+// a positive min and |min| otherwise. A zero width (all coordinates equal
+// and positive divides 0 by 0 in the reference, which then aborts binning and
+// reads past its mapping) maps the whole dimension to bin 0.
+__global__ void k_bin_params(const double *ext, uint32_t nsets, BinParams *bps)
+{
+    uint32_t d = blockIdx.x * kBlock + threadIdx.x;
+    if (d >= nsets) return;
+    const double *e = ext + 4 * (size_t)d;
+    double min_x = e[0] < DBL_MAX ? e[0] : DBL_MAX;
+    double max_x = e[1] > DBL_MIN ? e[1] : DBL_MIN;
+    double min_y = e[2] < DBL_MAX ? e[2] : DBL_MAX;
+    double max_y = e[3] > DBL_MIN ? e[3] : DBL_MIN;
+    BinParams bp;
+    bp.xw = __dmul_rn(__dsub_rn(max_x, min_x), 1.01);
+    bp.yw = __dmul_rn(__dsub_rn(max_y, min_y), 1.01);
+    bp.xoff = (min_x > 0) ? -min_x : fabs(min_x);
+    bp.yoff = (min_y > 0) ? -min_y : fabs(min_y);
+    bp.xdeg = !(bp.xw > 0.0);
+    bp.ydeg = !(bp.yw > 0.0);
+    bps[d] = bp;
+}
+
 __global__ void k_bin_keys(const double *x, const double *y, uint32_t n,
                            const BinParams *bps, const uint32_t *seg, uint32_t nsets,
                            uint32_t *keys, uint32_t *vals, uint32_t *bad)
@@ -414,53 +439,12 @@ int build_rank_order(Context &c, Points &P, const char *tag)
     k_seg_extent_final<<<ceil_div(ns, kBlock / kWave), kBlock, 0, c.stream()>>>(part, bbase, ns,
                                                                                ext);
     VDB_HIP(hipGetLastError());
-    P.hext.resize((size_t)ns * 4);
-    uint32_t hnf = 0;
-    VDB_HIP(hipMemcpyAsync(P.hext.data(), ext, sizeof(double) * 4 * ns, hipMemcpyDeviceToHost,
-                           c.stream()));
-    VDB_HIP(hipMemcpyAsync(&hnf, nonfin, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream()));
-    VDB_HIP(hipStreamSynchronize(c.stream()));
-    if (hnf) {
-        set_error("non-finite coordinate in input");
-        return kErrNonFinite;
-    }
-    P.ext[0] = DBL_MAX;
-    P.ext[1] = -DBL_MAX;
-    P.ext[2] = DBL_MAX;
-    P.ext[3] = -DBL_MAX;
-    std::vector<BinParams> bps(ns);
-    for (uint32_t d = 0; d < ns; d++) {
-        const double *e = &P.hext[4 * (size_t)d];
-        if (P.hseg[d + 1] > P.hseg[d]) {
-            P.ext[0] = std::min(P.ext[0], e[0]);
-            P.ext[1] = std::max(P.ext[1], e[1]);
-            P.ext[2] = std::min(P.ext[2], e[2]);
-            P.ext[3] = std::max(P.ext[3], e[3]);
-        }
-        // src/main.cpp:465-492: min from DBL_MAX, max from DBL_MIN (smallest
-        // positive double), width = (max - min) * 1.01
-        double min_x = e[0] < DBL_MAX ? e[0] : DBL_MAX;
-        double max_x = e[1] > DBL_MIN ? e[1] : DBL_MIN;
-        double min_y = e[2] < DBL_MAX ? e[2] : DBL_MAX;
-        double max_y = e[3] > DBL_MIN ? e[3] : DBL_MIN;
-        BinParams &bp = bps[d];
-        bp.xw = (max_x - min_x) * 1.01;
-        bp.yw = (max_y - min_y) * 1.01;
-        bp.xoff = (min_x > 0) ? -min_x : std::fabs(min_x); // :513-527
-        bp.yoff = (min_y > 0) ? -min_y : std::fabs(min_y);
-        // zero width (all coordinates equal and positive) divides 0 by 0 in the
-        // reference, which then aborts binning and reads past its mapping; here
-        // the whole dimension maps to bin 0
-        bp.xdeg = !(bp.xw > 0.0);
-        bp.ydeg = !(bp.yw > 0.0);
-    }
-    if (P.ext[0] > P.ext[1]) {
-        P.ext[0] = P.ext[1] = P.ext[2] = P.ext[3] = 0.0;
-    }
+    // bin parameters on the device: the extent is read back only once, with
+    // the non-finite count and the bin check, after the sort
     BinParams *dbp;
     VDB_TRY(c.get(slot(tag, "binparams").c_str(), ns, &dbp));
-    VDB_HIP(hipMemcpyAsync(dbp, bps.data(), sizeof(BinParams) * ns, hipMemcpyHostToDevice,
-                           c.stream()));
+    k_bin_params<<<ceil_div(ns, kBlock), kBlock, 0, c.stream()>>>(ext, ns, dbp);
+    VDB_HIP(hipGetLastError());
 
     uint32_t *keys, *vals, *tk, *tv, *bad;
     VDB_TRY(c.get(slot(tag, "bk").c_str(), n, &keys));
@@ -482,15 +466,42 @@ int build_rank_order(Context &c, Points &P, const char *tag)
     VDB_TRY(c.get(slot(tag, "yb").c_str(), n, &P.yb));
     k_gather_xy<<<g, kBlock, 0, c.stream()>>>(P.x, P.y, P.mapping, n, P.xb, P.yb);
     VDB_HIP(hipGetLastError());
-    uint32_t hb = 0;
-    void *h = nullptr;
-    VDB_TRY(c.host_scratch(sizeof(uint32_t), &h));
-    VDB_HIP(hipMemcpyAsync(h, bad, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream()));
+    // one read back: extents, non-finite count, bin check (a non-finite
+    // coordinate or an overflowing bin only made the keys meaningless: every
+    // access above stays in bounds)
+    char *h = nullptr;
+    VDB_TRY(c.host_scratch(32 * (size_t)ns + 8, (void **)&h));
+    VDB_HIP(hipMemcpyAsync(h, ext, sizeof(double) * 4 * ns, hipMemcpyDeviceToHost, c.stream()));
+    VDB_HIP(hipMemcpyAsync(h + 32 * (size_t)ns, nonfin, 4, hipMemcpyDeviceToHost, c.stream()));
+    VDB_HIP(hipMemcpyAsync(h + 32 * (size_t)ns + 4, bad, 4, hipMemcpyDeviceToHost, c.stream()));
     VDB_HIP(hipStreamSynchronize(c.stream()));
-    memcpy(&hb, h, sizeof(uint32_t));
+    uint32_t hnf = 0, hb = 0;
+    memcpy(&hnf, h + 32 * (size_t)ns, 4);
+    memcpy(&hb, h + 32 * (size_t)ns + 4, 4);
+    if (hnf) {
+        set_error("non-finite coordinate in input");
+        return kErrNonFinite;
+    }
     if (hb) {
         set_error("bins out of range (coordinate arithmetic overflowed)");
         return kErrUnsupported;
+    }
+    P.hext.assign((const double *)h, (const double *)h + 4 * (size_t)ns);
+    P.ext[0] = DBL_MAX;
+    P.ext[1] = -DBL_MAX;
+    P.ext[2] = DBL_MAX;
+    P.ext[3] = -DBL_MAX;
+    for (uint32_t d = 0; d < ns; d++) {
+        const double *e = &P.hext[4 * (size_t)d];
+        if (P.hseg[d + 1] > P.hseg[d]) {
+            P.ext[0] = std::min(P.ext[0], e[0]);
+            P.ext[1] = std::max(P.ext[1], e[1]);
+            P.ext[2] = std::min(P.ext[2], e[2]);
+            P.ext[3] = std::max(P.ext[3], e[3]);
+        }
+    }
+    if (P.ext[0] > P.ext[1]) {
+        P.ext[0] = P.ext[1] = P.ext[2] = P.ext[3] = 0.0;
     }
     return kOk;
 }
