@@ -18,22 +18,24 @@ using namespace vdb;
 
 namespace {
 
+// Phase timer for vdb_stats: events only when stats are asked for.
 struct Timer {
     hipEvent_t a = nullptr, b = nullptr;
     hipStream_t s;
-    explicit Timer(hipStream_t st) : s(st)
+    Timer(hipStream_t st, bool on) : s(st)
     {
-        hipEventCreate(&a);
-        hipEventCreate(&b);
+        if (!on) return;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
         hipEventRecord(a, s);
     }
     ~Timer()
     {
-        hipEventDestroy(a);
-        hipEventDestroy(b);
+        if (a) hipEventDestroy(a);
+        if (b) hipEventDestroy(b);
     }
     double ms()
     {
+        if (!a || !b) return 0.0;
         hipEventRecord(b, s);
         hipEventSynchronize(b);
         float m = 0;
@@ -96,13 +98,13 @@ int run_per_variant(Context &c, Points &P, const double *eps, const uint32_t *mi
         double e = eps[kv.second[0]];
         Grid G;
         Items I;
-        Timer t0(c.stream());
+        Timer t0(c.stream(), st != nullptr);
         int k = 0;
         VDB_TRY(tune_cells_per_eps(c, P, e, precision, cells, &k));
         VDB_TRY(build_eps_grid(c, P, e, k, "g1", G));
         VDB_TRY(build_items(c, G, "g1", I));
         if (st) st->ms_prepare += t0.ms();
-        Timer t1(c.stream());
+        Timer t1(c.stream(), st != nullptr);
         VDB_TRY(count_neighbors(c, G, I, e, precision, counts, (uint64_t *)rc));
         if (st) st->ms_count += t1.ms();
         if (verbose) {
@@ -147,7 +149,7 @@ int cluster_points(Context &c, Points &P, const char *tag, const double *eps,
     int precision = opt ? opt->precision : VDB_PREC_FP64;
     int border = opt ? opt->border : VDB_BORDER_REFERENCE;
     bool verbose = opt && opt->verbose;
-    Timer tprep(c.stream());
+    Timer tprep(c.stream(), st != nullptr);
     VDB_TRY(build_rank_order(c, P, tag));
     if (st) st->ms_prepare += tprep.ms();
     int used = path;
@@ -270,7 +272,7 @@ int cluster_impl(const double *x, const double *y, uint32_t n, const double *eps
     VDB_TRY(get_context(&cp));
     Context &c = *cp;
     std::lock_guard<std::mutex> lock(c.mu);
-    Timer total(c.stream());
+    Timer total(c.stream(), st != nullptr);
     Points P;
     VDB_TRY(upload_points(c, x, y, n, flags, P));
     uint32_t *out = labels;
@@ -408,7 +410,7 @@ int vdb_cluster_multi(const double *x, const double *y, const uint64_t *offsets,
     VDB_TRY(get_context(&cp));
     Context &c = *cp;
     std::lock_guard<std::mutex> lock(c.mu);
-    Timer total(c.stream());
+    Timer total(c.stream(), st != nullptr);
     Points all;
     VDB_TRY(upload_points(c, x, y, (uint32_t)ntot, flags, all));
     uint32_t *out = labels;
@@ -531,10 +533,10 @@ int vdb_neighbor_table(const double *x, const double *y, uint32_t n, double eps,
     VDB_TRY(get_context(&cp));
     Context &c = *cp;
     std::lock_guard<std::mutex> lock(c.mu);
-    Timer ttot(c.stream());
+    Timer ttot(c.stream(), st != nullptr);
     Points P;
     VDB_TRY(upload_points(c, x, y, n, flags, P));
-    Timer tprep(c.stream());
+    Timer tprep(c.stream(), st != nullptr);
     VDB_TRY(build_rank_order(c, P, "pts"));
     Grid G;
     Items I;
