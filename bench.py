@@ -64,7 +64,8 @@ def fill_bytes(n, pairs):
 
 # batched-engine kernels timed for the headline roofline
 BATCH_KERNELS = ("k_knn", "k_sweep", "k_test", "k_labels_all", "k_rows_all", "k_labels_j",
-                 "k_rows_from_pos", "k_roots_min", "k_pair_elig", "k_pairs", "k_pair_count")
+                 "k_rows_from_pos", "k_roots_min", "k_pair_elig", "k_pair_scatter", "k_pairs",
+                 "k_pair_count")
 
 
 def batch_kernel_bytes(name, info, W):
@@ -89,14 +90,17 @@ def batch_kernel_bytes(name, info, W):
       k_rows_from_pos per point: its position (4), dataset bounds (8), its
                     8 staged labels (32) and 8 output labels (32)
       k_pair_elig   per cell pair: cells and levels (10), both cells' core
-                    levels for each minpts value of the pass (2 g), key and
-                    value out (8 g), g = min(J, 8)
+                    levels for each minpts value (2 J), a level byte per
+                    minpts value out (8 per plane of 8 values)
+      k_pair_scatter per cell pair: cells and levels (10) and its level
+                    bytes (8 per plane); per record written (a pair entering
+                    at its level) 12
       k_pairs       per cell: its window rows (8 W) and box (32); per stored
                     pair 10 bytes
       k_pair_count  per cell: window rows (8 W), box (32), count (4)"""
     n, J, L = info["points"], info["minpts_values"], info["list_len"]
     nl, nc, npairs = info["listed_points"], info["cells"], info["cell_pairs"]
-    g = min(J, 8)
+    planes = -(-J // 8)
     V = info.get("snapshots") or info["levels"] * J  # one per requested (level, minpts)
     fr, ca, te = info.get("fresh_pairs", 0), info.get("carried_pairs", 0), info.get("test_records", 0)
     return {
@@ -106,7 +110,8 @@ def batch_kernel_bytes(name, info, W):
         "k_knn": 16 * n + nl * (8 + 12 * L) + nc * (44 + 8 * W),
         "k_labels_j": 77 * n * -(-V // 8),
         "k_rows_from_pos": 76 * n * -(-V // 8),
-        "k_pair_elig": npairs * (10 + 10 * g),
+        "k_pair_elig": npairs * (10 + 2 * J + 8 * planes),
+        "k_pair_scatter": (npairs * (10 + 8 * planes) + 12 * fr) if fr else None,
         "k_pairs": nc * (8 * W + 32) + npairs * 10,
         "k_pair_count": nc * (8 * W + 36),
     }.get(name)
@@ -326,20 +331,23 @@ def main():
         log("batch: timing %d steps" % args.steps)
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
-        phases = []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ev0.record(stream)
+        # the drop-in call as a caller makes it: no phase statistics (their
+        # event reads add host waits)
         for _ in range(args.steps):
-            st = vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, eps, mp, labels.data_ptr(),
-                                   path=vd.PATH_BATCH)
-            phases.append(st)
+            vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, eps, mp, labels.data_ptr(),
+                              path=vd.PATH_BATCH, stats=False)
         ev1.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         ms_step = ev0.elapsed_time(ev1) / args.steps
-        assert all(p["path_used"] == vd.PATH_BATCH for p in phases)
         value = V / (ms_step / 1e3)
+        # phase times and the engine used, from calls made after the timed ones
+        phases = [vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, eps, mp, labels.data_ptr(),
+                                    path=vd.PATH_BATCH) for _ in range(3)]
+        assert all(p["path_used"] == vd.PATH_BATCH for p in phases)
         avg = {k: float(np.mean([p[k] for p in phases])) for k in
                ("ms_total", "ms_prepare", "ms_count", "ms_uf", "ms_labels")}
         out.update(value=value, ms_per_step=ms_step, wall_s=wall, batch_phases_ms=avg)
@@ -415,7 +423,7 @@ def main():
             sa.record(stream)
             for _ in range(args.steps):
                 vd.cluster_device(dx.data_ptr(), dy.data_ptr(), n, eps, mp, labels.data_ptr(),
-                                  path=vd.PATH_BATCH, border=vd.BORDER_STANDARD)
+                                  path=vd.PATH_BATCH, border=vd.BORDER_STANDARD, stats=False)
             sb.record(stream)
             torch.cuda.synchronize()
             ms_std = sa.elapsed_time(sb) / args.steps

@@ -1,8 +1,11 @@
 """Batched engine at the boundaries of its pair sort. The cell pairs of all
-minpts values of a pass are sorted by (minpts value, eps level): J*(K+1)
-buckets. Up to 16 buckets the one-pass bucket sort uses 4-bit keys, up to
-128 7-bit keys, beyond that the radix sort runs. Every case must give the
-oracle's labels (reference rule) and scikit-learn's (standard rule)."""
+minpts values of a pass are ordered by (minpts value, eps level): J*(K+1)
+buckets. Up to 128 buckets the eligibility pass counts each tile's pairs per
+bucket and one scatter writes the records (levels matched with 4-bit keys
+up to 16 levels, 7-bit keys beyond; more than 8 minpts values take two
+eligibility planes); beyond 128 buckets the radix sort runs. Every case
+must give the oracle's labels (reference rule) and scikit-learn's
+(standard rule)."""
 import sys
 
 import numpy as np
@@ -26,9 +29,11 @@ def aniso20k():
     return datasets.aniso(20_000, seed=31)
 
 
-# (eps levels K, minpts values): buckets J*(K+1) = 16, 32, 128, 132
-CASES = [(15, (4,)), (15, (4, 9)), (31, (3, 4, 9, 20)), (32, (3, 4, 9, 20))]
-IDS = ["16-buckets", "32-buckets", "128-buckets", "132-radix"]
+# (eps levels K, minpts values): buckets J*(K+1) = 16, 32, 128, 128 (16
+# minpts values: two planes), 132
+CASES = [(15, (4,)), (15, (4, 9)), (31, (3, 4, 9, 20)),
+         (7, (3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 16, 18, 20, 24, 28, 32)), (32, (3, 4, 9, 20))]
+IDS = ["16-buckets", "32-buckets", "128-buckets", "128-buckets-16-minpts", "132-radix"]
 
 
 def _grid(levels, minpts):

@@ -827,78 +827,240 @@ __device__ bool small_pair_connected(const LevelArgs &a, uint32_t A, uint32_t B)
 
 // Eligibility level of every pair for the minpts values j0 .. j0+nj-1
 // (nj <= kEligGroup): the first level at which both cells hold a core point
-// and some pair of their boxes is within eps. K marks pairs that never
-// become eligible. Output g of the group goes to key/val + g*npairs, the key
-// being (jrel0 + g)*(K+1) + level: the pass's sort key over minpts value and
-// level.
-// For pairs of small cells (at most kSmallPairWork point pairs) the level is
-// exact instead: the first level at which some core-core pair is within eps,
+// and some pair of their boxes is within eps; K marks pairs that never
+// become eligible. For pairs of small cells (at most kSmallPairWork point
+// pairs) the level is exact instead: the first level at which some
+// core-core pair is within eps,
 //   min over (p, q) of max(level(s(p,q)), kc_j(p), kc_j(q)),
-// and the value carries bit 31 so the sweep unites them without a test. The
-// point distances are computed once for the whole group.
+// flagged so that the sweep unites them without a test. The point distances
+// are computed once for the whole group.
+//
+// Output: byte g of lev[i] = level | exact << 7 (levels stay below 128),
+// bytes of minpts values beyond the group K. With `hist`, the kernel also
+// counts its tile's pairs per (minpts value, level) bucket for the record
+// scatter: tile t = the kBlock consecutive pairs of block t (a pair per
+// thread: the eligibility tests are latency-bound, so every pair gets its
+// own lane), bucket (jrel0 + g)*(K+1) + level, hist[bucket * tiles + t]
+// (bucket-major, so that an exclusive scan gives every tile's first slot in
+// every bucket); the buckets of level K stay empty (those pairs get no
+// record).
 constexpr int kEligGroup = 8;
-__global__ void k_pair_elig(const uint32_t *pa, const uint32_t *pb, const uint16_t *pk,
-                            const uint8_t *cellkc_cm, const uint32_t *cell_start,
-                            const double *xs, const double *ys, const uint8_t *kc,
-                            const double *Tk, int J, int j0, int nj, int K, uint32_t npairs,
-                            uint32_t jrel0, uint32_t *key, uint32_t *val)
+constexpr uint32_t kEligTile = kBlock;
+// the scatter takes kPairItems eligibility tiles per block
+constexpr int kPairItems = 8;
+constexpr uint32_t kPairTile = kEligTile * kPairItems;
+constexpr int kPairBuckets = 128; // buckets of a pass for the record scatter
+
+template <int BITS>
+__global__ void __launch_bounds__(kBlock)
+    k_pair_elig(const uint32_t *pa, const uint32_t *pb, const uint16_t *pk,
+                const uint8_t *cellkc_cm, const uint32_t *cell_start, const double *xs,
+                const double *ys, const uint8_t *kc, const double *Tk, int J, int j0, int nj,
+                int K, uint32_t npairs, uint32_t jrel0, uint2 *lev, uint32_t *hist,
+                unsigned tiles)
+{
+    __shared__ uint32_t h[kWPB][kPairBuckets];
+    if (hist) {
+        for (int r = threadIdx.x; r < kWPB * kPairBuckets; r += kBlock) (&h[0][0])[r] = 0;
+        __syncthreads();
+    }
+    const int w = threadIdx.x / kWave;
+    {
+        uint32_t i = blockIdx.x * kEligTile + threadIdx.x;
+        bool valid = i < npairs;
+        int e[kEligGroup];
+#pragma unroll
+        for (int g = 0; g < kEligGroup; g++) e[g] = K;
+        uint32_t exact = 0; // bit g: level of j0+g computed exactly
+        if (valid) {
+            uint32_t A = pa[i], B = pb[i];
+            int e0 = pk[i] & 0xff;
+            uint32_t a0 = cell_start[A], a1 = cell_start[A + 1];
+            uint32_t b0 = cell_start[B], b1 = cell_start[B + 1];
+            bool small = (uint64_t)(a1 - a0) * (b1 - b0) <= kSmallPairWork;
+#pragma unroll
+            for (int g = 0; g < kEligGroup; g++) {
+                if (g < nj) {
+                    // a cell's levels for all minpts values lie side by side
+                    int v = e0;
+                    int ca = cellkc_cm[(size_t)A * J + j0 + g];
+                    int cb = cellkc_cm[(size_t)B * J + j0 + g];
+                    v = v > ca ? v : ca;
+                    v = v > cb ? v : cb;
+                    e[g] = v < K ? v : K;
+                    if (small && v < K) exact |= 1u << g;
+                }
+            }
+            if (exact) {
+                int best[kEligGroup];
+#pragma unroll
+                for (int g = 0; g < kEligGroup; g++) best[g] = K;
+                for (uint32_t p = a0; p < a1; p++) {
+                    double px = xs[p], py = ys[p];
+                    const uint8_t *kp = kc + (size_t)p * J + j0;
+                    for (uint32_t q = b0; q < b1; q++) {
+                        double sq = dist2_exact(px, py, xs[q], ys[q]);
+                        int ls = 0; // first level whose eps admits the pair
+                        while (ls < K && sq > Tk[ls]) ls++;
+                        if (ls >= K) continue;
+                        const uint8_t *kq = kc + (size_t)q * J + j0;
+#pragma unroll
+                        for (int g = 0; g < kEligGroup; g++) {
+                            if (!((exact >> g) & 1u)) continue;
+                            int lv = ls, x = kp[g], y = kq[g];
+                            lv = lv > x ? lv : x;
+                            lv = lv > y ? lv : y;
+                            best[g] = lv < best[g] ? lv : best[g];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int g = 0; g < kEligGroup; g++)
+                    if ((exact >> g) & 1u) e[g] = best[g];
+            }
+            uint32_t lo = 0, hi = 0;
+#pragma unroll
+            for (int g = 0; g < kEligGroup; g++) {
+                uint32_t byte = (uint32_t)e[g] | (((exact >> g) & 1u) << 7);
+                if (g < 4) lo |= byte << (8 * g);
+                else hi |= byte << (8 * (g - 4));
+            }
+            lev[i] = make_uint2(lo, hi);
+        }
+        if (hist) {
+            // one lane per (bucket, wave) adds the wave's count to the
+            // wave's own row: no two lanes write one word
+#pragma unroll
+            for (int g = 0; g < kEligGroup; g++) {
+                if (g >= nj) break;
+                bool take = valid && e[g] < K;
+                uint32_t d = take ? (uint32_t)e[g] : (1u << BITS);
+                uint64_t peers = bucket_peers<BITS>(d);
+                if (take && mask_rank(peers) == 0)
+                    h[w][g * (K + 1) + e[g]] += (uint32_t)popc64(peers);
+            }
+        }
+    }
+    if (hist) {
+        __syncthreads();
+        const int rows = nj * (K + 1);
+        for (int r = threadIdx.x; r < rows; r += kBlock) {
+            uint32_t s = 0;
+#pragma unroll
+            for (int q = 0; q < kWPB; q++) s += h[q][r];
+            hist[((size_t)jrel0 * (K + 1) + r) * tiles + blockIdx.x] = s;
+        }
+    }
+}
+
+// The pair records of a pass in (minpts value, level) order, stable: block
+// t takes the kPairItems eligibility tiles from tile kPairItems*t on, whose
+// records of bucket d go from offs[d*etiles + kPairItems*t] on (the
+// exclusive scan of hist over etiles tiles), in pair order. Pairs that never
+// become eligible get no record. Record: (a, b, levels | exact << 16). lev
+// holds ngroups = ceil(nj / kEligGroup) planes of npairs words.
+template <int BITS>
+__global__ void __launch_bounds__(kBlock)
+    k_pair_scatter(const uint32_t *pa, const uint32_t *pb, const uint16_t *pk, const uint2 *lev,
+                   uint32_t npairs, int nj, int K, const uint32_t *offs, unsigned etiles,
+                   PairRec *out)
+{
+    __shared__ uint32_t base_d[kPairBuckets];
+    __shared__ uint32_t wcnt[kWPB][kPairBuckets];
+    const int nb = nj * (K + 1);
+    for (int d = threadIdx.x; d < nb; d += kBlock)
+        base_d[d] = offs[(size_t)d * etiles + (size_t)blockIdx.x * kPairItems];
+    for (int r = threadIdx.x; r < kWPB * kPairBuckets; r += kBlock) (&wcnt[0][0])[r] = 0;
+    __syncthreads();
+    const int w = threadIdx.x / kWave;
+    const int ngroups = (nj + kEligGroup - 1) / kEligGroup;
+    for (int it = 0; it < kPairItems; it++) {
+        uint32_t i = blockIdx.x * kPairTile + (uint32_t)it * kBlock + threadIdx.x;
+        bool valid = i < npairs;
+        uint32_t A = 0, B = 0, C = 0;
+        if (valid) {
+            A = pa[i];
+            B = pb[i];
+            C = pk[i];
+        }
+        for (int q = 0; q < ngroups; q++) {
+            const int gn = nj - q * kEligGroup < kEligGroup ? nj - q * kEligGroup : kEligGroup;
+            const int r0 = q * kEligGroup * (K + 1); // first bucket of the group
+            uint2 word = valid ? lev[(size_t)q * npairs + i] : make_uint2(0u, 0u);
+            uint32_t rank[kEligGroup];
+            uint32_t bytes[kEligGroup];
+#pragma unroll
+            for (int g = 0; g < kEligGroup; g++) {
+                bytes[g] = ((g < 4 ? word.x : word.y) >> (8 * (g & 3))) & 0xffu;
+                rank[g] = 0;
+                if (g >= gn) continue;
+                uint32_t l = bytes[g] & 0x7fu;
+                bool take = valid && l < (uint32_t)K;
+                uint64_t peers = bucket_peers<BITS>(take ? l : (1u << BITS));
+                rank[g] = (uint32_t)mask_rank(peers);
+                if (take && rank[g] == 0) wcnt[w][r0 + g * (K + 1) + (int)l] = (uint32_t)popc64(peers);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int g = 0; g < kEligGroup; g++) {
+                if (g >= gn) continue;
+                uint32_t l = bytes[g] & 0x7fu;
+                if (!valid || l >= (uint32_t)K) continue;
+                int d = r0 + g * (K + 1) + (int)l;
+                uint32_t pos = base_d[d] + rank[g];
+                for (int u = 0; u < w; u++) pos += wcnt[u][d];
+                out[pos] = PairRec{A, B, C | ((bytes[g] >> 7) << 16)};
+            }
+            __syncthreads();
+            const int r1 = r0 + gn * (K + 1);
+            for (int d = r0 + threadIdx.x; d < r1; d += kBlock) {
+                uint32_t s = 0;
+#pragma unroll
+                for (int u = 0; u < kWPB; u++) {
+                    s += wcnt[u][d];
+                    wcnt[u][d] = 0;
+                }
+                base_d[d] += s;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// bucket starts of the scattered records: bstart[d] = the slot of tile 0 in
+// bucket d, bstart[nb] = the number of records
+__global__ void k_pair_bounds(const uint32_t *offs, const uint32_t *hist, unsigned tiles, int nb,
+                              uint32_t *bstart)
+{
+    for (int d = threadIdx.x; d <= nb; d += kBlock) {
+        if (d < nb) {
+            bstart[d] = offs[(size_t)d * tiles];
+        } else {
+            size_t last = (size_t)nb * tiles - 1;
+            bstart[d] = offs[last] + hist[last];
+        }
+    }
+}
+
+// Passes of more than kPairBuckets buckets sort by radix instead: key
+// (jrel0 + g)*(K+1) + level and value i | exact << 31 for every pair and
+// minpts value g of plane q (pairs that never become eligible sort last in
+// their minpts value's range).
+__global__ void k_level_keys(const uint2 *lev, uint32_t npairs, int nj, int K, uint32_t *key,
+                             uint32_t *val)
 {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= npairs) return;
-    uint32_t A = pa[i], B = pb[i];
-    int e0 = pk[i] & 0xff;
-    uint32_t a0 = cell_start[A], a1 = cell_start[A + 1];
-    uint32_t b0 = cell_start[B], b1 = cell_start[B + 1];
-    int e[kEligGroup];
-    uint32_t exact = 0; // bit g: level of j0+g computed exactly
-    bool small = (uint64_t)(a1 - a0) * (b1 - b0) <= kSmallPairWork;
-#pragma unroll
-    for (int g = 0; g < kEligGroup; g++) {
-        e[g] = K;
-        if (g < nj) {
-            // a cell's levels for all minpts values lie side by side
-            int v = e0;
-            int ca = cellkc_cm[(size_t)A * J + j0 + g], cb = cellkc_cm[(size_t)B * J + j0 + g];
-            v = v > ca ? v : ca;
-            v = v > cb ? v : cb;
-            e[g] = v;
-            if (small && v < K) exact |= 1u << g;
+    const int ngroups = (nj + kEligGroup - 1) / kEligGroup;
+    for (int q = 0; q < ngroups; q++) {
+        uint2 word = lev[(size_t)q * npairs + i];
+        for (int g = 0; g < kEligGroup; g++) {
+            int j = q * kEligGroup + g;
+            if (j >= nj) break;
+            uint32_t byte = ((g < 4 ? word.x : word.y) >> (8 * (g & 3))) & 0xffu;
+            key[(size_t)j * npairs + i] = (uint32_t)j * (uint32_t)(K + 1) + (byte & 0x7fu);
+            val[(size_t)j * npairs + i] = i | ((byte >> 7) << 31);
         }
-    }
-    if (exact) {
-        int best[kEligGroup];
-#pragma unroll
-        for (int g = 0; g < kEligGroup; g++) best[g] = K;
-        for (uint32_t p = a0; p < a1; p++) {
-            double px = xs[p], py = ys[p];
-            const uint8_t *kp = kc + (size_t)p * J + j0;
-            for (uint32_t q = b0; q < b1; q++) {
-                double sq = dist2_exact(px, py, xs[q], ys[q]);
-                int ls = 0; // first level whose eps admits the pair
-                while (ls < K && sq > Tk[ls]) ls++;
-                if (ls >= K) continue;
-                const uint8_t *kq = kc + (size_t)q * J + j0;
-#pragma unroll
-                for (int g = 0; g < kEligGroup; g++) {
-                    if (!((exact >> g) & 1u)) continue;
-                    int lv = ls, x = kp[g], y = kq[g];
-                    lv = lv > x ? lv : x;
-                    lv = lv > y ? lv : y;
-                    best[g] = lv < best[g] ? lv : best[g];
-                }
-            }
-        }
-#pragma unroll
-        for (int g = 0; g < kEligGroup; g++)
-            if ((exact >> g) & 1u) e[g] = best[g];
-    }
-#pragma unroll
-    for (int g = 0; g < kEligGroup; g++) {
-        if (g >= nj) break;
-        // one sort key for all minpts values of the pass: (j, level)
-        key[(size_t)g * npairs + i] =
-            (jrel0 + (uint32_t)g) * (uint32_t)(K + 1) + (uint32_t)(e[g] < K ? e[g] : K);
-        val[(size_t)g * npairs + i] = i | (((exact >> g) & 1u) ? 0x80000000u : 0u);
     }
 }
 
@@ -2081,11 +2243,22 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     VDB_TRY(c.get("b.tests", (size_t)jc * tcap, &tests));
     VDB_TRY(c.get("b.tests2", (size_t)jc * tcap, &tests2));
     VDB_TRY(c.get("b.cnts", (size_t)jc * ncnt, &cnts));
-    VDB_TRY(c.get("b.ekey", (size_t)jc * psz, &ekey));
-    VDB_TRY(c.get("b.eval", (size_t)jc * psz, &eval));
-    // radix sort buffers: only passes of more than 128 (minpts, level) buckets
-    stk = stv = nullptr;
-    if ((size_t)jc * (K + 1) > 128) {
+    // eligibility levels: a byte per (pair, minpts value), planes of 8
+    const size_t nplanes = ((size_t)jc + kEligGroup - 1) / kEligGroup;
+    uint2 *plev;
+    VDB_TRY(c.get("b.lev", nplanes * psz * 2, (uint32_t **)&plev));
+    // passes of up to kPairBuckets (minpts, level) buckets scatter the
+    // records by tile counts; larger ones radix-sort keys and values
+    const bool pair_scatter = (size_t)jc * (K + 1) <= (size_t)kPairBuckets;
+    const unsigned ptiles = ceil_div(std::max<size_t>(psz, 1), kEligTile);
+    uint32_t *phist = nullptr, *poffs = nullptr;
+    ekey = eval = stk = stv = nullptr;
+    if (pair_scatter) {
+        VDB_TRY(c.get("b.phist", (size_t)jc * (K + 1) * ptiles, &phist));
+        VDB_TRY(c.get("b.poffs", (size_t)jc * (K + 1) * ptiles, &poffs));
+    } else {
+        VDB_TRY(c.get("b.ekey", (size_t)jc * psz, &ekey));
+        VDB_TRY(c.get("b.eval", (size_t)jc * psz, &eval));
         VDB_TRY(c.get("b.stk", (size_t)jc * psz, &stk));
         VDB_TRY(c.get("b.stv", (size_t)jc * psz, &stv));
     }
@@ -2190,22 +2363,45 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
             const uint32_t nb = (uint32_t)nj * (uint32_t)(K + 1);
 
             if (npairs) {
+                // eligibility tiles of this pass (npairs <= psz)
+                const unsigned tiles = ceil_div(npairs, kEligTile);
+                uint32_t *hist = pair_scatter ? phist : nullptr;
                 for (int g0 = 0; g0 < nj; g0 += kEligGroup) {
                     int gn = std::min(kEligGroup, nj - g0);
+                    uint2 *plane = plev + (size_t)(g0 / kEligGroup) * npairs;
                     VDB_TRY(c.kt_begin("k_pair_elig"));
-                    k_pair_elig<<<ceil_div(npairs, kBlock), kBlock, 0, c.stream()>>>(
-                        pa, pb, pk, cellkc_cm, G.cell_start, G.xs, G.ys, kc, dTk, J, j0 + g0, gn, K,
-                        npairs, (uint32_t)g0, ekey + (size_t)g0 * npairs, eval + (size_t)g0 * npairs);
+                    if (K <= 16)
+                        k_pair_elig<4><<<tiles, kBlock, 0, c.stream()>>>(
+                            pa, pb, pk, cellkc_cm, G.cell_start, G.xs, G.ys, kc, dTk, J, j0 + g0,
+                            gn, K, npairs, (uint32_t)g0, plane, hist, tiles);
+                    else
+                        k_pair_elig<7><<<tiles, kBlock, 0, c.stream()>>>(
+                            pa, pb, pk, cellkc_cm, G.cell_start, G.xs, G.ys, kc, dTk, J, j0 + g0,
+                            gn, K, npairs, (uint32_t)g0, plane, hist, tiles);
                     VDB_TRY(c.kt_end("k_pair_elig"));
                     VDB_HIP(hipGetLastError());
                 }
-                uint32_t nkeys = (uint32_t)nj * npairs;
-                if (nb <= 128) {
-                    // one counting pass, which also gives the bucket starts,
-                    // writing the pair records in key order
-                    VDB_TRY(bucket_sort_records(c, ekey, eval, pa, pb, pk, (uint32_t *)frec, nkeys,
-                                                (int)nb, bstart));
+                if (pair_scatter) {
+                    // tile counts -> every tile's first slot in every bucket,
+                    // then the records in (minpts value, level) order
+                    VDB_TRY(scan_exclusive_u32(c, phist, poffs, (size_t)nb * tiles, nullptr));
+                    VDB_TRY(c.kt_begin("k_pair_scatter"));
+                    if (K <= 16)
+                        k_pair_scatter<4><<<ceil_div(npairs, kPairTile), kBlock, 0, c.stream()>>>(
+                            pa, pb, pk, plev, npairs, nj, K, poffs, tiles, frec);
+                    else
+                        k_pair_scatter<7><<<ceil_div(npairs, kPairTile), kBlock, 0, c.stream()>>>(
+                            pa, pb, pk, plev, npairs, nj, K, poffs, tiles, frec);
+                    VDB_TRY(c.kt_end("k_pair_scatter"));
+                    VDB_HIP(hipGetLastError());
+                    k_pair_bounds<<<1, kBlock, 0, c.stream()>>>(poffs, phist, tiles, (int)nb,
+                                                                 bstart);
+                    VDB_HIP(hipGetLastError());
                 } else {
+                    uint32_t nkeys = (uint32_t)nj * npairs;
+                    k_level_keys<<<ceil_div(npairs, kBlock), kBlock, 0, c.stream()>>>(
+                        plev, npairs, nj, K, ekey, eval);
+                    VDB_HIP(hipGetLastError());
                     uint32_t *sk, *sv;
                     VDB_TRY(radix_sort_pairs(c, ekey, eval, stk, stv, nkeys, bits_for(nb), &sk, &sv));
                     k_bucket_starts<<<ceil_div((size_t)nkeys + 1, kBlock), kBlock, 0, c.stream()>>>(

@@ -121,6 +121,20 @@ template <typename T> __device__ __forceinline__ T uniform(T v)
     return __shfl(v, 0);
 }
 
+// lanes of the wave holding the same key d < 2^BITS (d == 2^BITS: no key, a
+// lane that matches no key lane), from BITS + 1 ballots
+template <int BITS> __device__ __forceinline__ uint64_t bucket_peers(uint32_t d)
+{
+    uint64_t peers = ~0ull;
+#pragma unroll
+    for (int b = 0; b <= BITS; b++) {
+        bool bit = (d >> b) & 1u;
+        uint64_t bal = ballot(bit);
+        peers &= bit ? bal : ~bal;
+    }
+    return peers;
+}
+
 inline unsigned ceil_div(uint64_t a, uint64_t b) { return (unsigned)((a + b - 1) / b); }
 
 } // namespace vdb

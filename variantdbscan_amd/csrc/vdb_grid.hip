@@ -416,21 +416,24 @@ int build_rank_order(Context &c, Points &P, const char *tag)
     uint32_t n = P.n;
     if (P.hseg.size() != (size_t)P.nsets + 1) single_segment(P);
     uint32_t ns = P.nsets;
-    VDB_TRY(c.get(slot(tag, "seg").c_str(), (size_t)ns + 1, &P.seg));
-    VDB_HIP(hipMemcpyAsync(P.seg, P.hseg.data(), sizeof(uint32_t) * (ns + 1),
-                           hipMemcpyHostToDevice, c.stream()));
-    // pass-1 blocks of each dataset
-    std::vector<uint32_t> blk(ns + 1, 0);
+    // dataset bounds and the first extent block of each dataset, in one
+    // upload: seg[0..ns], then bbase[0..ns]
+    std::vector<uint32_t> up(2 * ((size_t)ns + 1), 0);
+    std::copy(P.hseg.begin(), P.hseg.end(), up.begin());
+    uint32_t *blk = up.data() + ns + 1;
     for (uint32_t d = 0; d < ns; d++)
         blk[d + 1] = blk[d] + (uint32_t)ceil_div(P.hseg[d + 1] - P.hseg[d], kExtChunk);
+    uint32_t *segb;
+    VDB_TRY(c.get(slot(tag, "seg").c_str(), up.size(), &segb));
+    VDB_HIP(hipMemcpyAsync(segb, up.data(), sizeof(uint32_t) * up.size(), hipMemcpyHostToDevice,
+                           c.stream()));
+    P.seg = segb;
+    uint32_t *bbase = segb + ns + 1;
     double *ext = nullptr, *part = nullptr;
-    uint32_t *nonfin = nullptr, *bbase = nullptr;
+    uint32_t *nonfin = nullptr;
     VDB_TRY(c.get(slot(tag, "ext").c_str(), (size_t)ns * 4, &ext));
     VDB_TRY(c.get(slot(tag, "extpart").c_str(), (size_t)std::max(blk[ns], 1u) * 4, &part));
-    VDB_TRY(c.get(slot(tag, "extbase").c_str(), (size_t)ns + 1, &bbase));
     VDB_TRY(c.get(slot(tag, "nonfin").c_str(), 1, &nonfin));
-    VDB_HIP(hipMemcpyAsync(bbase, blk.data(), sizeof(uint32_t) * (ns + 1), hipMemcpyHostToDevice,
-                           c.stream()));
     VDB_HIP(hipMemsetAsync(nonfin, 0, sizeof(uint32_t), c.stream()));
     if (blk[ns])
         k_seg_extent_part<<<blk[ns], kBlock, 0, c.stream()>>>(P.x, P.y, P.seg, bbase, ns, part,
@@ -605,17 +608,23 @@ int build_grid(Context &c, const Points &P, double w_min, int R, const char *tag
     G.npts = n;
     G.nsets = ns;
     uint64_t nkeys = (uint64_t)kbase[ns];
-    VDB_TRY(c.get(slot(tag, "sgx0").c_str(), ns, &G.sgx0));
-    VDB_TRY(c.get(slot(tag, "sgy0").c_str(), ns, &G.sgy0));
-    VDB_TRY(c.get(slot(tag, "sncx").c_str(), ns, &G.sncx));
-    VDB_TRY(c.get(slot(tag, "sncy").c_str(), ns, &G.sncy));
-    VDB_TRY(c.get(slot(tag, "skbase").c_str(), (size_t)ns + 1, &G.skbase));
-    VDB_HIP(hipMemcpyAsync(G.sgx0, gx0.data(), 8 * ns, hipMemcpyHostToDevice, c.stream()));
-    VDB_HIP(hipMemcpyAsync(G.sgy0, gy0.data(), 8 * ns, hipMemcpyHostToDevice, c.stream()));
-    VDB_HIP(hipMemcpyAsync(G.sncx, ncx.data(), 4 * ns, hipMemcpyHostToDevice, c.stream()));
-    VDB_HIP(hipMemcpyAsync(G.sncy, ncy.data(), 4 * ns, hipMemcpyHostToDevice, c.stream()));
-    VDB_HIP(hipMemcpyAsync(G.skbase, kbase.data(), 4 * ((size_t)ns + 1), hipMemcpyHostToDevice,
-                           c.stream()));
+    // the per-dataset grid parameters in one upload: origins (2 x 8 ns
+    // bytes), then cells per axis (2 x 4 ns) and key bases (4 (ns + 1))
+    const size_t pbytes = 16 * (size_t)ns + 8 * (size_t)ns + 4 * ((size_t)ns + 1);
+    std::vector<unsigned char> hp(pbytes);
+    memcpy(hp.data(), gx0.data(), 8 * (size_t)ns);
+    memcpy(hp.data() + 8 * (size_t)ns, gy0.data(), 8 * (size_t)ns);
+    memcpy(hp.data() + 16 * (size_t)ns, ncx.data(), 4 * (size_t)ns);
+    memcpy(hp.data() + 20 * (size_t)ns, ncy.data(), 4 * (size_t)ns);
+    memcpy(hp.data() + 24 * (size_t)ns, kbase.data(), 4 * ((size_t)ns + 1));
+    unsigned char *dp;
+    VDB_TRY(c.get(slot(tag, "params").c_str(), pbytes, &dp));
+    VDB_HIP(hipMemcpyAsync(dp, hp.data(), pbytes, hipMemcpyHostToDevice, c.stream()));
+    G.sgx0 = (double *)dp;
+    G.sgy0 = (double *)(dp + 8 * (size_t)ns);
+    G.sncx = (uint32_t *)(dp + 16 * (size_t)ns);
+    G.sncy = (uint32_t *)(dp + 20 * (size_t)ns);
+    G.skbase = (uint32_t *)(dp + 24 * (size_t)ns);
 
     uint32_t *keys, *vals, *tk, *tv, *flags, *excl;
     VDB_TRY(c.get(slot(tag, "gk").c_str(), n, &keys));

@@ -1,6 +1,5 @@
 // Device-wide primitives written for wave64: exclusive scans, a stable LSD
-// radix sort of (u32 key, u32 value) pairs, a one-pass bucket sort for small
-// keys, and float64 extent reductions.
+// radix sort of (u32 key, u32 value) pairs and float64 extent reductions.
 #pragma once
 
 #include "vdb_context.h"
@@ -21,21 +20,6 @@ int scan_exclusive_u32_u64(Context &c, const uint32_t *in, uint64_t *out, size_t
 int radix_sort_pairs(Context &c, uint32_t *keys, uint32_t *vals, uint32_t *tmp_keys,
                      uint32_t *tmp_vals, size_t n, int key_bits, uint32_t **keys_out,
                      uint32_t **vals_out);
-
-// Stable sort of n >= 1 values by keys in [0, nbuckets), nbuckets <= 128, in
-// one counting pass: out_vals gets the values in key order (the keys are not
-// written), starts[d] (device, nbuckets + 1 entries) the first position of
-// key d and starts[nbuckets] = n.
-int bucket_sort_values(Context &c, const uint32_t *keys, const uint32_t *vals,
-                       uint32_t *out_vals, uint32_t n, int nbuckets, uint32_t *starts);
-
-// The same sort, writing in place of each value v a three-word record
-// (a[i], b[i], c[i] | (v >> 31) << 16) with i = v & 0x7fffffff: the
-// records of three parallel arrays, gathered in input order (coalesced) and
-// stored in key order, so that a consumer streams them.
-int bucket_sort_records(Context &c, const uint32_t *keys, const uint32_t *vals,
-                        const uint32_t *a, const uint32_t *b, const uint16_t *cc,
-                        uint32_t *out_rec, uint32_t n, int nbuckets, uint32_t *starts);
 
 // ext_dev[0..3] = (min x, max x, min y, max y) over n points (n >= 1).
 int extent_f64(Context &c, const double *x, const double *y, size_t n, double *ext_dev);

@@ -240,124 +240,6 @@ __global__ void __launch_bounds__(kSortBlock)
     }
 }
 
-// ---------------------------------------- bucket sort (<= 16 or 128 keys)
-// One stable counting pass for keys below kBuckets = 2^BITS: a tile histogram
-// made with ballots (lanes sharing a key are counted by one lane, no LDS
-// atomics), a digit-major scan whose entries at tile 0 are the bucket starts,
-// and a scatter of the values only.
-constexpr int kMaxBucketBits = 7;
-
-// lanes of the wave holding the same key d (d == 2^BITS: no key)
-template <int BITS> __device__ __forceinline__ uint64_t bucket_peers(uint32_t d)
-{
-    uint64_t peers = ~0ull;
-#pragma unroll
-    for (int b = 0; b <= BITS; b++) {
-        bool bit = (d >> b) & 1u;
-        uint64_t bal = ballot(bit);
-        peers &= bit ? bal : ~bal;
-    }
-    return peers;
-}
-
-template <int BITS>
-__global__ void __launch_bounds__(kSortBlock)
-    k_bucket_hist(const uint32_t *keys, size_t n, uint32_t *hist, unsigned tiles)
-{
-    constexpr int kBuckets = 1 << BITS;
-    __shared__ uint32_t h[kSortWaves][kBuckets];
-    for (int i = threadIdx.x; i < kSortWaves * kBuckets; i += kSortBlock) (&h[0][0])[i] = 0;
-    __syncthreads();
-    int w = threadIdx.x / kWave;
-    size_t base = (size_t)blockIdx.x * kSortTile;
-    for (int i = 0; i < kSortItems; i++) {
-        size_t idx = base + (size_t)i * kSortBlock + threadIdx.x;
-        uint32_t d = idx < n ? keys[idx] : (uint32_t)kBuckets;
-        uint64_t peers = bucket_peers<BITS>(d);
-        // one lane per key: the wave's own row, no other writer
-        if (d < (uint32_t)kBuckets && mask_rank(peers) == 0) h[w][d] += (uint32_t)popc64(peers);
-    }
-    __syncthreads();
-    for (int d = threadIdx.x; d < kBuckets; d += kSortBlock) {
-        uint32_t s = 0;
-        for (int q = 0; q < kSortWaves; q++) s += h[q][d];
-        hist[(size_t)d * tiles + blockIdx.x] = s; // digit-major for the scan
-    }
-}
-
-// what the scatter moves: the value itself, or the record it indexes
-struct MoveValue {
-    const uint32_t *vals;
-    uint32_t *out;
-    using T = uint32_t;
-    __device__ T load(size_t i) const { return vals[i]; }
-    __device__ void store(uint32_t pos, const T &v) const { out[pos] = v; }
-};
-struct MoveRecord {
-    const uint32_t *vals, *a, *b;
-    const uint16_t *c;
-    uint3 *out;
-    using T = uint3;
-    __device__ T load(size_t i) const
-    {
-        uint32_t v = vals[i], q = v & 0x7fffffffu;
-        return make_uint3(a[q], b[q], (uint32_t)c[q] | ((v >> 31) << 16));
-    }
-    __device__ void store(uint32_t pos, const T &v) const { out[pos] = v; }
-};
-
-template <int BITS, class Move>
-__global__ void __launch_bounds__(kSortBlock)
-    k_bucket_scatter(const uint32_t *keys, Move mv, size_t n, const uint32_t *offs,
-                     unsigned tiles)
-{
-    constexpr int kBuckets = 1 << BITS;
-    __shared__ uint32_t base_d[kBuckets];
-    __shared__ uint32_t wcnt[kSortWaves][kBuckets];
-    for (int d = threadIdx.x; d < kBuckets; d += kSortBlock)
-        base_d[d] = offs[(size_t)d * tiles + blockIdx.x];
-    for (int i = threadIdx.x; i < kSortWaves * kBuckets; i += kSortBlock) (&wcnt[0][0])[i] = 0;
-    __syncthreads();
-    int w = threadIdx.x / kWave;
-    size_t tbase = (size_t)blockIdx.x * kSortTile;
-    for (int it = 0; it < kSortItems; it++) {
-        size_t idx = tbase + (size_t)it * kSortBlock + threadIdx.x;
-        bool valid = idx < n;
-        uint32_t d = valid ? keys[idx] : (uint32_t)kBuckets;
-        typename Move::T v{};
-        if (valid) v = mv.load(idx);
-        uint64_t peers = bucket_peers<BITS>(d);
-        int rank_w = mask_rank(peers);
-        if (valid && rank_w == 0) wcnt[w][d] = (uint32_t)popc64(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t pre = 0;
-            for (int q = 0; q < w; q++) pre += wcnt[q][d];
-            mv.store(base_d[d] + pre + (uint32_t)rank_w, v);
-        }
-        __syncthreads();
-        for (int dd = threadIdx.x; dd < kBuckets; dd += kSortBlock) {
-            uint32_t s = 0;
-            for (int q = 0; q < kSortWaves; q++) {
-                s += wcnt[q][dd];
-                wcnt[q][dd] = 0;
-            }
-            base_d[dd] += s;
-        }
-        __syncthreads();
-    }
-}
-
-// starts[d] = first sorted position of key d (the scan entry of tile 0),
-// starts[nb] = n
-__global__ void k_bucket_bounds(const uint32_t *offs, unsigned tiles, int nb, uint32_t n,
-                                uint32_t *starts)
-{
-    int d = threadIdx.x;
-    if (d < nb) starts[d] = offs[(size_t)d * tiles];
-    if (d == nb) starts[d] = n;
-}
-
 // ---------------------------------------------------------------- extent
 constexpr int kExtBlock = 256;
 
@@ -478,55 +360,6 @@ int radix_sort_pairs(Context &c, uint32_t *keys, uint32_t *vals, uint32_t *tmp_k
     return kOk;
 }
 
-template <int BITS, class Move>
-int bucket_sort_bits(Context &c, const uint32_t *keys, const Move &mv, uint32_t n,
-                     uint32_t *offs_out[1])
-{
-    constexpr int kBuckets = 1 << BITS;
-    unsigned tiles = ceil_div(n, kSortTile);
-    uint32_t *hist = nullptr, *offs = nullptr;
-    VDB_TRY(c.get("bucket.hist", (size_t)tiles * kBuckets, &hist));
-    VDB_TRY(c.get("bucket.offs", (size_t)tiles * kBuckets, &offs));
-    k_bucket_hist<BITS><<<tiles, kSortBlock, 0, c.stream()>>>(keys, n, hist, tiles);
-    VDB_HIP(hipGetLastError());
-    VDB_TRY(scan_exclusive_u32(c, hist, offs, (size_t)tiles * kBuckets, nullptr));
-    k_bucket_scatter<BITS, Move><<<tiles, kSortBlock, 0, c.stream()>>>(keys, mv, n, offs, tiles);
-    VDB_HIP(hipGetLastError());
-    *offs_out = offs;
-    return kOk;
-}
-
-template <class Move>
-int bucket_sort_move(Context &c, const uint32_t *keys, const Move &mv, uint32_t n, int nbuckets,
-                     uint32_t *starts)
-{
-    if (nbuckets < 1 || nbuckets > (1 << kMaxBucketBits) || n == 0) {
-        set_error("bucket sort: 1..128 buckets and at least one value");
-        return kErrInvalid;
-    }
-    unsigned tiles = ceil_div(n, kSortTile);
-    uint32_t *offs = nullptr;
-    if (nbuckets <= 16) VDB_TRY(bucket_sort_bits<4>(c, keys, mv, n, &offs));
-    else VDB_TRY(bucket_sort_bits<kMaxBucketBits>(c, keys, mv, n, &offs));
-    // one thread per bucket start (up to 129)
-    k_bucket_bounds<<<1, kSortBlock, 0, c.stream()>>>(offs, tiles, nbuckets, n, starts);
-    VDB_HIP(hipGetLastError());
-    return kOk;
-}
-
-int bucket_sort_values(Context &c, const uint32_t *keys, const uint32_t *vals,
-                       uint32_t *out_vals, uint32_t n, int nbuckets, uint32_t *starts)
-{
-    return bucket_sort_move(c, keys, MoveValue{vals, out_vals}, n, nbuckets, starts);
-}
-
-int bucket_sort_records(Context &c, const uint32_t *keys, const uint32_t *vals,
-                        const uint32_t *a, const uint32_t *b, const uint16_t *cc,
-                        uint32_t *out_rec, uint32_t n, int nbuckets, uint32_t *starts)
-{
-    return bucket_sort_move(c, keys, MoveRecord{vals, a, b, cc, (uint3 *)out_rec}, n, nbuckets,
-                            starts);
-}
 
 int extent_f64(Context &c, const double *x, const double *y, size_t n, double *ext_dev)
 {
