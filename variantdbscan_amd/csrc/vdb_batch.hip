@@ -1249,28 +1249,33 @@ __global__ void __launch_bounds__(kBlock) k_sweep(LevelArgs a, LevelLists L)
                 }
             }
         }
-        // Lanes sharing the A-side root (consecutive pairs of one cell) hook
-        // every root of their group straight under the group's smallest
-        // root, instead of racing to relink the shared root one pair at a
-        // time. A hook that loses its race falls back to a full union.
-        uint32_t m_l = kNone;
-        bool lead = false;
-        uint64_t todo = ballot(uni);
-        while (todo) {
-            int leader = __builtin_ctzll(todo);
-            uint32_t r = __shfl(ra, leader);
-            bool mine = uni && ra == r;
-            todo &= ~ballot(mine);
-            uint32_t m = mine ? (rb < ra ? rb : ra) : kNone;
-            for (int o = kWave / 2; o > 0; o >>= 1) {
-                uint32_t t = __shfl_xor(m, o);
-                m = t < m ? t : m;
+        // Runs of consecutive lanes uniting under one A-side root (the pairs
+        // of one cell come in order) hook every root of the run straight
+        // under the run's smallest root, instead of racing to relink the
+        // shared root one pair at a time: a segmented min over the wave, a
+        // fixed number of shuffles however many runs there are. Runs of the
+        // same root elsewhere in the wave hook separately; a hook that loses
+        // its race falls back to a full union.
+        if (ballot(uni)) {
+            const int lane = lane_id();
+            uint32_t pra = __shfl_up(ra, 1);
+            bool puni = __shfl_up((int)uni, 1) != 0;
+            bool head = !uni || lane == 0 || !puni || pra != ra;
+            uint64_t H = ballot(head); // run starts (lanes outside runs too)
+            uint64_t upto = lane == kWave - 1 ? ~0ull : (2ull << lane) - 1ull;
+            int hpos = 63 - __builtin_clzll(H & upto);
+            uint64_t above = H & ~upto;
+            int last = above ? __builtin_ctzll(above) - 1 : kWave - 1;
+            uint32_t m = uni ? (rb < ra ? rb : ra) : kNone;
+#pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) {
+                uint32_t t = __shfl_up(m, d);
+                if (lane - d >= hpos) m = t < m ? t : m;
             }
-            if (mine) m_l = m;
-            if (lane_id() == leader) lead = true;
+            m = __shfl(m, last); // the run's minimum, held by its last lane
+            if (uni && rb != m) cf_hook(parent, rb, m);
+            if (uni && head && ra != m) cf_hook(parent, ra, m);
         }
-        if (uni && rb != m_l) cf_hook(parent, rb, m_l);
-        if (lead && ra != m_l) cf_hook(parent, ra, m_l);
         wave_append_rec(kp, pr, keep, nkeep);
         if (chunked) {
             chunk_append(tnear, pr, near, nnear, tcap, ovf, chn);
@@ -2056,9 +2061,34 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     char *hs = nullptr;
     VDB_TRY(c.host_scratch(32 + 4 * (4 + ncnt_all + (size_t)J * (K + 1) + 1), (void **)&hs));
 
+    // ---- truncated distance-sorted table: the listed points first, so
+    // that the list kernel does not wait behind the pair count
+    Ev tknn(c.stream(), st != nullptr);
+    uint32_t *flag, *excl, *list_pos, *lst_idx, *nlist_dev;
+    VDB_TRY(c.get("b.flag", n, &flag));
+    VDB_TRY(c.get("b.excl", n, &excl));
+    VDB_TRY(c.get("b.lst_idx", n, &lst_idx));
+    VDB_TRY(c.get("b.nlist", 1, &nlist_dev));
+    unsigned gn = ceil_div(n, kBlock);
+    k_list_flags<<<gn, kBlock, 0, c.stream()>>>(G.cell_start, G.cell_of_pos, n, L, flag);
+    VDB_HIP(hipGetLastError());
+    VDB_TRY(scan_exclusive_u32(c, flag, excl, n, nlist_dev));
+    // Lists are sized for every point, so that nothing waits for their count
+    // (the truncated-list kernel reads it on the device), unless that takes
+    // more than 8 GB: then the count is read first.
+    const bool exact_lists = (size_t)n * L * 12 > ((size_t)8 << 30);
+    uint32_t nlist_cap = n;
+    if (exact_lists) {
+        VDB_HIP(hipMemcpyAsync(hs + 16, nlist_dev, 4, hipMemcpyDeviceToHost, c.stream()));
+        VDB_HIP(hipStreamSynchronize(c.stream()));
+        memcpy(&nlist_cap, hs + 16, 4);
+    }
+    VDB_TRY(c.get("b.list_pos", nlist_cap ? nlist_cap : 1, &list_pos));
+    k_list_index<<<gn, kBlock, 0, c.stream()>>>(flag, excl, n, list_pos, lst_idx);
+    VDB_HIP(hipGetLastError());
     // ---- cell pairs within eps_max: counted on the library's second
-    // stream beside the lists (both read only the grid); the pair total and
-    // the clique check come back in one read
+    // stream beside the truncated lists (both read only the grid); the pair
+    // total and the clique check come back in one read
     hipStream_t aux;
     VDB_TRY(c.aux_stream(&aux)); // (also creates the ordering events)
     const hipStream_t mainst = c.stream();
@@ -2088,30 +2118,6 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         VDB_HIP(hipEventRecord(c.aux_event(1), ps));
     }
 
-    // ---- truncated distance-sorted table
-    Ev tknn(c.stream(), st != nullptr);
-    uint32_t *flag, *excl, *list_pos, *lst_idx, *nlist_dev;
-    VDB_TRY(c.get("b.flag", n, &flag));
-    VDB_TRY(c.get("b.excl", n, &excl));
-    VDB_TRY(c.get("b.lst_idx", n, &lst_idx));
-    VDB_TRY(c.get("b.nlist", 1, &nlist_dev));
-    unsigned gn = ceil_div(n, kBlock);
-    k_list_flags<<<gn, kBlock, 0, c.stream()>>>(G.cell_start, G.cell_of_pos, n, L, flag);
-    VDB_HIP(hipGetLastError());
-    VDB_TRY(scan_exclusive_u32(c, flag, excl, n, nlist_dev));
-    // Lists are sized for every point, so that nothing waits for their count
-    // (the truncated-list kernel reads it on the device), unless that takes
-    // more than 8 GB: then the count is read first.
-    const bool exact_lists = (size_t)n * L * 12 > ((size_t)8 << 30);
-    uint32_t nlist_cap = n;
-    if (exact_lists) {
-        VDB_HIP(hipMemcpyAsync(hs + 16, nlist_dev, 4, hipMemcpyDeviceToHost, c.stream()));
-        VDB_HIP(hipStreamSynchronize(c.stream()));
-        memcpy(&nlist_cap, hs + 16, 4);
-    }
-    VDB_TRY(c.get("b.list_pos", nlist_cap ? nlist_cap : 1, &list_pos));
-    k_list_index<<<gn, kBlock, 0, c.stream()>>>(flag, excl, n, list_pos, lst_idx);
-    VDB_HIP(hipGetLastError());
     double *lst_s;
     uint32_t *lst_id, *lst_n;
     size_t lsz = (size_t)(nlist_cap ? (nlist_cap + 63) / 64 * 64 : 64) * L; // blocks of 64 lists
