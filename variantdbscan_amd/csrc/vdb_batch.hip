@@ -212,8 +212,8 @@ __device__ __forceinline__ void knn_block(double s, uint32_t q, bool hit, int L,
         if (cnt == L && !before(cs, cq, kth, kid)) continue;
         bool lt = lane < L && before(es, eid, cs, cq);
         int pos = popc64(ballot(lt));
-        double ps = __shfl_up(es, 1);
-        uint32_t pid = __shfl_up(eid, 1);
+        double ps = lane_up1_f64(es);
+        uint32_t pid = lane_up1(eid);
         if (lane == pos) {
             es = cs;
             eid = cq;
@@ -236,28 +236,62 @@ constexpr uint32_t kKnnRowSweep = 512;
 // windows holding at most this many points are swept as one flat list
 constexpr uint32_t kKnnFlat = 512;
 
+// one compare-exchange stage of a bitonic network: partner lane ^ J, groups
+// of k sorted ascending when (lane & k) == 0
+template <int J>
+__device__ __forceinline__ void bitonic_stage(double &es, uint32_t &eid, int k, int lane)
+{
+    double ps = lane_xor_f64<J>(es);
+    uint32_t pid = lane_xor<J>(eid);
+    bool low = (lane & J) == 0, asc = (lane & k) == 0;
+    // the lower lane of an ascending pair keeps the smaller entry
+    bool take = (low == asc) ? before(ps, pid, es, eid) : before(es, eid, ps, pid);
+    if (take) {
+        es = ps;
+        eid = pid;
+    }
+}
+
 // sort the lane-resident entries (lane e holds entry e) by (s, id), ascending.
 // Only lanes below cnt hold entries (the rest hold (INF, NONE)), so the
 // bitonic network stops at the first power of two >= cnt: its stages sort
-// every aligned group of that size, the first one ascending.
+// every aligned group of that size, the first one ascending. The partner
+// exchanges are DPP moves and row swaps, not LDS permutes.
 __device__ __forceinline__ void wave_sort(double &es, uint32_t &eid, int cnt)
 {
     int lane = lane_id();
-#pragma unroll
-    for (int k = 2; k <= kWave; k <<= 1) {
-        if (k >= 2 * cnt) continue;
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            double ps = __shfl_xor(es, j);
-            uint32_t pid = __shfl_xor(eid, j);
-            bool low = (lane & j) == 0, asc = (lane & k) == 0;
-            // the lower lane of an ascending pair keeps the smaller entry
-            bool take = (low == asc) ? before(ps, pid, es, eid) : before(es, eid, ps, pid);
-            if (take) {
-                es = ps;
-                eid = pid;
-            }
-        }
+    if (cnt > 1) {
+        bitonic_stage<1>(es, eid, 2, lane);
+    }
+    if (cnt > 2) {
+        bitonic_stage<2>(es, eid, 4, lane);
+        bitonic_stage<1>(es, eid, 4, lane);
+    }
+    if (cnt > 4) {
+        bitonic_stage<4>(es, eid, 8, lane);
+        bitonic_stage<2>(es, eid, 8, lane);
+        bitonic_stage<1>(es, eid, 8, lane);
+    }
+    if (cnt > 8) {
+        bitonic_stage<8>(es, eid, 16, lane);
+        bitonic_stage<4>(es, eid, 16, lane);
+        bitonic_stage<2>(es, eid, 16, lane);
+        bitonic_stage<1>(es, eid, 16, lane);
+    }
+    if (cnt > 16) {
+        bitonic_stage<16>(es, eid, 32, lane);
+        bitonic_stage<8>(es, eid, 32, lane);
+        bitonic_stage<4>(es, eid, 32, lane);
+        bitonic_stage<2>(es, eid, 32, lane);
+        bitonic_stage<1>(es, eid, 32, lane);
+    }
+    if (cnt > 32) {
+        bitonic_stage<32>(es, eid, 64, lane);
+        bitonic_stage<16>(es, eid, 64, lane);
+        bitonic_stage<8>(es, eid, 64, lane);
+        bitonic_stage<4>(es, eid, 64, lane);
+        bitonic_stage<2>(es, eid, 64, lane);
+        bitonic_stage<1>(es, eid, 64, lane);
     }
 }
 

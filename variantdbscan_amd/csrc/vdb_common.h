@@ -121,6 +121,49 @@ template <typename T> __device__ __forceinline__ T uniform(T v)
     return __shfl(v, 0);
 }
 
+// v of lane (lane ^ J), J a constant power of two below 64, without the LDS
+// permute: DPP moves inside rows of 16 lanes (J = 1, 2: quad permutes; 4:
+// quad reversal then row half-mirror; 8: row rotate by 8) and the gfx950
+// row swaps across rows (J = 16, 32: permlane16/32_swap of v with itself,
+// the lane taking the half that came from the other row).
+template <int J> __device__ __forceinline__ uint32_t lane_xor(uint32_t v)
+{
+    static_assert(J == 1 || J == 2 || J == 4 || J == 8 || J == 16 || J == 32, "xor mask");
+    if constexpr (J == 1) {
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xf, 0xf, false);
+    } else if constexpr (J == 2) {
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xf, 0xf, false);
+    } else if constexpr (J == 4) {
+        int t = __builtin_amdgcn_update_dpp(0, (int)v, 0x1B, 0xf, 0xf, false);
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, t, 0x141, 0xf, 0xf, false);
+    } else if constexpr (J == 8) {
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xf, 0xf, false);
+    } else if constexpr (J == 16) {
+        auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return (__lane_id() & 16) ? r[0] : r[1];
+    } else {
+        auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return (__lane_id() & 32) ? r[0] : r[1];
+    }
+}
+// v of lane - 1 (lane 0 keeps its own): the DPP whole-wave shift
+__device__ __forceinline__ uint32_t lane_up1(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ double lane_up1_f64(double v)
+{
+    uint64_t b = (uint64_t)__double_as_longlong(v);
+    uint32_t lo = lane_up1((uint32_t)b), hi = lane_up1((uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+template <int J> __device__ __forceinline__ double lane_xor_f64(double v)
+{
+    uint64_t b = (uint64_t)__double_as_longlong(v);
+    uint32_t lo = lane_xor<J>((uint32_t)b), hi = lane_xor<J>((uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 // lanes of the wave holding the same key d < 2^BITS (d == 2^BITS: no key, a
 // lane that matches no key lane), from BITS + 1 ballots
 template <int BITS> __device__ __forceinline__ uint64_t bucket_peers(uint32_t d)
