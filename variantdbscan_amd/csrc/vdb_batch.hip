@@ -176,17 +176,38 @@ __global__ void k_list_flags(const uint32_t *cell_start, const uint32_t *cell_of
     flag[p] = (cell_start[c + 1] - cell_start[c] < L) ? 1u : 0u;
 }
 
-__global__ void k_list_index(const uint32_t *flag, const uint32_t *excl, uint32_t n,
-                             uint32_t *list_pos, uint32_t *lst_idx)
+// A listed point's list index, and the point itself (coordinates and cell)
+// stored at that index: the list kernel's first loads then need nothing else.
+__global__ void k_list_index(const uint32_t *flag, const uint32_t *excl, const double *xs,
+                             const double *ys, const uint32_t *cell_of_pos, uint32_t n,
+                             double2 *list_xy, uint32_t *list_cell, uint32_t *lst_idx)
 {
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= n) return;
     if (flag[p]) {
-        list_pos[excl[p]] = p;
-        lst_idx[p] = excl[p];
+        uint32_t i = excl[p];
+        list_xy[i] = make_double2(xs[p], ys[p]);
+        list_cell[i] = cell_of_pos[p];
+        lst_idx[p] = i;
     } else {
         lst_idx[p] = kNone;
     }
+}
+
+// Point range (start, count) of every window row of every cell: the list
+// kernel reads these with the rows' cell ranges in one round of loads.
+__global__ void k_row_points(const uint32_t *row_lo, const uint32_t *row_hi,
+                             const uint32_t *cell_start, size_t nrows, uint2 *row_pts)
+{
+    size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= nrows) return;
+    uint32_t lo = row_lo[t], hi = row_hi[t];
+    uint2 r = make_uint2(0u, 0u);
+    if (lo < hi) {
+        r.x = cell_start[lo];
+        r.y = cell_start[hi] - r.x;
+    }
+    row_pts[t] = r;
 }
 
 // ------------------------------------------------------ truncated table
@@ -298,16 +319,23 @@ __device__ __forceinline__ void wave_sort(double &es, uint32_t &eid, int cnt)
 // 8 waves per SIMD: the search is latency-bound, so occupancy pays more
 // than the scalar registers it costs (their spills go to vector lanes)
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
-    k_knn(FG g, const uint32_t *__restrict__ list_pos, const uint32_t *__restrict__ nlist_dev,
-          int L, double Tmax,
-          double T0, double *__restrict__ lst_s, uint32_t *__restrict__ lst_id,
-          uint32_t *__restrict__ lst_n)
+    k_knn(FG g, const double2 *__restrict__ list_xy, const uint32_t *__restrict__ list_cell,
+          const uint2 *__restrict__ row_pts, const uint32_t *__restrict__ nlist_dev,
+          int L, const double *__restrict__ Tk, int K, double *__restrict__ lst_s,
+          uint32_t *__restrict__ lst_id, uint32_t *__restrict__ lst_n)
 {
+    // the level thresholds, for the level order of the entries
+    __shared__ double sTk[kMaxLevels];
+    for (int k = threadIdx.x; k < K; k += kBlock) sTk[k] = Tk[k];
+    __syncthreads();
+    const double Tmax = sTk[K - 1], T0 = sTk[0];
     // Two modes. While at most 64 candidates within eps_max have been seen,
-    // they are appended unsorted (compacted through LDS) and sorted once at
-    // the end: a sparse point costs one sort instead of one insertion per
-    // candidate. A block that would overflow 64 entries sorts, keeps the L
-    // smallest and switches to sorted insertion with a shrinking threshold.
+    // they are appended unsorted (compacted through LDS) and put in level
+    // order once at the end: a sparse point costs one counting pass instead
+    // of one insertion per candidate. A block that would overflow 64 entries
+    // sorts by (s, id), keeps the L smallest and switches to sorted
+    // insertion with a shrinking threshold (an order by s is also one by
+    // level).
     __shared__ double abuf_s[kWPB][kWave];
     __shared__ uint32_t abuf_id[kWPB][kWave];
     uint32_t i = blockIdx.x * kWPB + wave_id_in_block();
@@ -315,9 +343,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 
     double *bs = abuf_s[wave_id_in_block()];
     uint32_t *bid = abuf_id[wave_id_in_block()];
     int lane = lane_id();
-    uint32_t p = list_pos[i];
-    double px = g.xs[p], py = g.ys[p];
-    uint32_t c = g.cell_of_pos[p];
+    double2 pxy = list_xy[i];
+    double px = pxy.x, py = pxy.y;
+    uint32_t c = list_cell[i];
     double es = INFINITY;  // entry s of this lane
     uint32_t eid = kNone;  // entry id
     int cnt = 0;           // entries held (uniform)
@@ -326,6 +354,47 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 
     double thr = Tmax;     // candidates must have s <= thr
     bool done = false;
     bool appending = true;
+    // The list's consumers need its entries in order of level (the first k
+    // with s <= T_k), not of s: the core levels are the levels of entries
+    // m_j - 1, and a label walk stops at the first entry beyond its largest
+    // eps. Appended entries that end the search in append mode are only put
+    // in level order (a stable counting pass over the <= 64 lane-resident
+    // entries), and truncated to L: the entries cut are all of the level at
+    // which the point holds L neighbours, where it is core for every minpts
+    // value, so no walk reads them.
+    auto settle_levels = [&]() {
+        int lvl = K;
+        if (lane < cnt) {
+            lvl = 0;
+            while (lvl < K - 1 && es > sTk[lvl]) lvl++;
+        }
+        const uint64_t valid = cnt >= kWave ? ~0ull : ((1ull << cnt) - 1ull);
+        // lanes of a smaller level (less) and of the same level (eq)
+        uint64_t less = 0, eq = valid;
+#pragma unroll
+        for (int b = 5; b >= 0; b--) {
+            bool bit = (lvl >> b) & 1;
+            uint64_t bal = ballot(bit) & valid;
+            if (bit) less |= eq & ~bal;
+            eq &= bit ? bal : (valid & ~bal);
+        }
+        if (lane < cnt) {
+            int at = popc64(less) + mask_rank(eq);
+            bs[at] = es;
+            bid[at] = eid;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (lane < cnt && lane < L) {
+            es = bs[lane];
+            eid = bid[lane];
+        } else {
+            es = INFINITY;
+            eid = kNone;
+        }
+        __builtin_amdgcn_wave_barrier();
+        cnt = cnt < L ? cnt : L;
+        appending = false;
+    };
     // sort the appended entries, keep the L smallest, enter insertion mode
     auto settle = [&]() {
         wave_sort(es, eid, cnt);
@@ -362,10 +431,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 
                 __builtin_amdgcn_wave_barrier();
                 cnt += h;
                 // L entries within eps_0: core in every variant, list unused
-                if (cnt >= L && popc64(ballot(es <= T0)) >= L) {
-                    settle();
-                    done = true;
-                }
+                // (put in level order at the end)
+                if (cnt >= L && popc64(ballot(es <= T0)) >= L) done = true;
                 return;
             }
             settle();
@@ -386,10 +453,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 
         size_t t = (size_t)c * g.W + (dl + g.W / 2);
         rlo = g.row_lo[t];
         rhi = g.row_hi[t];
-        if (rlo < rhi) {
-            ra = g.cell_start[rlo];
-            rn = g.cell_start[rhi] - ra;
-        }
+        uint2 rp = row_pts[t];
+        ra = rp.x;
+        rn = rp.y;
     }
     uint32_t pre = rn; // inclusive prefix over rows
     for (int o = 1; o < kWave; o <<= 1) {
@@ -481,7 +547,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 
             }
         }
     }
-    if (appending) settle();
+    if (appending) settle_levels();
     if (lane < L) {
         lst_s[lst_at(i, lane, L)] = es;
         lst_id[lst_at(i, lane, L)] = eid;
@@ -1326,71 +1392,78 @@ __global__ void __launch_bounds__(kBlock) k_sweep(LevelArgs a, LevelLists L)
 // Wave per test pair: any core-core pair within eps_k connects the two cells.
 // The smaller cell A is screened 64 points at a time (core at level k and
 // within eps of B's box); only the survivors are tested, one after another,
-// against B. B's points are held in registers when B fits one wave.
+// against B. B's points are held in registers when B fits one wave. The
+// loads go out in two rounds: both cells' point ranges with both parents,
+// then B's box with the first 64 points of each cell (indices clamped into
+// the cell, so no load waits behind a branch).
 __device__ void test_pair(const LevelArgs &a, const PairRec &pr, uint32_t *parent,
                           PairRec *list_out, uint32_t *nout)
 {
     if (pr.a == kNone) return; // unused slot of a test chunk
     uint32_t A = pr.a, B = pr.b;
-    uint32_t na = a.cell_start[A + 1] - a.cell_start[A];
-    uint32_t nb = a.cell_start[B + 1] - a.cell_start[B];
-    if (na > nb) { // lanes sweep the larger cell
+    uint32_t a0 = a.cell_start[A], a1 = a.cell_start[A + 1];
+    uint32_t b0 = a.cell_start[B], b1 = a.cell_start[B + 1];
+    uint32_t pa = ld_u32(parent + A), pb = ld_u32(parent + B);
+    if (a1 - a0 > b1 - b0) { // lanes sweep the larger cell
         uint32_t t = A;
         A = B;
         B = t;
+        t = a0;
+        a0 = b0;
+        b0 = t;
+        t = a1;
+        a1 = b1;
+        b1 = t;
     }
-    bool found = cf_same(parent, A, B);
-    if (!found) {
-        int lane = lane_id();
-        uint32_t a0 = a.cell_start[A], a1 = a.cell_start[A + 1];
-        uint32_t b0 = a.cell_start[B], b1 = a.cell_start[B + 1];
-        const double *bB = a.box + 4 * (size_t)B;
-        double bx0 = bB[0], bx1 = bB[1], by0 = bB[2], by1 = bB[3];
-        // B in one wave: lane q holds point b0+q (NaN when absent or not core)
-        bool bsmall = b1 - b0 <= (uint32_t)kWave;
-        double qx = __longlong_as_double(0x7ff8000000000000ll), qy = qx;
-        if (bsmall) {
-            uint32_t q = b0 + lane;
-            if (q < b1 && a.kc[(size_t)q * a.J + a.j] <= a.k) {
-                qx = a.xs[q];
-                qy = a.ys[q];
-            }
+    bool found = pa == pb || cf_find(parent, pa) == cf_find(parent, pb);
+    if (found) return;
+    const int lane = lane_id();
+    const uint32_t na = a1 - a0, nb = b1 - b0; // both >= 1
+    const double *bB = a.box + 4 * (size_t)B;
+    double bx0 = bB[0], bx1 = bB[1], by0 = bB[2], by1 = bB[3];
+    uint32_t qi = b0 + min((uint32_t)lane, nb - 1);
+    int kq = a.kc[(size_t)qi * a.J + a.j];
+    double qx = a.xs[qi], qy = a.ys[qi];
+    uint32_t pi = a0 + min((uint32_t)lane, na - 1);
+    int kp = a.kc[(size_t)pi * a.J + a.j];
+    double px = a.xs[pi], py = a.ys[pi];
+    // B in one wave: lane q holds point b0+q (NaN when absent or not core)
+    const bool bsmall = nb <= (uint32_t)kWave;
+    if (!((uint32_t)lane < nb && kq <= a.k)) qx = qy = __longlong_as_double(0x7ff8000000000000ll);
+    for (uint32_t p0 = a0; p0 < a1 && !found; p0 += kWave) {
+        uint32_t p = p0 + lane;
+        if (p0 != a0) { // later chunks of a cell larger than a wave
+            uint32_t pc = min(p, a1 - 1);
+            kp = a.kc[(size_t)pc * a.J + a.j];
+            px = a.xs[pc];
+            py = a.ys[pc];
         }
-        for (uint32_t p0 = a0; p0 < a1 && !found; p0 += kWave) {
-            uint32_t p = p0 + lane;
-            double px = 0.0, py = 0.0;
-            bool cand = false;
-            if (p < a1 && a.kc[(size_t)p * a.J + a.j] <= a.k) {
-                px = a.xs[p];
-                py = a.ys[p];
-                // some point of B can be within eps of p
-                cand = box_dist2_min(px, py, bx0, bx1, by0, by1) <= a.T;
+        // a core point of A from which some point of B can be within eps
+        bool cand = p < a1 && kp <= a.k && box_dist2_min(px, py, bx0, bx1, by0, by1) <= a.T;
+        uint64_t cm = ballot(cand);
+        while (cm && !found) {
+            int bit = __builtin_ctzll(cm);
+            cm &= cm - 1;
+            double cx = __shfl(px, bit), cy = __shfl(py, bit);
+            if (bsmall) {
+                // NaN coordinates never pass the test
+                found = ballot(dist2_exact(cx, cy, qx, qy) <= a.T) != 0;
+                continue;
             }
-            uint64_t cm = ballot(cand);
-            while (cm && !found) {
-                int bit = __builtin_ctzll(cm);
-                cm &= cm - 1;
-                double cx = __shfl(px, bit), cy = __shfl(py, bit);
-                if (bsmall) {
-                    // NaN coordinates never pass the test
-                    found = ballot(dist2_exact(cx, cy, qx, qy) <= a.T) != 0;
-                    continue;
-                }
-                for (uint32_t q0 = b0; q0 < b1; q0 += kWave) {
-                    uint32_t q = q0 + lane;
-                    bool hit = q < b1 && a.kc[(size_t)q * a.J + a.j] <= a.k &&
-                               dist2_exact(cx, cy, a.xs[q], a.ys[q]) <= a.T;
-                    if (ballot(hit)) {
-                        found = true;
-                        break;
-                    }
+            for (uint32_t q0 = b0; q0 < b1; q0 += kWave) {
+                uint32_t q = q0 + lane;
+                bool hit = q < b1 && a.kc[(size_t)q * a.J + a.j] <= a.k &&
+                           dist2_exact(cx, cy, a.xs[q], a.ys[q]) <= a.T;
+                if (ballot(hit)) {
+                    found = true;
+                    break;
                 }
             }
         }
-        if (lane == 0) {
-            if (found) cf_unite(parent, A, B);
-            else list_out[atomicAdd(nout, 1u)] = pr;
-        }
+    }
+    if (lane == 0) {
+        if (found) cf_unite(parent, A, B);
+        else list_out[atomicAdd(nout, 1u)] = pr;
     }
 }
 
@@ -2098,7 +2171,8 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     // ---- truncated distance-sorted table: the listed points first, so
     // that the list kernel does not wait behind the pair count
     Ev tknn(c.stream(), st != nullptr);
-    uint32_t *flag, *excl, *list_pos, *lst_idx, *nlist_dev;
+    uint32_t *flag, *excl, *list_cell, *lst_idx, *nlist_dev;
+    double2 *list_xy;
     VDB_TRY(c.get("b.flag", n, &flag));
     VDB_TRY(c.get("b.excl", n, &excl));
     VDB_TRY(c.get("b.lst_idx", n, &lst_idx));
@@ -2117,9 +2191,21 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         VDB_HIP(hipStreamSynchronize(c.stream()));
         memcpy(&nlist_cap, hs + 16, 4);
     }
-    VDB_TRY(c.get("b.list_pos", nlist_cap ? nlist_cap : 1, &list_pos));
-    k_list_index<<<gn, kBlock, 0, c.stream()>>>(flag, excl, n, list_pos, lst_idx);
+    VDB_TRY(c.get("b.list_xy", nlist_cap ? nlist_cap : 1, &list_xy));
+    VDB_TRY(c.get("b.list_cell", nlist_cap ? nlist_cap : 1, &list_cell));
+    k_list_index<<<gn, kBlock, 0, c.stream()>>>(flag, excl, G.xs, G.ys, G.cell_of_pos, n, list_xy,
+                                                list_cell, lst_idx);
     VDB_HIP(hipGetLastError());
+    // the window rows' point ranges
+    const size_t nwrows = (size_t)G.ncell * (2 * R + 1);
+    uint2 *row_pts;
+    VDB_TRY(c.get("b.row_pts", nwrows ? nwrows : 1, &row_pts));
+    if (nwrows) {
+        k_row_points<<<ceil_div(nwrows, kBlock), kBlock, 0, c.stream()>>>(G.row_lo, G.row_hi,
+                                                                         G.cell_start, nwrows,
+                                                                         row_pts);
+        VDB_HIP(hipGetLastError());
+    }
     // ---- cell pairs within eps_max: counted on the library's second
     // stream beside the truncated lists (both read only the grid); the pair
     // total and the clique check come back in one read
@@ -2161,7 +2247,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     if (nlist_cap) {
         VDB_TRY(c.kt_begin("k_knn"));
         k_knn<<<ceil_div(nlist_cap, kWPB), kBlock, 0, c.stream()>>>(
-            fview(G), list_pos, nlist_dev, (int)L, Th[K - 1], Th[0], lst_s, lst_id, lst_n);
+            fview(G), list_xy, list_cell, row_pts, nlist_dev, (int)L, dTk, K, lst_s, lst_id, lst_n);
         VDB_TRY(c.kt_end("k_knn"));
     }
     VDB_HIP(hipGetLastError());
