@@ -324,6 +324,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 
           int L, const double *__restrict__ Tk, int K, double *__restrict__ lst_s,
           uint32_t *__restrict__ lst_id, uint32_t *__restrict__ lst_n)
 {
+    // the grid is sized for every point, the list count known on the device
+    // only: blocks wholly beyond it leave before anything else
+    const uint32_t nlist = *nlist_dev;
+    if (blockIdx.x * kWPB >= nlist) return;
     // the level thresholds, for the level order of the entries
     __shared__ double sTk[kMaxLevels];
     for (int k = threadIdx.x; k < K; k += kBlock) sTk[k] = Tk[k];
@@ -339,7 +343,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 
     __shared__ double abuf_s[kWPB][kWave];
     __shared__ uint32_t abuf_id[kWPB][kWave];
     uint32_t i = blockIdx.x * kWPB + wave_id_in_block();
-    if (i >= *nlist_dev) return; // launched for an upper bound of the count
+    if (i >= nlist) return;
     double *bs = abuf_s[wave_id_in_block()];
     uint32_t *bid = abuf_id[wave_id_in_block()];
     int lane = lane_id();
@@ -960,10 +964,14 @@ __global__ void __launch_bounds__(kBlock)
                 unsigned tiles)
 {
     __shared__ uint32_t h[kWPB][kPairBuckets];
-    if (hist) {
+    __shared__ double sTk[kMaxLevels]; // level thresholds for the exact levels
+    if (hist)
         for (int r = threadIdx.x; r < kWPB * kPairBuckets; r += kBlock) (&h[0][0])[r] = 0;
-        __syncthreads();
-    }
+    for (int k = threadIdx.x; k < K; k += kBlock) sTk[k] = Tk[k];
+    __syncthreads();
+    // with whole groups of 8 minpts values, a cell's or point's 8 levels are
+    // one aligned 8-byte word
+    const bool vec8 = (J & 7) == 0;
     const int w = threadIdx.x / kWave;
     {
         uint32_t i = blockIdx.x * kEligTile + threadIdx.x;
@@ -978,13 +986,20 @@ __global__ void __launch_bounds__(kBlock)
             uint32_t a0 = cell_start[A], a1 = cell_start[A + 1];
             uint32_t b0 = cell_start[B], b1 = cell_start[B + 1];
             bool small = (uint64_t)(a1 - a0) * (b1 - b0) <= kSmallPairWork;
+            // a cell's levels for all minpts values lie side by side
+            uint64_t wa = 0, wb = 0;
+            if (vec8) {
+                wa = *(const uint64_t *)(cellkc_cm + (size_t)A * J + j0);
+                wb = *(const uint64_t *)(cellkc_cm + (size_t)B * J + j0);
+            }
 #pragma unroll
             for (int g = 0; g < kEligGroup; g++) {
                 if (g < nj) {
-                    // a cell's levels for all minpts values lie side by side
                     int v = e0;
-                    int ca = cellkc_cm[(size_t)A * J + j0 + g];
-                    int cb = cellkc_cm[(size_t)B * J + j0 + g];
+                    int ca = vec8 ? (int)((wa >> (8 * g)) & 0xffu)
+                                  : cellkc_cm[(size_t)A * J + j0 + g];
+                    int cb = vec8 ? (int)((wb >> (8 * g)) & 0xffu)
+                                  : cellkc_cm[(size_t)B * J + j0 + g];
                     v = v > ca ? v : ca;
                     v = v > cb ? v : cb;
                     e[g] = v < K ? v : K;
@@ -998,16 +1013,20 @@ __global__ void __launch_bounds__(kBlock)
                 for (uint32_t p = a0; p < a1; p++) {
                     double px = xs[p], py = ys[p];
                     const uint8_t *kp = kc + (size_t)p * J + j0;
+                    uint64_t wp = vec8 ? *(const uint64_t *)kp : 0;
                     for (uint32_t q = b0; q < b1; q++) {
                         double sq = dist2_exact(px, py, xs[q], ys[q]);
                         int ls = 0; // first level whose eps admits the pair
-                        while (ls < K && sq > Tk[ls]) ls++;
+                        while (ls < K && sq > sTk[ls]) ls++;
                         if (ls >= K) continue;
                         const uint8_t *kq = kc + (size_t)q * J + j0;
+                        uint64_t wq = vec8 ? *(const uint64_t *)kq : 0;
 #pragma unroll
                         for (int g = 0; g < kEligGroup; g++) {
                             if (!((exact >> g) & 1u)) continue;
-                            int lv = ls, x = kp[g], y = kq[g];
+                            int lv = ls;
+                            int x = vec8 ? (int)((wp >> (8 * g)) & 0xffu) : kp[g];
+                            int y = vec8 ? (int)((wq >> (8 * g)) & 0xffu) : kq[g];
                             lv = lv > x ? lv : x;
                             lv = lv > y ? lv : y;
                             best[g] = lv < best[g] ? lv : best[g];
