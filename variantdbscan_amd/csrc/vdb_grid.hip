@@ -401,6 +401,47 @@ __global__ void k_row_ranges(const uint32_t *cell_key, uint32_t ncell, const uin
     }
 }
 
+// The same row ranges from a dense table over the key space: krank[k] = the
+// number of non-empty cells whose key is below k (an exclusive scan of the
+// occupied keys), so a row's run is [krank[k0], krank[k1 + 1]). A thread per
+// (cell, row), no search: for key spaces that are a small multiple of the
+// cell count.
+__global__ void k_key_flags(const uint32_t *cell_key, uint32_t ncell, uint32_t *kflag)
+{
+    uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c < ncell) kflag[cell_key[c]] = 1u;
+}
+
+__global__ void k_row_ranges_dense(const uint32_t *cell_key, uint32_t ncell, const uint32_t *sncx,
+                                   const uint32_t *sncy, const uint32_t *skbase, uint32_t nsets,
+                                   int R, const uint32_t *krank, uint32_t *row_lo,
+                                   uint32_t *row_hi, uint32_t *cell_ds)
+{
+    uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    uint32_t W = 2 * R + 1;
+    if (t >= (uint64_t)ncell * W) return;
+    uint32_t c = (uint32_t)(t / W);
+    int dr = (int)(t - (uint64_t)c * W) - R;
+    uint32_t key = cell_key[c];
+    uint32_t ds = seg_of(skbase, nsets, key);
+    if (dr == 0) cell_ds[c] = ds;
+    uint32_t ncx = sncx[ds], ncy = sncy[ds], kb = skbase[ds];
+    uint32_t local = key - kb;
+    int64_t cx = local / ncy, cy = local % ncy;
+    int64_t rx = cx + dr;
+    uint32_t lo = 0, hi = 0;
+    if (rx >= 0 && rx < (int64_t)ncx) {
+        int64_t y0 = cy - R < 0 ? 0 : cy - R;
+        int64_t y1 = cy + R > (int64_t)ncy - 1 ? (int64_t)ncy - 1 : cy + R;
+        uint64_t k0 = kb + (uint64_t)(rx * ncy + y0);
+        uint64_t k1 = kb + (uint64_t)(rx * ncy + y1);
+        lo = krank[k0];
+        hi = krank[k1 + 1];
+    }
+    row_lo[t] = lo;
+    row_hi[t] = hi;
+}
+
 std::string slot(const char *tag, const char *name) { return std::string(tag) + "." + name; }
 
 } // namespace
@@ -675,6 +716,24 @@ int build_grid(Context &c, const Points &P, double w_min, int R, const char *tag
     VDB_TRY(c.get(slot(tag, "row_lo").c_str(), ncell * W, &G.row_lo));
     VDB_TRY(c.get(slot(tag, "row_hi").c_str(), ncell * W, &G.row_hi));
     VDB_TRY(c.get(slot(tag, "cell_ds").c_str(), ncell, &G.cell_ds));
+    // a dense key table when the key space is at most 16 cells per key
+    // (and not in the allocation-free mode, whose scratch is small)
+    const bool dense = !c.in_arena() && ncell &&
+                       nkeys <= std::max<uint64_t>(16ull * ncell, (uint64_t)1 << 22);
+    if (dense) {
+        uint32_t *kflag, *krank;
+        VDB_TRY(c.get(slot(tag, "kflag").c_str(), nkeys + 1, &kflag));
+        VDB_TRY(c.get(slot(tag, "krank").c_str(), nkeys + 1, &krank));
+        VDB_HIP(hipMemsetAsync(kflag, 0, sizeof(uint32_t) * (nkeys + 1), c.stream()));
+        k_key_flags<<<ceil_div(ncell, kBlock), kBlock, 0, c.stream()>>>(G.cell_key, ncell, kflag);
+        VDB_HIP(hipGetLastError());
+        VDB_TRY(scan_exclusive_u32(c, kflag, krank, nkeys + 1, nullptr));
+        k_row_ranges_dense<<<ceil_div((uint64_t)ncell * W, kBlock), kBlock, 0, c.stream()>>>(
+            G.cell_key, ncell, G.sncx, G.sncy, G.skbase, ns, R, krank, G.row_lo, G.row_hi,
+            G.cell_ds);
+        VDB_HIP(hipGetLastError());
+        return kOk;
+    }
     k_row_ranges<<<ceil_div(ceil_div(ncell, kRowRun) * W, kBlock), kBlock, 0, c.stream()>>>(
         G.cell_key, ncell, G.sncx, G.sncy, G.skbase, ns, R, G.row_lo, G.row_hi, G.cell_ds);
     VDB_HIP(hipGetLastError());
