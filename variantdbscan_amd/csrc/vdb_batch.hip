@@ -194,21 +194,6 @@ __global__ void k_list_index(const uint32_t *flag, const uint32_t *excl, const d
     }
 }
 
-// Point range (start, count) of every window row of every cell: the list
-// kernel reads these with the rows' cell ranges in one round of loads.
-__global__ void k_row_points(const uint32_t *row_lo, const uint32_t *row_hi,
-                             const uint32_t *cell_start, size_t nrows, uint2 *row_pts)
-{
-    size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= nrows) return;
-    uint32_t lo = row_lo[t], hi = row_hi[t];
-    uint2 r = make_uint2(0u, 0u);
-    if (lo < hi) {
-        r.x = cell_start[lo];
-        r.y = cell_start[hi] - r.x;
-    }
-    row_pts[t] = r;
-}
 
 // ------------------------------------------------------ truncated table
 // One wave per listed point. Lane e < L holds entry e of the sorted list.
@@ -2159,6 +2144,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         return kErrUnsupported;
     }
     Grid G;
+    G.want_row_pts = true; // the truncated-list kernel reads the rows' point ranges
     bool enlarged = false;
     VDB_TRY(build_grid(c, P, d, R, "gb", G, &enlarged));
     if (enlarged) {
@@ -2215,16 +2201,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     k_list_index<<<gn, kBlock, 0, c.stream()>>>(flag, excl, G.xs, G.ys, G.cell_of_pos, n, list_xy,
                                                 list_cell, lst_idx);
     VDB_HIP(hipGetLastError());
-    // the window rows' point ranges
-    const size_t nwrows = (size_t)G.ncell * (2 * R + 1);
-    uint2 *row_pts;
-    VDB_TRY(c.get("b.row_pts", nwrows ? nwrows : 1, &row_pts));
-    if (nwrows) {
-        k_row_points<<<ceil_div(nwrows, kBlock), kBlock, 0, c.stream()>>>(G.row_lo, G.row_hi,
-                                                                         G.cell_start, nwrows,
-                                                                         row_pts);
-        VDB_HIP(hipGetLastError());
-    }
+    const uint2 *row_pts = G.row_pts; // the window rows' point ranges
     // ---- cell pairs within eps_max: counted on the library's second
     // stream beside the truncated lists (both read only the grid); the pair
     // total and the clique check come back in one read
@@ -2301,7 +2278,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
     tknn.stop();
 
     // ---- the pair count (and the clique check) read back
-    VDB_HIP(hipEventSynchronize(c.aux_event(1)));
+    VDB_TRY(c.wait_event(c.aux_event(1)));
     uint64_t npairs64 = 0;
     uint32_t hbad = 0;
     memcpy(&npairs64, hs, 8);
@@ -2787,7 +2764,7 @@ int cluster_batch(Context &c, const Points &P, const double *eps, const uint32_t
         VDB_HIP(hipMemcpyAsync(hb, bstart, sizeof(uint32_t) * ((size_t)J * (K + 1) + 1),
                                hipMemcpyDeviceToHost, c.stream()));
     }
-    VDB_HIP(hipStreamSynchronize(c.stream()));
+    VDB_TRY(c.wait(c.stream()));
     uint32_t htovf = 0, nlist = 0;
     memcpy(&htovf, hs + 32, 4);
     memcpy(&nlist, hs + 36, 4);

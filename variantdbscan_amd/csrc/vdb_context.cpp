@@ -1,5 +1,7 @@
 #include "vdb_context.h"
 
+#include <chrono>
+
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -37,6 +39,7 @@ Context::~Context()
     if (host_) hipHostFree(host_);
     for (hipEvent_t e : aux_ev_)
         if (e) hipEventDestroy(e);
+    if (wait_ev_) hipEventDestroy(wait_ev_);
     if (aux_) hipStreamDestroy(aux_);
     if (own_stream_) hipStreamDestroy(own_stream_);
 }
@@ -363,6 +366,30 @@ int Context::graph_run(uint64_t shape, const std::function<int()> &seq)
 int Context::sync()
 {
     VDB_HIP(hipStreamSynchronize(stream_));
+    return kOk;
+}
+
+int Context::wait(hipStream_t s)
+{
+    if (arena_) { // the allocation-free mode creates nothing
+        VDB_HIP(hipStreamSynchronize(s));
+        return kOk;
+    }
+    if (!wait_ev_) VDB_HIP(hipEventCreateWithFlags(&wait_ev_, hipEventDisableTiming));
+    VDB_HIP(hipEventRecord(wait_ev_, s));
+    return wait_event(wait_ev_);
+}
+
+int Context::wait_event(hipEvent_t e)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        hipError_t r = hipEventQuery(e);
+        if (r == hipSuccess) return kOk;
+        if (r != hipErrorNotReady) VDB_HIP(r);
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) break;
+    }
+    VDB_HIP(hipEventSynchronize(e));
     return kOk;
 }
 

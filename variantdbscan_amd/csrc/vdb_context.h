@@ -69,6 +69,12 @@ public:
     void *noalloc_scratch = nullptr; // this device's static scratch array
 
     int sync();
+    // Host waits of a call's read-backs: spin on an event instead of a
+    // blocking wait, which may sleep and wake tens of microseconds after the
+    // work is done (the GPU idles until the host launches again); after
+    // about 20 ms of spinning the wait blocks.
+    int wait(hipStream_t s);
+    int wait_event(hipEvent_t e);
 
     std::mutex mu; // serialises library calls on this device
 
@@ -117,6 +123,7 @@ private:
     hipStream_t stream_ = nullptr;
     hipStream_t aux_ = nullptr;
     hipEvent_t aux_ev_[kAuxEvents] = {};
+    hipEvent_t wait_ev_ = nullptr;
     std::unordered_map<std::string, Buffer> pool_;
     void *host_ = nullptr;
     size_t host_bytes_ = 0;

@@ -55,6 +55,10 @@ struct Grid {
     double *cell_box = nullptr;     // [ncell*4] x0,x1,y0,y1 of the cell's points
     uint32_t *row_lo = nullptr;     // [ncell*(2R+1)] first cell of window row
     uint32_t *row_hi = nullptr;     // [ncell*(2R+1)] one past the last
+    // with want_row_pts set before build_grid: (first point, point count)
+    // of every window row, beside row_lo/row_hi
+    bool want_row_pts = false;
+    uint2 *row_pts = nullptr;       // [ncell*(2R+1)]
     // after classify_rows (per-variant grids): inside every window row,
     // cells [row_f0, row_f1) are within eps of the whole centre cell (every
     // pair accepted without a test); row_lo/row_hi are then trimmed to cells

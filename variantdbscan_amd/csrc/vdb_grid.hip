@@ -10,15 +10,15 @@
 
 namespace vdb {
 
-double threshold_for_eps(double eps)
+// the largest t >= 0 with sqrt(t) <= eps, by bisection over the bits
+// (doubles >= 0 order like their bits)
+static double threshold_bisect(double eps)
 {
-    if (!(eps >= 0.0)) return -1.0;
-    if (std::isinf(eps)) return INFINITY;
     uint64_t lo = 0, hi = 0x7fefffffffffffffull;
     double dhi;
     memcpy(&dhi, &hi, 8);
     if (std::sqrt(dhi) <= eps) return dhi;
-    while (hi - lo > 1) { // sqrt(lo) <= eps < sqrt(hi); doubles >= 0 order like their bits
+    while (hi - lo > 1) { // sqrt(lo) <= eps < sqrt(hi)
         uint64_t mid = lo + (hi - lo) / 2;
         double dm;
         memcpy(&dm, &mid, 8);
@@ -27,6 +27,30 @@ double threshold_for_eps(double eps)
     double t;
     memcpy(&t, &lo, 8);
     return t;
+}
+
+double threshold_for_eps(double eps)
+{
+    if (!(eps >= 0.0)) return -1.0;
+    if (std::isinf(eps)) return INFINITY;
+    // sqrt is correctly rounded, hence monotone: the answer lies a few ulps
+    // from eps*eps, so step there (the bisection takes a case that would
+    // need more steps); the host calls this for every variant several times
+    // per call
+    double t = eps * eps;
+    if (t > DBL_MAX) t = DBL_MAX;
+    int steps = 0;
+    while (std::sqrt(t) > eps && steps < 64) {
+        t = std::nextafter(t, 0.0);
+        steps++;
+    }
+    while (steps < 64) {
+        double u = std::nextafter(t, INFINITY);
+        if (std::isinf(u) || std::sqrt(u) > eps) break;
+        t = u;
+        steps++;
+    }
+    return steps < 64 ? t : threshold_bisect(eps);
 }
 
 namespace {
@@ -63,7 +87,7 @@ __device__ __forceinline__ void ext_wave_reduce(double *m)
 
 __global__ void __launch_bounds__(kBlock)
     k_seg_extent_part(const double *x, const double *y, const uint32_t *seg,
-                      const uint32_t *bbase, uint32_t nsets, double *part, uint32_t *nonfinite)
+                      const uint32_t *bbase, uint32_t nsets, double *part, uint32_t *pbad)
 {
     uint32_t d = seg_of(bbase, nsets, blockIdx.x);
     uint32_t a = seg[d] + (blockIdx.x - bbase[d]) * kExtChunk;
@@ -93,21 +117,29 @@ __global__ void __launch_bounds__(kBlock)
             bad += sb[q];
         }
         for (int q = 0; q < 4; q++) part[4 * (size_t)blockIdx.x + q] = m[q];
-        if (bad) atomicAdd(nonfinite, bad);
+        pbad[blockIdx.x] = bad; // non-finite coordinates of the block
     }
 }
 
-__global__ void k_seg_extent_final(const double *part, const uint32_t *bbase, uint32_t nsets,
-                                   double *ext)
+// a wave per dataset: its extent, and its count of non-finite coordinates
+__global__ void k_seg_extent_final(const double *part, const uint32_t *pbad,
+                                   const uint32_t *bbase, uint32_t nsets, double *ext,
+                                   uint32_t *nonfinite)
 {
     uint32_t d = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     if (d >= nsets) return;
     double m[4] = {DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX};
-    for (uint32_t b = bbase[d] + lane_id(); b < bbase[d + 1]; b += kWave)
+    uint32_t bad = 0;
+    for (uint32_t b = bbase[d] + lane_id(); b < bbase[d + 1]; b += kWave) {
         ext_merge(m, part + 4 * (size_t)b);
+        bad += pbad[b];
+    }
     ext_wave_reduce(m);
-    if (lane_id() == 0)
+    for (int o = kWave / 2; o > 0; o >>= 1) bad += __shfl_xor(bad, o);
+    if (lane_id() == 0) {
         for (int q = 0; q < 4; q++) ext[4 * (size_t)d + q] = m[q];
+        nonfinite[d] = bad;
+    }
 }
 
 // Bin parameters of every dataset from its extent (thread per dataset), as
@@ -116,9 +148,10 @@ __global__ void k_seg_extent_final(const double *part, const uint32_t *bbase, ui
 // a positive min and |min| otherwise. A zero width (all coordinates equal
 // and positive divides 0 by 0 in the reference, which then aborts binning and
 // reads past its mapping) maps the whole dimension to bin 0.
-__global__ void k_bin_params(const double *ext, uint32_t nsets, BinParams *bps)
+__global__ void k_bin_params(const double *ext, uint32_t nsets, BinParams *bps, uint32_t *bad)
 {
     uint32_t d = blockIdx.x * kBlock + threadIdx.x;
+    if (d == 0) *bad = 0u; // the bin check of k_bin_keys, which runs next
     if (d >= nsets) return;
     const double *e = ext + 4 * (size_t)d;
     double min_x = e[0] < DBL_MAX ? e[0] : DBL_MAX;
@@ -191,9 +224,11 @@ __global__ void k_head_flags(const uint32_t *keys, uint32_t n, uint32_t *flags)
 }
 
 // excl = exclusive scan of head flags: cell index of pos i is excl[i]+flag[i]-1
+// kflag (may be null): the occupied keys are marked, for a dense key table
 __global__ void k_cells(const uint32_t *keys, const uint32_t *flags, const uint32_t *excl,
                         uint32_t n, uint32_t *cell_of_pos, uint32_t *cell_key,
-                        uint32_t *cell_start, const uint32_t *pos_rank, uint32_t *rank_pos)
+                        uint32_t *cell_start, const uint32_t *pos_rank, uint32_t *rank_pos,
+                        uint32_t *kflag)
 {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
@@ -203,6 +238,7 @@ __global__ void k_cells(const uint32_t *keys, const uint32_t *flags, const uint3
     if (flags[i]) {
         cell_key[c] = keys[i];
         cell_start[c] = i;
+        if (kflag) kflag[keys[i]] = 1u;
     }
 }
 
@@ -212,7 +248,7 @@ __global__ void k_cell_tail(uint32_t *cell_start, uint32_t ncell, uint32_t n)
 }
 
 // Bounding box of every cell's points. Thread per cell for the common small
-// cell (its points read serially); cells holding more than kBoxSerial points
+// cell (its points read 8 at a time); cells holding more than kBoxSerial points
 // are then reduced by the whole wave, one after another.
 constexpr uint32_t kBoxSerial = 16;
 constexpr int kBoxUnroll = 8;
@@ -228,12 +264,23 @@ __global__ void k_cell_box(const double *xs, const double *ys, const uint32_t *c
     bool big = b - a > kBoxSerial;
     if (c < ncell && !big) {
         double x0 = DBL_MAX, x1 = -DBL_MAX, y0 = DBL_MAX, y1 = -DBL_MAX;
-        for (uint32_t p = a; p < b; p++) {
-            double u = xs[p], v = ys[p];
-            x0 = u < x0 ? u : x0;
-            x1 = u > x1 ? u : x1;
-            y0 = v < y0 ? v : y0;
-            y1 = v > y1 ? v : y1;
+        // 8 points per round of loads, indices clamped into the cell (a
+        // cell is never empty), so the loads go out together
+        for (uint32_t p0 = a; p0 < b; p0 += kBoxUnroll) {
+            double u[kBoxUnroll], v[kBoxUnroll];
+#pragma unroll
+            for (int t = 0; t < kBoxUnroll; t++) {
+                uint32_t p = p0 + t < b ? p0 + t : b - 1;
+                u[t] = xs[p];
+                v[t] = ys[p];
+            }
+#pragma unroll
+            for (int t = 0; t < kBoxUnroll; t++) {
+                x0 = u[t] < x0 ? u[t] : x0;
+                x1 = u[t] > x1 ? u[t] : x1;
+                y0 = v[t] < y0 ? v[t] : y0;
+                y1 = v[t] > y1 ? v[t] : y1;
+            }
         }
         box[4 * (size_t)c + 0] = x0;
         box[4 * (size_t)c + 1] = x1;
@@ -403,19 +450,14 @@ __global__ void k_row_ranges(const uint32_t *cell_key, uint32_t ncell, const uin
 
 // The same row ranges from a dense table over the key space: krank[k] = the
 // number of non-empty cells whose key is below k (an exclusive scan of the
-// occupied keys), so a row's run is [krank[k0], krank[k1 + 1]). A thread per
-// (cell, row), no search: for key spaces that are a small multiple of the
-// cell count.
-__global__ void k_key_flags(const uint32_t *cell_key, uint32_t ncell, uint32_t *kflag)
-{
-    uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-    if (c < ncell) kflag[cell_key[c]] = 1u;
-}
-
+// occupied keys, which k_cells marks), so a row's run is [krank[k0],
+// krank[k1 + 1]). A thread per (cell, row), no search: for key spaces that
+// are a small multiple of the cell count. With row_pts, the run's points too.
 __global__ void k_row_ranges_dense(const uint32_t *cell_key, uint32_t ncell, const uint32_t *sncx,
                                    const uint32_t *sncy, const uint32_t *skbase, uint32_t nsets,
-                                   int R, const uint32_t *krank, uint32_t *row_lo,
-                                   uint32_t *row_hi, uint32_t *cell_ds)
+                                   int R, const uint32_t *krank, const uint32_t *cell_start,
+                                   uint32_t *row_lo, uint32_t *row_hi, uint32_t *cell_ds,
+                                   uint2 *row_pts)
 {
     uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     uint32_t W = 2 * R + 1;
@@ -440,6 +482,29 @@ __global__ void k_row_ranges_dense(const uint32_t *cell_key, uint32_t ncell, con
     }
     row_lo[t] = lo;
     row_hi[t] = hi;
+    if (row_pts) {
+        uint2 r = make_uint2(0u, 0u);
+        if (lo < hi) {
+            r.x = cell_start[lo];
+            r.y = cell_start[hi] - r.x;
+        }
+        row_pts[t] = r;
+    }
+}
+
+// Point range (start, count) of every window row, from the row's cells
+__global__ void k_row_points(const uint32_t *row_lo, const uint32_t *row_hi,
+                             const uint32_t *cell_start, size_t nrows, uint2 *row_pts)
+{
+    size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= nrows) return;
+    uint32_t lo = row_lo[t], hi = row_hi[t];
+    uint2 r = make_uint2(0u, 0u);
+    if (lo < hi) {
+        r.x = cell_start[lo];
+        r.y = cell_start[hi] - r.x;
+    }
+    row_pts[t] = r;
 }
 
 std::string slot(const char *tag, const char *name) { return std::string(tag) + "." + name; }
@@ -470,42 +535,45 @@ int build_rank_order(Context &c, Points &P, const char *tag)
                            c.stream()));
     P.seg = segb;
     uint32_t *bbase = segb + ns + 1;
-    double *ext = nullptr, *part = nullptr;
-    uint32_t *nonfin = nullptr;
-    VDB_TRY(c.get(slot(tag, "ext").c_str(), (size_t)ns * 4, &ext));
+    // what comes back to the host, in one region: the extents (4 doubles a
+    // dataset), the non-finite counts (one a dataset), the bin check
+    const size_t cbytes = 32 * (size_t)ns + 4 * (size_t)ns + 4;
+    unsigned char *check;
+    double *part = nullptr;
+    uint32_t *pbad = nullptr;
+    VDB_TRY(c.get(slot(tag, "check").c_str(), cbytes, &check));
+    double *ext = (double *)check;
+    uint32_t *nonfin = (uint32_t *)(check + 32 * (size_t)ns);
+    uint32_t *bad = nonfin + ns;
     VDB_TRY(c.get(slot(tag, "extpart").c_str(), (size_t)std::max(blk[ns], 1u) * 4, &part));
-    VDB_TRY(c.get(slot(tag, "nonfin").c_str(), 1, &nonfin));
-    VDB_HIP(hipMemsetAsync(nonfin, 0, sizeof(uint32_t), c.stream()));
+    VDB_TRY(c.get(slot(tag, "extbad").c_str(), (size_t)std::max(blk[ns], 1u), &pbad));
     if (blk[ns])
         k_seg_extent_part<<<blk[ns], kBlock, 0, c.stream()>>>(P.x, P.y, P.seg, bbase, ns, part,
-                                                             nonfin);
+                                                             pbad);
     VDB_HIP(hipGetLastError());
-    k_seg_extent_final<<<ceil_div(ns, kBlock / kWave), kBlock, 0, c.stream()>>>(part, bbase, ns,
-                                                                               ext);
+    k_seg_extent_final<<<ceil_div(ns, kBlock / kWave), kBlock, 0, c.stream()>>>(
+        part, pbad, bbase, ns, ext, nonfin);
     VDB_HIP(hipGetLastError());
     // bin parameters on the device: the extent is read back only once, with
-    // the non-finite count and the bin check, after the sort
+    // the non-finite counts and the bin check, after the sort
     BinParams *dbp;
     VDB_TRY(c.get(slot(tag, "binparams").c_str(), ns, &dbp));
-    k_bin_params<<<ceil_div(ns, kBlock), kBlock, 0, c.stream()>>>(ext, ns, dbp);
+    k_bin_params<<<ceil_div(ns, kBlock), kBlock, 0, c.stream()>>>(ext, ns, dbp, bad);
     VDB_HIP(hipGetLastError());
 
-    uint32_t *keys, *vals, *tk, *tv, *bad;
+    // the sort's own buffers: its result stays where the last pass left it
+    uint32_t *keys, *vals, *tk, *tv;
     VDB_TRY(c.get(slot(tag, "bk").c_str(), n, &keys));
     VDB_TRY(c.get(slot(tag, "bv").c_str(), n, &vals));
-    VDB_TRY(c.get("sort.tk", n, &tk));
-    VDB_TRY(c.get("sort.tv", n, &tv));
-    VDB_TRY(c.get(slot(tag, "bad").c_str(), 1, &bad));
-    VDB_HIP(hipMemsetAsync(bad, 0, sizeof(uint32_t), c.stream()));
+    VDB_TRY(c.get(slot(tag, "btk").c_str(), n, &tk));
+    VDB_TRY(c.get(slot(tag, "btv").c_str(), n, &tv));
     unsigned g = ceil_div(n, kBlock);
     k_bin_keys<<<g, kBlock, 0, c.stream()>>>(P.x, P.y, n, dbp, P.seg, ns, keys, vals, bad);
     VDB_HIP(hipGetLastError());
     uint32_t *sk, *sv;
     VDB_TRY(radix_sort_pairs(c, keys, vals, tk, tv, n, bits_for((uint64_t)ns * 90000), &sk,
                              &sv));
-    VDB_TRY(c.get(slot(tag, "mapping").c_str(), n, &P.mapping));
-    VDB_HIP(hipMemcpyAsync(P.mapping, sv, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice,
-                           c.stream()));
+    P.mapping = sv;
     VDB_TRY(c.get(slot(tag, "xb").c_str(), n, &P.xb));
     VDB_TRY(c.get(slot(tag, "yb").c_str(), n, &P.yb));
     k_gather_xy<<<g, kBlock, 0, c.stream()>>>(P.x, P.y, P.mapping, n, P.xb, P.yb);
@@ -514,14 +582,16 @@ int build_rank_order(Context &c, Points &P, const char *tag)
     // coordinate or an overflowing bin only made the keys meaningless: every
     // access above stays in bounds)
     char *h = nullptr;
-    VDB_TRY(c.host_scratch(32 * (size_t)ns + 8, (void **)&h));
-    VDB_HIP(hipMemcpyAsync(h, ext, sizeof(double) * 4 * ns, hipMemcpyDeviceToHost, c.stream()));
-    VDB_HIP(hipMemcpyAsync(h + 32 * (size_t)ns, nonfin, 4, hipMemcpyDeviceToHost, c.stream()));
-    VDB_HIP(hipMemcpyAsync(h + 32 * (size_t)ns + 4, bad, 4, hipMemcpyDeviceToHost, c.stream()));
-    VDB_HIP(hipStreamSynchronize(c.stream()));
+    VDB_TRY(c.host_scratch(cbytes, (void **)&h));
+    VDB_HIP(hipMemcpyAsync(h, check, cbytes, hipMemcpyDeviceToHost, c.stream()));
+    VDB_TRY(c.wait(c.stream()));
     uint32_t hnf = 0, hb = 0;
-    memcpy(&hnf, h + 32 * (size_t)ns, 4);
-    memcpy(&hb, h + 32 * (size_t)ns + 4, 4);
+    for (uint32_t d = 0; d < ns; d++) {
+        uint32_t v;
+        memcpy(&v, h + 32 * (size_t)ns + 4 * (size_t)d, 4);
+        hnf += v;
+    }
+    memcpy(&hb, h + 36 * (size_t)ns, 4);
     if (hnf) {
         set_error("non-finite coordinate in input");
         return kErrNonFinite;
@@ -667,24 +737,20 @@ int build_grid(Context &c, const Points &P, double w_min, int R, const char *tag
     G.sncy = (uint32_t *)(dp + 20 * (size_t)ns);
     G.skbase = (uint32_t *)(dp + 24 * (size_t)ns);
 
+    // the sort's own buffers: its result stays where the last pass left it
     uint32_t *keys, *vals, *tk, *tv, *flags, *excl;
     VDB_TRY(c.get(slot(tag, "gk").c_str(), n, &keys));
     VDB_TRY(c.get(slot(tag, "gv").c_str(), n, &vals));
-    VDB_TRY(c.get("sort.tk", n, &tk));
-    VDB_TRY(c.get("sort.tv", n, &tv));
+    VDB_TRY(c.get(slot(tag, "gtk").c_str(), n, &tk));
+    VDB_TRY(c.get(slot(tag, "gtv").c_str(), n, &tv));
     unsigned g = ceil_div(n, kBlock);
     k_cell_keys<<<g, kBlock, 0, c.stream()>>>(P.xb, P.yb, n, P.seg, ns, G.sgx0, G.sgy0, w,
                                               G.sncx, G.sncy, G.skbase, keys, vals);
     VDB_HIP(hipGetLastError());
     uint32_t *sk, *sv;
     VDB_TRY(radix_sort_pairs(c, keys, vals, tk, tv, n, bits_for(nkeys), &sk, &sv));
-    VDB_TRY(c.get(slot(tag, "pos_rank").c_str(), n, &G.pos_rank));
-    VDB_HIP(hipMemcpyAsync(G.pos_rank, sv, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice,
-                           c.stream()));
-    uint32_t *skeys;
-    VDB_TRY(c.get(slot(tag, "skeys").c_str(), n, &skeys));
-    VDB_HIP(hipMemcpyAsync(skeys, sk, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice,
-                           c.stream()));
+    G.pos_rank = sv;
+    uint32_t *skeys = sk;
     VDB_TRY(c.get(slot(tag, "xs").c_str(), n, &G.xs));
     VDB_TRY(c.get(slot(tag, "ys").c_str(), n, &G.ys));
     k_gather_xy<<<g, kBlock, 0, c.stream()>>>(P.xb, P.yb, G.pos_rank, n, G.xs, G.ys);
@@ -701,8 +767,18 @@ int build_grid(Context &c, const Points &P, double w_min, int R, const char *tag
     VDB_TRY(c.get(slot(tag, "rank_pos").c_str(), n, &G.rank_pos));
     VDB_TRY(c.get(slot(tag, "cell_key").c_str(), ncell, &G.cell_key));
     VDB_TRY(c.get(slot(tag, "cell_start").c_str(), (size_t)ncell + 1, &G.cell_start));
+    // window rows from a dense key table when the key space is at most 16
+    // keys per cell (not in the allocation-free mode, whose scratch is
+    // small); k_cells marks the occupied keys
+    const bool dense = !c.in_arena() && ncell &&
+                       nkeys <= std::max<uint64_t>(16ull * ncell, (uint64_t)1 << 22);
+    uint32_t *kflag = nullptr;
+    if (dense) {
+        VDB_TRY(c.get(slot(tag, "kflag").c_str(), nkeys + 1, &kflag));
+        VDB_HIP(hipMemsetAsync(kflag, 0, sizeof(uint32_t) * (nkeys + 1), c.stream()));
+    }
     k_cells<<<g, kBlock, 0, c.stream()>>>(skeys, flags, excl, n, G.cell_of_pos, G.cell_key,
-                                          G.cell_start, G.pos_rank, G.rank_pos);
+                                          G.cell_start, G.pos_rank, G.rank_pos, kflag);
     VDB_HIP(hipGetLastError());
     k_cell_tail<<<1, 64, 0, c.stream()>>>(G.cell_start, ncell, n);
     VDB_HIP(hipGetLastError());
@@ -716,27 +792,27 @@ int build_grid(Context &c, const Points &P, double w_min, int R, const char *tag
     VDB_TRY(c.get(slot(tag, "row_lo").c_str(), ncell * W, &G.row_lo));
     VDB_TRY(c.get(slot(tag, "row_hi").c_str(), ncell * W, &G.row_hi));
     VDB_TRY(c.get(slot(tag, "cell_ds").c_str(), ncell, &G.cell_ds));
-    // a dense key table when the key space is at most 16 cells per key
-    // (and not in the allocation-free mode, whose scratch is small)
-    const bool dense = !c.in_arena() && ncell &&
-                       nkeys <= std::max<uint64_t>(16ull * ncell, (uint64_t)1 << 22);
+    G.row_pts = nullptr;
+    if (G.want_row_pts)
+        VDB_TRY(c.get(slot(tag, "row_pts").c_str(), std::max<size_t>(ncell * W, 1), &G.row_pts));
     if (dense) {
-        uint32_t *kflag, *krank;
-        VDB_TRY(c.get(slot(tag, "kflag").c_str(), nkeys + 1, &kflag));
+        uint32_t *krank;
         VDB_TRY(c.get(slot(tag, "krank").c_str(), nkeys + 1, &krank));
-        VDB_HIP(hipMemsetAsync(kflag, 0, sizeof(uint32_t) * (nkeys + 1), c.stream()));
-        k_key_flags<<<ceil_div(ncell, kBlock), kBlock, 0, c.stream()>>>(G.cell_key, ncell, kflag);
-        VDB_HIP(hipGetLastError());
         VDB_TRY(scan_exclusive_u32(c, kflag, krank, nkeys + 1, nullptr));
         k_row_ranges_dense<<<ceil_div((uint64_t)ncell * W, kBlock), kBlock, 0, c.stream()>>>(
-            G.cell_key, ncell, G.sncx, G.sncy, G.skbase, ns, R, krank, G.row_lo, G.row_hi,
-            G.cell_ds);
+            G.cell_key, ncell, G.sncx, G.sncy, G.skbase, ns, R, krank, G.cell_start, G.row_lo,
+            G.row_hi, G.cell_ds, G.row_pts);
         VDB_HIP(hipGetLastError());
         return kOk;
     }
     k_row_ranges<<<ceil_div(ceil_div(ncell, kRowRun) * W, kBlock), kBlock, 0, c.stream()>>>(
         G.cell_key, ncell, G.sncx, G.sncy, G.skbase, ns, R, G.row_lo, G.row_hi, G.cell_ds);
     VDB_HIP(hipGetLastError());
+    if (G.row_pts && ncell) {
+        k_row_points<<<ceil_div(ncell * W, kBlock), kBlock, 0, c.stream()>>>(
+            G.row_lo, G.row_hi, G.cell_start, ncell * W, G.row_pts);
+        VDB_HIP(hipGetLastError());
+    }
     return kOk;
 }
 

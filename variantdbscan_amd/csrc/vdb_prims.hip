@@ -329,7 +329,7 @@ int compact_count(Context &c, const uint32_t *flags, uint32_t *out, size_t n,
     void *h = nullptr;
     VDB_TRY(c.host_scratch(sizeof(uint32_t), &h));
     VDB_HIP(hipMemcpyAsync(h, tot, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream()));
-    VDB_HIP(hipStreamSynchronize(c.stream()));
+    VDB_TRY(c.wait(c.stream()));
     *total_host = *(uint32_t *)h;
     return kOk;
 }
