@@ -242,18 +242,21 @@ __global__ void k_cells(const uint32_t *keys, const uint32_t *flags, const uint3
     }
 }
 
-__global__ void k_cell_tail(uint32_t *cell_start, uint32_t ncell, uint32_t n)
+__global__ void k_cell_tail(uint32_t *cell_start, uint32_t ncell, uint32_t n, uint32_t *nbig)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) cell_start[ncell] = n;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        cell_start[ncell] = n;
+        *nbig = 0; // the box pass's list of large cells
+    }
 }
 
 // Bounding box of every cell's points. Thread per cell for the common small
-// cell (its points read 8 at a time); cells holding more than kBoxSerial points
-// are then reduced by the whole wave, one after another.
+// cell (its points read 8 at a time); cells holding more than kBoxSerial
+// points go to a list that k_cell_box_big reduces, a wave per cell.
 constexpr uint32_t kBoxSerial = 16;
 constexpr int kBoxUnroll = 8;
 __global__ void k_cell_box(const double *xs, const double *ys, const uint32_t *cell_start,
-                           uint32_t ncell, double *box)
+                           uint32_t ncell, double *box, uint32_t *big, uint32_t *nbig)
 {
     uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     uint32_t a = 0, b = 0;
@@ -261,8 +264,8 @@ __global__ void k_cell_box(const double *xs, const double *ys, const uint32_t *c
         a = cell_start[c];
         b = cell_start[c + 1];
     }
-    bool big = b - a > kBoxSerial;
-    if (c < ncell && !big) {
+    bool large = c < ncell && b - a > kBoxSerial;
+    if (c < ncell && !large) {
         double x0 = DBL_MAX, x1 = -DBL_MAX, y0 = DBL_MAX, y1 = -DBL_MAX;
         // 8 points per round of loads, indices clamped into the cell (a
         // cell is never empty), so the loads go out together
@@ -287,12 +290,25 @@ __global__ void k_cell_box(const double *xs, const double *ys, const uint32_t *c
         box[4 * (size_t)c + 2] = y0;
         box[4 * (size_t)c + 3] = y1;
     }
-    uint64_t mb = ballot(big);
-    while (mb) {
-        int bit = __builtin_ctzll(mb);
-        mb &= mb - 1;
-        uint32_t cc = c - (uint32_t)lane_id() + (uint32_t)bit;
-        uint32_t ca = __shfl(a, bit), cb = __shfl(b, bit);
+    uint64_t m = ballot(large);
+    if (m) {
+        int leader = __builtin_ctzll(m);
+        uint32_t base = 0;
+        if (lane_id() == leader) base = atomicAdd(nbig, (uint32_t)popc64(m));
+        base = __shfl(base, leader);
+        if (large) big[base + mask_rank(m)] = c;
+    }
+}
+
+// the listed large cells, a wave per cell over a grid sized for the GPU
+__global__ void k_cell_box_big(const double *xs, const double *ys, const uint32_t *cell_start,
+                               const uint32_t *big, const uint32_t *nbig, double *box)
+{
+    const uint32_t nb = *nbig;
+    for (uint32_t w = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; w < nb;
+         w += gridDim.x * (kBlock / kWave)) {
+        uint32_t cc = big[w];
+        uint32_t ca = cell_start[cc], cb = cell_start[cc + 1];
         double x0 = DBL_MAX, x1 = -DBL_MAX, y0 = DBL_MAX, y1 = -DBL_MAX;
         // kBoxUnroll strides per step, all loads issued before the first
         // min/max: a dense cell (thousands of points) is a long walk
@@ -780,13 +796,21 @@ int build_grid(Context &c, const Points &P, double w_min, int R, const char *tag
     k_cells<<<g, kBlock, 0, c.stream()>>>(skeys, flags, excl, n, G.cell_of_pos, G.cell_key,
                                           G.cell_start, G.pos_rank, G.rank_pos, kflag);
     VDB_HIP(hipGetLastError());
-    k_cell_tail<<<1, 64, 0, c.stream()>>>(G.cell_start, ncell, n);
+    uint32_t *bigc, *nbig;
+    VDB_TRY(c.get(slot(tag, "bigcells").c_str(), std::max<uint32_t>(ncell, 1), &bigc));
+    VDB_TRY(c.get(slot(tag, "nbig").c_str(), 1, &nbig));
+    k_cell_tail<<<1, 64, 0, c.stream()>>>(G.cell_start, ncell, n, nbig);
     VDB_HIP(hipGetLastError());
 
     VDB_TRY(c.get(slot(tag, "cell_box").c_str(), (size_t)ncell * 4, &G.cell_box));
-    k_cell_box<<<ceil_div(ncell, kBlock), kBlock, 0, c.stream()>>>(
-        G.xs, G.ys, G.cell_start, ncell, G.cell_box);
-    VDB_HIP(hipGetLastError());
+    if (ncell) {
+        k_cell_box<<<ceil_div(ncell, kBlock), kBlock, 0, c.stream()>>>(
+            G.xs, G.ys, G.cell_start, ncell, G.cell_box, bigc, nbig);
+        VDB_HIP(hipGetLastError());
+        k_cell_box_big<<<std::min(ceil_div(ncell, kBlock / kWave), 1024u), kBlock, 0,
+                         c.stream()>>>(G.xs, G.ys, G.cell_start, bigc, nbig, G.cell_box);
+        VDB_HIP(hipGetLastError());
+    }
 
     size_t W = 2 * (size_t)R + 1;
     VDB_TRY(c.get(slot(tag, "row_lo").c_str(), ncell * W, &G.row_lo));
