@@ -85,7 +85,7 @@ typedef struct vdb_options {
     int precision;              /* VDB_PREC_* */
     uint64_t table_budget_bytes;/* 0 = most of the free device memory */
     int verbose;
-    int cells_per_eps;          /* per-variant grid: cells per eps (1..4), 0 = library's choice */
+    int cells_per_eps;          /* per-variant grid: cells per eps (1..8), 0 = library's choice */
     int border;                 /* VDB_BORDER_* */
 } vdb_options;
 

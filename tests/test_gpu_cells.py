@@ -16,7 +16,7 @@ from variantdbscan_amd import datasets
 
 pytestmark = pytest.mark.gpu
 
-CELLS = [1, 2, 3, 4]
+CELLS = [1, 2, 3, 4, 6, 8]
 PATHS = [vd.PATH_FUSED, vd.PATH_TABLE]
 PRECS = [vd.PREC_FP64, vd.PREC_FP32_RECHECK]
 

@@ -143,7 +143,7 @@ def cluster(x, y, eps, minpts, path=PATH_AUTO, precision=PREC_FP64, verbose=Fals
             table_budget_bytes=0, return_stats=False, cells_per_eps=0, border=BORDER_REFERENCE,
             reuse=False):
     """Labels (V, n) for host arrays; retArr layout of libVDBSCAN.
-    cells_per_eps (1..4) sets the per-variant grid's cells per eps; 0 lets
+    cells_per_eps (1..8) sets the per-variant grid's cells per eps; 0 lets
     the library choose. border=BORDER_REFERENCE gives the reference's uint32
     labels (0 = noise, clusters 1..); BORDER_STANDARD gives int32 labels with
     textbook DBSCAN borders (-1 = noise, clusters 0.. by smallest core index,

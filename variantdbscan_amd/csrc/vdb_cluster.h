@@ -42,7 +42,7 @@ int build_eps_grid(Context &c, const Points &P, double eps, int k, const char *t
 
 // Cells per eps for the per-variant kernels: the option's value when set
 // (1..kMaxCellsPerEps), else 0 (let tune_cells_per_eps choose).
-constexpr int kMaxCellsPerEps = 4;
+constexpr int kMaxCellsPerEps = 8;
 int cells_per_eps(const vdb_options *opt);
 
 // Autotuner: the first time a workload (point count, eps, extent, precision)

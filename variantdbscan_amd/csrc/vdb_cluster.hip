@@ -1060,7 +1060,12 @@ int tune_cells_per_eps(Context &c, const Points &P, double eps, int precision, i
     unsigned long long *rc;
     VDB_TRY(c.get("tune.counts", P.n, &counts));
     VDB_TRY(c.get("tune.recheck", 1, &rc));
-    int best = kDefaultCellsPerEps;
+    // Finer cells test fewer candidates (more of the window is accepted whole)
+    // but add rows per window; past the point where the count stops gaining,
+    // the fill and the core pass run a little slower on the finer grid. The
+    // choice is the coarsest k whose count is within kTuneSlack of the best.
+    constexpr float kTuneSlack = 1.05f;
+    float ms_k[kMaxCellsPerEps + 1] = {};
     float best_ms = 0;
     for (int k = 1; k <= kMaxCellsPerEps; k++) {
         Grid G;
@@ -1070,12 +1075,15 @@ int tune_cells_per_eps(Context &c, const Points &P, double eps, int precision, i
         VDB_TRY(build_eps_grid(c, P, eps, k, "g1", G));
         VDB_TRY(build_items(c, G, "g1", I));
         VDB_TRY(count_neighbors(c, G, I, eps, precision, counts, (uint64_t *)rc));
-        float ms = tm.stop();
-        if (k == 1 || ms < best_ms) {
-            best = k;
-            best_ms = ms;
-        }
+        ms_k[k] = tm.stop();
+        if (k == 1 || ms_k[k] < best_ms) best_ms = ms_k[k];
     }
+    int best = kMaxCellsPerEps;
+    for (int k = 1; k <= kMaxCellsPerEps; k++)
+        if (ms_k[k] <= best_ms * kTuneSlack) {
+            best = k;
+            break;
+        }
     c.tuned_put(key, best);
     *k_out = best;
     return kOk;

@@ -129,7 +129,7 @@ extern "C" int vdb_cluster_noalloc(const double *x, const double *y, uint32_t n,
                 opt->border < VDB_BORDER_REFERENCE || opt->border > VDB_BORDER_STANDARD ||
                 opt->cells_per_eps < 0 || opt->cells_per_eps > kMaxCellsPerEps)) {
         set_error("vdb_cluster_noalloc runs the fused per-variant engine: path AUTO or FUSED, "
-                  "known precision and border options, cells_per_eps 0..4");
+                  "known precision and border options, cells_per_eps 0..8");
         return kErrInvalid;
     }
     Context *cp = nullptr;
