@@ -350,8 +350,11 @@ __global__ void __launch_bounds__(kBlock)
         uint32_t end = upto < rlen[b] ? upto : rlen[b];
         uint32_t hi = end > fl[b] ? lo + (end - fl[b]) : lo;
         uint32_t l = (uint32_t)lane_id();
+        // the table is far larger than the caches and is not read back by
+        // this kernel: streaming (non-temporal) stores
         if (l >= lo && l < hi)
-            ids[(at - lo) + l] = ring[b * kRing + ((fl[b] - lo + l) & (kRing - 1))];
+            __builtin_nontemporal_store(ring[b * kRing + ((fl[b] - lo + l) & (kRing - 1))],
+                                        &ids[(at - lo) + l]);
         fl[b] = upto;
         bnd[b] = upto + kWave;
     };
@@ -433,7 +436,9 @@ __global__ void __launch_bounds__(kBlock)
                     uint32_t *dst = ids + (run[b] + fl[b]);
                     for (uint32_t s = 0; s < nseg; s++) {
                         uint32_t t = fl[b] + s * kWave + (uint32_t)lane_id();
-                        if (t < rlen[b]) dst[s * kWave + lane_id()] = q + s * kWave + lane_id();
+                        if (t < rlen[b])
+                            __builtin_nontemporal_store(q + s * kWave + (uint32_t)lane_id(),
+                                                        &dst[s * kWave + lane_id()]);
                     }
                     fl[b] += nseg * kWave;
                     cur[b] = fl[b];
