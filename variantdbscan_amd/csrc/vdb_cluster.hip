@@ -1053,8 +1053,11 @@ int tune_cells_per_eps(Context &c, const Points &P, double eps, int precision, i
         memcpy(&u, &v, sizeof u);
         return (unsigned long long)u;
     };
+    // the key names the tuned range (1..kMaxCellsPerEps): a choice cached
+    // over a narrower range is tuned again
     char key[160];
-    snprintf(key, sizeof key, "cells:%u:%d:%llx:%llx:%llx:%llx:%llx", P.n, precision, bits(eps),
+    snprintf(key, sizeof key, "cells%d:%u:%d:%llx:%llx:%llx:%llx:%llx", kMaxCellsPerEps, P.n,
+             precision, bits(eps),
              bits(P.ext[0]), bits(P.ext[1]), bits(P.ext[2]), bits(P.ext[3]));
     int cached = 0;
     if (c.tuned_get(key, &cached) && cached >= 1 && cached <= kMaxCellsPerEps) {
