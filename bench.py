@@ -277,7 +277,8 @@ def main():
     ap.add_argument("--only", choices=["batch", "table"], default=None)
     ap.add_argument("--no-standard", action="store_true",
                     help="skip the standard-border-rule timing of the batch leg")
-    ap.add_argument("--table-steps", type=int, default=2)
+    ap.add_argument("--table-steps", type=int, default=4,
+                    help="table-path calls; the first (allocation, autotuning) is untimed")
     ap.add_argument("--kernel-steps", type=int, default=3,
                     help="steps per kernel of the headline roofline's per-kernel timing")
     ap.add_argument("--calibrate-copy", action="store_true",
