@@ -1471,6 +1471,71 @@ __device__ void test_pair(const LevelArgs &a, const PairRec &pr, uint32_t *paren
     }
 }
 
+// Two tests in one wave, one per 32-lane half (lanes 32h..32h+31 take r_h),
+// for pairs whose cells both hold at most 32 points: each half holds its B
+// cell in registers and screens its A cell in one step, so two dependent
+// load chains are in flight per wave. A half whose pair has a larger cell
+// tests nothing; its bit (1 for r0, 2 for r1) in the wave-uniform result
+// asks the caller for a whole-wave test. Absent records have a == NONE.
+__device__ uint32_t test_pair_halves(const LevelArgs &a, const PairRec &r0, const PairRec &r1,
+                                     uint32_t *parent, PairRec *list_out, uint32_t *nout)
+{
+    const int lane = lane_id();
+    const int h = lane >> 5;
+    const uint32_t l = (uint32_t)lane & 31u;
+    const PairRec pr = h ? r1 : r0;
+    const bool live = pr.a != kNone;
+    uint32_t A = live ? pr.a : 0u, B = live ? pr.b : 0u;
+    uint32_t a0 = a.cell_start[A], a1 = a.cell_start[A + 1];
+    uint32_t b0 = a.cell_start[B], b1 = a.cell_start[B + 1];
+    const bool fits = live && a1 - a0 <= 32u && b1 - b0 <= 32u;
+    const uint64_t lm = ballot(live), fm = ballot(fits);
+    uint32_t back = 0;
+    if ((lm & 0xffffffffull) && !(fm & 0xffffffffull)) back |= 1u;
+    if ((lm >> 32) && !(fm >> 32)) back |= 2u;
+    if (fits) {
+        uint32_t pa = ld_u32(parent + A), pb = ld_u32(parent + B);
+        bool found = pa == pb || cf_find(parent, pa) == cf_find(parent, pb);
+        if (!found) {
+            if (a1 - a0 > b1 - b0) { // lanes hold the larger cell
+                uint32_t t = A;
+                A = B;
+                B = t;
+                t = a0;
+                a0 = b0;
+                b0 = t;
+                t = a1;
+                a1 = b1;
+                b1 = t;
+            }
+            const uint32_t na = a1 - a0, nb = b1 - b0; // 1..32 each
+            const double *bB = a.box + 4 * (size_t)B;
+            double bx0 = bB[0], bx1 = bB[1], by0 = bB[2], by1 = bB[3];
+            uint32_t qi = b0 + min(l, nb - 1);
+            int kq = a.kc[(size_t)qi * a.J + a.j];
+            double qx = a.xs[qi], qy = a.ys[qi];
+            uint32_t pi = a0 + min(l, na - 1);
+            int kp = a.kc[(size_t)pi * a.J + a.j];
+            double px = a.xs[pi], py = a.ys[pi];
+            // lane l holds point b0+l of B (NaN when absent or not core)
+            if (!(l < nb && kq <= a.k)) qx = qy = __longlong_as_double(0x7ff8000000000000ll);
+            bool cand = l < na && kp <= a.k && box_dist2_min(px, py, bx0, bx1, by0, by1) <= a.T;
+            uint32_t cm = (uint32_t)(ballot(cand) >> (32 * h));
+            while (cm && !found) {
+                int bit = __builtin_ctz(cm);
+                cm &= cm - 1;
+                double cx = __shfl(px, 32 * h + bit), cy = __shfl(py, 32 * h + bit);
+                found = (uint32_t)(ballot(dist2_exact(cx, cy, qx, qy) <= a.T) >> (32 * h)) != 0u;
+            }
+            if (l == 0) {
+                if (found) cf_unite(parent, A, B);
+                else list_out[atomicAdd(nout, 1u)] = pr;
+            }
+        }
+    }
+    return back;
+}
+
 // the near tests, then the far ones, in one grid-stride pass (waves take
 // them in order, so most far tests start after the near ones)
 __global__ void __launch_bounds__(kBlock) k_test(LevelArgs a, LevelLists L)
@@ -1509,12 +1574,25 @@ __global__ void __launch_bounds__(kBlock) k_test(LevelArgs a, LevelLists L)
         PairRec r{kNone, kNone, 0};
         if (w < nt) r = w < nn ? near[w] : far[w - nn];
         uint64_t real = ballot(r.a != kNone);
+        // two records at a time, a half-wave each when their cells are small
+        // (a record of a later pair in the same wave may then be tested
+        // although an earlier one connected its cells: redundant work only,
+        // since pairs kept for the next level are settled there)
         while (real) {
-            int bit = __builtin_ctzll(real);
+            int bit0 = __builtin_ctzll(real);
             real &= real - 1;
-            PairRec pr{(uint32_t)__shfl((int)r.a, bit), (uint32_t)__shfl((int)r.b, bit),
-                       (uint32_t)__shfl((int)r.k, bit)};
-            test_pair(a, pr, parent, list_out, nout);
+            PairRec p0{(uint32_t)__shfl((int)r.a, bit0), (uint32_t)__shfl((int)r.b, bit0),
+                       (uint32_t)__shfl((int)r.k, bit0)};
+            PairRec p1{kNone, kNone, 0};
+            if (real) {
+                int bit1 = __builtin_ctzll(real);
+                real &= real - 1;
+                p1 = PairRec{(uint32_t)__shfl((int)r.a, bit1), (uint32_t)__shfl((int)r.b, bit1),
+                             (uint32_t)__shfl((int)r.k, bit1)};
+            }
+            uint32_t back = test_pair_halves(a, p0, p1, parent, list_out, nout);
+            if (back & 1u) test_pair(a, p0, parent, list_out, nout);
+            if (back & 2u) test_pair(a, p1, parent, list_out, nout);
         }
     }
 }
